@@ -1,0 +1,179 @@
+"""Benchmark: RK4 sweeps/s of the density-matrix workload on one MI355X.
+
+Workload (the reference's GPU density-matrix loop, rydberg_density_matrix_gpu.jl:211-222):
+three-level Rydberg ladder, vectorised Lindbladian (d = 9), N = 100 atoms at
+seeded uniform positions in [0, 100], C6/r^6 fitted by 4 exponentials
+(MPO bond w = 14), D_max = 128, dt = 0.005, starting from the ground-state
+product MPS (mpsgroundstate).  One step = one RK4stp_apply_H_half! call
+(10-argument form, result overwrites rho).
+
+Prints one JSON line.  ``value`` is FP64 RK4 sweeps/s with all state
+resident in HBM.  ``roofline`` comes from a second pass over the same steps
+in which every GEMM launch is bracketed by HIP events on the library's
+stream (dominant kernel: the complex FP64 MFMA GEMM).  ``cpu_baseline`` is
+the numpy/OpenBLAS oracle timed on this host over one full RK4 step of the
+same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "matrixproductstates.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "RK4 sweeps/s at N=100,D=128 (FP64 vs FP32+refine); p50/p99 trajectory latency"
+PEAK = {"fp64": 78.6, "fp32": 157.3}   # TFLOP/s, MI355X dense MFMA
+
+
+def model_flops(dims, d, w):
+    """Algorithmic GEMM flops of one RK4 half step (8 flops / complex MAC), from the
+    reference's contraction shapes: per site 20 w d Dl Dr D + 8 w^2 d^2 Dl Dr + 25 d D^3-type terms."""
+    tot = 0.0
+    N = len(dims) - 1
+    for j in range(N):
+        Dl, Dr = dims[j], dims[j + 1]
+        wl = 1 if j == 0 else w
+        wr = 1 if j == N - 1 else w
+        wm = max(wl, wr)
+        # 5 op-term GEMMs of (Dl d) x (wm D) x D per stage x 4 stages  ~ 20 w d D^3
+        op = 20.0 * wm * d * Dl * Dr * max(Dl, Dr)
+        mpo = 8.0 * wl * wr * d * d * Dl * Dr
+        ovl = 25.0 * d * Dl * Dr * max(Dl, Dr)
+        tot += op + mpo + ovl
+    return 8.0 * tot
+
+
+def setup(N, D, n_exp, dt, seed, ctx):
+    import mpsamd as M
+    from mpsamd import models
+    p = models.RydbergParams(n_exp=n_exp)
+    z = models.positions(N, float(N), seed)
+    L = [M.MPO(h, ctx) for h in models.rydberg_rk4_mpos(p, z, dt)]
+    rho, dims = M.mpsgroundstate(N, D, 9, ctx)
+    rhos = [rho.similar() for _ in range(3)]
+    for r in rhos:
+        r.copy_from(rho)
+    envs = [M.build_env(dims, dims, ctx=ctx) for _ in range(3)]
+    envop = M.build_env(dims, dims, [1] + [p.w] * (N - 1) + [1], ctx=ctx)
+    return p, z, L, rho, rhos, envs, envop, dims
+
+
+def cpu_baseline(N, D, n_exp, dt, seed, steps_cpu=1):
+    from oracle import mps_oracle as O
+    from mpsamd import models
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count()
+    p = models.RydbergParams(n_exp=n_exp)
+    z = models.positions(N, float(N), seed)
+    L = models.rydberg_rk4_mpos(p, z, dt)
+    rho, dims = O.mpsgroundstate(N, D, 9)
+    wop = [1] + [p.w] * (N - 1) + [1]
+    envop = O.build_env_op(dims, dims, wop)
+    e = [O.build_env(dims, dims) for _ in range(3)]
+    r1, r2, r3 = ([a.copy() for a in rho] for _ in range(3))
+    t0 = time.perf_counter()
+    for _ in range(steps_cpu):
+        O.RK4stp_apply_H_half_(dt, L, rho, r1, r2, r3, *e, envop)
+    t = time.perf_counter() - t0
+    return {"value": steps_cpu / t, "unit": "sweeps/s", "cores": int(threads), "kind": "port",
+            "sample": f"{steps_cpu} full RK4stp_apply_H_half! step(s) of the same N={N} workload "
+                      f"(numpy/OpenBLAS zgemm + LAPACK zgeqrf/zungqr), {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=100)
+    ap.add_argument("--D", type=int, default=128)
+    ap.add_argument("--n-exp", type=int, default=4)
+    ap.add_argument("--dt", type=float, default=0.005)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
+                                   "(trajectories/requests would shard across GPUs with no collectives)"}))
+        sys.exit(2)
+
+    import mpsamd as M
+    ctx = M.Context(0)
+    p, z, L, rho, rhos, envs, envop, dims = setup(args.N, args.D, args.n_exp, args.dt, args.seed, ctx)
+    dt = args.dt
+
+    def step():
+        M.RK4stp_apply_H_half_(dt, L, rho, rhos[0], rhos[1], rhos[2], *envs, envop)
+
+    for i in range(args.warmup):
+        step()
+        print(f"warmup step {i} done", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    sweeps = args.steps / elapsed
+    print(f"timed {args.steps} steps: {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    tr = M.scal_prod_no_conj(M.MPS.from_host(__import__("mpsamd").models.identity_vec_mps(args.N), ctx), rho)
+
+    roof = None
+    prof = None
+    if not args.no_profile:
+        ctx.set_profiling(True)
+        for i in range(args.steps):
+            step()
+        st = ctx.kernel_stats()
+        ctx.set_profiling(False)
+        achieved = st["gemm_flops"] / (st["gemm_ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                "kernel": "zgemm_f64_kernel (+ split-K reduce)",
+                "launches": st["gemm_launches"], "avg_launch_us": round(st["gemm_ms"] * 1e3 / max(st["gemm_launches"], 1), 2)}
+        prof = {"gemm_ms_per_step": st["gemm_ms"] / args.steps, "qr_ms_per_step": st["qr_ms"] / args.steps,
+                "qr_launches_per_step": st["qr_launches"] / args.steps,
+                "gemm_gflop_per_step": st["gemm_flops"] / args.steps / 1e9}
+        print(f"profile: {prof}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.N, args.D, args.n_exp, args.dt, args.seed)
+
+    out = {
+        "metric": METRIC,
+        "value": round(sweeps, 4),
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Rydberg ladder density matrix, vectorised Lindbladian, N={args.N}, d=9, "
+                               f"w={p.w} ({args.n_exp} exponentials), D_max={args.D}, dt={dt}, "
+                               f"RK4stp_apply_H_half! from mpsgroundstate",
+                   "seed": args.seed, "model_gflop_per_step": round(model_flops(dims, 9, p.w) / 1e9, 1),
+                   "trace_rho": [round(tr.real, 12), round(tr.imag, 12)]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "profile": prof,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,221 @@
+/*
+ * mpship.h -- C ABI of the MI355X (gfx950) MPS time-evolution hot path.
+ *
+ * Drop-in replacement for the tensor-network hot path of
+ * jdnz/MatrixProductStates.jl: environment contractions (src/mps_base.jl),
+ * one-site variational compression sweeps (src/compression.jl), RK4 and
+ * Arnoldi steps (src/timeevolution.jl) and QR/SVD gauge moves
+ * (src/orthonormalisation.jl, compressionSVD! in src/compression.jl).
+ *
+ * Conventions (identical to the reference, which is column-major Julia):
+ *   - complex numbers are interleaved (re, im) doubles (Julia ComplexF64);
+ *   - MPS site A[Dl, d, Dr], MPO site O[wl, d_out, wr, d_in], column-major;
+ *   - operator environments E[Dket, w, Dbra], LEFT ones stored conjugated
+ *     (mps_base.jl:219-241); overlap environments L[Dbra, Dket], R[Dket, Dbra];
+ *   - sites and environment slots are 0-based: site j (Julia jj = j+1) has its
+ *     left environment in slot j and its right environment in slot j+1;
+ *   - every function returns 0 on success, a negative MPS_E_* code otherwise;
+ *     mps_last_error() gives a message (thread-local).
+ *
+ * All state (MPS sites, MPOs, environments) is device resident and owned by
+ * the library.  Calls on one context are ordered on the context's HIP stream;
+ * functions that return a host scalar synchronise that stream.  Distinct
+ * contexts may be used from distinct host threads.
+ */
+#ifndef MPSHIP_H
+#define MPSHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPS_OK 0
+#define MPS_E_ARG (-1)      /* invalid argument / DimensionMismatch        */
+#define MPS_E_HIP (-2)      /* HIP runtime error                            */
+#define MPS_E_ALLOC (-3)    /* device allocation failed                     */
+#define MPS_E_STATE (-4)    /* call not valid in the current state          */
+#define MPS_E_NUMERIC (-5)  /* numerical failure (e.g. SVD did not converge) */
+
+#define MPS_PREC_FP64 0     /* plain path: complex FP64 on MFMA (4M)        */
+#define MPS_PREC_FP32 1     /* precision form: operator terms in complex FP32 (3M) */
+
+typedef struct mps_ctx_s* mps_ctx_t;
+typedef struct mps_state_s* mps_state_t;  /* Vector{Array{TN,3}} on device    */
+typedef struct mps_mpo_s* mps_mpo_t;      /* Vector{Array{TN,4}} on device    */
+typedef struct mps_env_s* mps_env_t;      /* build_env result on device       */
+
+/* ---------------------------------------------------------------- context */
+const char* mps_last_error(void);
+int mps_version(void);
+int mps_ctx_create(int device, mps_ctx_t* out);
+int mps_ctx_destroy(mps_ctx_t ctx);
+/* Use an external HIP stream (hipStream_t passed as void*); NULL = own stream. */
+int mps_ctx_set_stream(mps_ctx_t ctx, void* stream);
+int mps_ctx_synchronize(mps_ctx_t ctx);
+/* Rank-revealing cutoff of the Householder generator: a reflector whose
+ * trailing column norm is <= tol*||A||_F is the identity (tau = 0).  0 keeps
+ * the exact LAPACK zgeqrf/zungqr semantics of qrq! (orthonormalisation.jl:180-198). */
+int mps_ctx_set_qr_rank_tol(mps_ctx_t ctx, double tol);
+/* MPS_PREC_FP64 (default) or MPS_PREC_FP32 for the operator (MPO) contractions. */
+int mps_ctx_set_precision(mps_ctx_t ctx, int prec);
+/* Enable / disable hipGraph capture of whole RK4 steps (default on). */
+int mps_ctx_set_graphs(mps_ctx_t ctx, int enable);
+/* Measurement extension (no counterpart in the reference): when enabled, every
+ * GEMM and QR launch is bracketed by HIP events on the context stream and
+ * graphs are not used.  mps_ctx_kernel_stats synchronises, fills
+ * out[0..7] = {gemm_ms, gemm_flops, gemm_bytes, gemm_launches, qr_ms,
+ * qr_flops, qr_launches, reserved} (algorithmic flops/bytes) and resets. */
+int mps_ctx_set_profiling(mps_ctx_t ctx, int enable);
+int mps_ctx_kernel_stats(mps_ctx_t ctx, double* out);
+
+/* ------------------------------------------------------- states (a1, a3) */
+/* Vector{Array{TN,3}} with N sites, local dims d[N], bond dims dims[N+1]
+ * (mpsdims, MatrixProductStates.jl:41-51). Contents are zero.           */
+int mps_state_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* dims, mps_state_t* out);
+int mps_state_destroy(mps_state_t s);
+int mps_state_dims(mps_state_t s, int64_t* dims_out /* N+1 */);
+int mps_state_upload_site(mps_state_t s, int site, const double* host, int64_t dl, int64_t d, int64_t dr);
+int mps_state_download_site(mps_state_t s, int site, double* host, int64_t dl, int64_t d, int64_t dr);
+/* dst .= copy.(src)  (e.g. spin_model_quantum_trajectory_rk4.jl:130) */
+int mps_state_copy(mps_state_t dst, mps_state_t src);
+/* mpsproductstate (constructors.jl:8-32, 37-61): local state vectors
+ * states[n*d + s] (complex interleaved, per site), left canonical. */
+int mps_state_product(mps_state_t s, const double* states);
+
+/* ------------------------------------------------------------- MPOs (a1) */
+int mps_mpo_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* wdims /* N+1 */, mps_mpo_t* out);
+int mps_mpo_destroy(mps_mpo_t o);
+int mps_mpo_upload_site(mps_mpo_t o, int site, const double* host, int64_t wl, int64_t d, int64_t wr);
+/* O[site][wl_idx, :, wr_idx, :] = op (d x d) -- the per-step writes of
+ * update_h! (spin_model_quantum_trajectory_rk4.jl:274-284) and
+ * update_L_Ryd! (rydberg_density_matrix_gpu.jl:393-403). */
+int mps_mpo_set_block(mps_mpo_t o, int site, int64_t wl_idx, int64_t wr_idx, const double* op);
+
+/* -------------------------------------------------------- environments (a2) */
+/* build_env(TN, TA, dims1, dims2, mpo_dims) (constructors.jl:347-359) */
+int mps_env_create_op(mps_ctx_t ctx, int N, const int64_t* dims1, const int64_t* dims2,
+                      const int64_t* wdims, mps_env_t* out);
+/* build_env(TN, TA, dims1, dims2) (constructors.jl:363-375) */
+int mps_env_create(mps_ctx_t ctx, int N, const int64_t* dims1, const int64_t* dims2, mps_env_t* out);
+int mps_env_destroy(mps_env_t e);
+int mps_env_download_slot(mps_env_t e, int slot, double* host, int64_t n_elems);
+
+/* ------------------------------------- L1 contractions on raw device memory
+ * (a4-a9, a23; src/mps_base.jl).  Pointers are device addresses of
+ * interleaved complex-double column-major arrays; shapes as in the reference. */
+/* prod_AOR (mps_base.jl:7-19): out[Dl,wl,d,Dr0] from A[Dl,d,Dr], O[wl,d,wr,d], R[Dr,wr,Dr0] */
+int mps_dev_prod_AOR(mps_ctx_t ctx, void* out, const void* A, int64_t Dl, int64_t d, int64_t Dr,
+                     const void* O, int64_t wl, int64_t wr, const void* R, int64_t Dr0);
+/* prod_AOL (mps_base.jl:26-39): out[Dl0,d,Dr,wr] from A[Dl,d,Dr], O, L[Dl,wl,Dl0] */
+int mps_dev_prod_AOL(mps_ctx_t ctx, void* out, const void* A, int64_t Dl, int64_t d, int64_t Dr,
+                     const void* O, int64_t wl, int64_t wr, const void* L, int64_t Dl0);
+/* prod_LR (3x4) (mps_base.jl:45-61): out[l3,r3,r4] = L[r1*r2,l3]^H * R[r1*r2, r3*r4] */
+int mps_dev_prod_LR_34(mps_ctx_t ctx, void* out, const void* L, const void* R,
+                       int64_t r1, int64_t r2, int64_t l3, int64_t r3, int64_t r4);
+/* prod_LR (4x3) (mps_base.jl:65-81): out[l1,l2,r3] = L[l1*l2, l3*l4] * R[l3*l4, r3] */
+int mps_dev_prod_LR_43(mps_ctx_t ctx, void* out, const void* L, const void* R,
+                       int64_t l1, int64_t l2, int64_t l3, int64_t l4, int64_t r3);
+/* prod_LR (2x3) (mps_base.jl:85-108): out = L[l1,r1] * R[r1, r2*r3]; co: L^T (92-118) */
+int mps_dev_prod_LR_23(mps_ctx_t ctx, void* out, const void* L, const void* R,
+                       int64_t l1, int64_t r1, int64_t r2, int64_t r3, int transpose_L);
+/* prod_LR (3x2) (mps_base.jl:122-137): out = L[l1*l2, l3] * R[l3, r2] */
+int mps_dev_prod_LR_32(mps_ctx_t ctx, void* out, const void* L, const void* R,
+                       int64_t l1, int64_t l2, int64_t l3, int64_t r2);
+/* update_re (mps_base.jl:142-160): out[r1,r2,a1] = AOR[r1*r2, r3*r4] * A0[a1, r3*r4]^H */
+int mps_dev_update_re(mps_ctx_t ctx, void* out, const void* A0, const void* AOR,
+                      int64_t r1, int64_t r2, int64_t r3, int64_t r4, int64_t a1);
+/* update_le (mps_base.jl:223-241): out[l3,l4,a3] = AOL[l1*l2, l3*l4]^H * A0[l1*l2, a3] */
+int mps_dev_update_le(mps_ctx_t ctx, void* out, const void* A0, const void* AOL,
+                      int64_t l1, int64_t l2, int64_t l3, int64_t l4, int64_t a3);
+/* update_renv (mps_base.jl:185-199): out[ar1, a1] = AR[ar1, d*a2] * A0[a1, d*a2]^H */
+int mps_dev_update_renv(mps_ctx_t ctx, void* out, const void* A0, const void* AR,
+                        int64_t a1, int64_t d, int64_t a2, int64_t ar1);
+/* update_lenv (mps_base.jl:267-281): out[a3, al3] = A0[a1*d, a3]^H * AL[a1*d, al3]
+ * mode 0: update_lenv; 1: update_lenv_co (283-289, AL^T conj(A0) -> [al3, a3]);
+ * 2: update_lenv_nc (319-325, A0^T AL). */
+int mps_dev_update_lenv(mps_ctx_t ctx, void* out, const void* A0, const void* AL,
+                        int64_t a1, int64_t d, int64_t a3, int64_t al3, int mode);
+/* local_exp (mps_base.jl:341-349): trace(O * A_p * A0_p^H) -> out (host, 2 doubles) */
+int mps_dev_local_exp(mps_ctx_t ctx, double* out, const void* A0, const void* A,
+                      int64_t a1, int64_t d, int64_t a3, const double* O_host);
+
+/* ---------------------------------------------- L2 gauge moves (a10-a12) */
+/* qrq! (orthonormalisation.jl:180-198): in place, the m x n matrix at dev
+ * (element (r,c) at r*rs + c*cs) is replaced by the Q of its Householder QR
+ * (zgeqrf + zungqr semantics, m >= n).  R (n x n, column-major, upper) is
+ * written to R_out when non-NULL (Julia qr, used by leftorth/rightorth). */
+int mps_dev_qrq(mps_ctx_t ctx, void* dev, int64_t m, int64_t n, int64_t rs, int64_t cs, void* R_out);
+/* leftorth! / rightorth! (orthonormalisation.jl:164-171, 227-234) on one site in place. */
+int mps_leftorth_site(mps_state_t s, int site);
+int mps_rightorth_site(mps_state_t s, int site);
+/* leftorthogonalizeQR! / leftorthonormalizationQR! (orthonormalisation.jl:29-56) */
+int mps_leftorthogonalize_qr(mps_state_t s, double* norm2_out);
+int mps_leftorthonormalize_qr(mps_state_t s, double* norm2_out);
+/* rightorthogonalizeQR! / rightorthonormalizationQR! (orthonormalisation.jl:64-90) */
+int mps_rightorthogonalize_qr(mps_state_t s, double* norm2_out);
+int mps_rightorthonormalize_qr(mps_state_t s, double* norm2_out);
+/* normalize_lo! (orthonormalisation.jl:97-103) and norm_lo (:117-121) */
+int mps_normalize_lo(mps_state_t s, double* norm2_out);
+int mps_norm_lo(mps_state_t s, double* norm2_out);
+
+/* ------------------------------------------------ L3 sweeps (a13-a19, a21) */
+/* compress_var_apply_H_left! (compression.jl:321-347) */
+int mps_compress_var_apply_H_left(mps_state_t A0, mps_env_t envop, mps_state_t Ain,
+                                  mps_state_t A, mps_mpo_t H);
+/* compress_sum_var_apply_H_left! (compression.jl:889-944); coef: numv complex
+ * (interleaved); A: numv states; env: numv-1 overlap environments. */
+int mps_compress_sum_var_apply_H_left(mps_state_t A0, const mps_env_t* env, mps_env_t envop,
+                                      mps_state_t Ain, int numv, const double* coef,
+                                      const mps_state_t* A, mps_mpo_t H);
+/* compress_sum_var_apply_H_right! (compression.jl:1007-1063).  A0 may be A[0]
+ * (the aliasing of timeevolution.jl:178-186). */
+int mps_compress_sum_var_apply_H_right(mps_state_t A0, const mps_env_t* env, mps_env_t envop,
+                                       mps_state_t Ain, int numv, const double* coef,
+                                       const mps_state_t* A, mps_mpo_t H);
+/* compress_var_apply_H! (compression.jl:217-258) */
+int mps_compress_var_apply_H(mps_state_t A0, mps_env_t envop, mps_state_t Ain, mps_state_t A,
+                             mps_mpo_t H, int maxsweeps);
+/* compress_var! (compression.jl:65-103) */
+int mps_compress_var(mps_state_t A0, mps_state_t A, mps_env_t env, int maxsweeps);
+/* compress_sum_var (compression.jl:496-565): result written to A0out (same
+ * dims as Ain); Ain is the initial guess and may alias A0out. */
+int mps_compress_sum_var(mps_state_t A0out, mps_state_t Ain, int numv, const double* coef,
+                         const mps_state_t* A, int maxsweeps);
+/* compress_sum_var_apply_H! (compression.jl:805-887), by intent (the
+ * reference's environment initialisation at :816-817 throws). */
+int mps_compress_sum_var_apply_H(mps_state_t A0, mps_env_t envop, int numv, const double* coef,
+                                 const mps_state_t* A, mps_mpo_t H, int maxsweeps);
+/* compressionSVD! (compression.jl:1280-1312): err_trim / ent_ent have N entries. */
+int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glob,
+                        double* err_trim, double* ent_ent);
+
+/* ------------------------------------------------ L3 time steppers (a16, a20) */
+/* RK4stp_apply_H_half! (timeevolution.jl:178-196).  psi0 == NULL selects the
+ * 10-argument form (result overwrites psi); otherwise the 11-argument form. */
+int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
+                            mps_state_t psi0, mps_state_t psi, mps_state_t psi1,
+                            mps_state_t psi2, mps_state_t psi3, mps_env_t env1,
+                            mps_env_t env2, mps_env_t env3, mps_env_t envop);
+/* Arnoldi_expm! (timeevolution.jl:88-119); coef_out (numv complex) optional. */
+int mps_arnoldi_expm(mps_mpo_t A, int numv, const mps_state_t* rhov, mps_env_t env,
+                     mps_env_t envop, double* coef_out);
+
+/* ------------------------------------ scalar products and observables (a22-a24) */
+/* scal_prod (MatrixProductStates.jl:345-358): <A|B> */
+int mps_scal_prod(mps_state_t A, mps_state_t B, double* out);
+/* scal_op_prod (MatrixProductStates.jl:327-340): conj(<A|O|B>) */
+int mps_scal_op_prod(mps_state_t A, mps_mpo_t O, mps_state_t B, double* out);
+/* scal_prod_no_conj (MatrixProductStates.jl:366-379): bilinear A . B */
+int mps_scal_prod_no_conj(mps_state_t A, mps_state_t B, double* out);
+/* measure_excitations! (spin_model_quantum_trajectory_rk4.jl:220-231):
+ * out[n] = real(local_exp(A[n] R, A[n], op)) for a left-canonical, normalised state. */
+int mps_measure_local(mps_state_t A, const double* op /* d x d complex */, double* out /* N */);
+/* apply_site_operator! (MatrixProductStates.jl:135-143) */
+int mps_apply_site_operator(mps_state_t A, int site, const double* op /* d x d complex */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSHIP_H */

@@ -1,0 +1,1198 @@
+// extern "C" boundary (include/mpship.h): handles, transfers, error codes,
+// hipGraph caching of RK4 steps.
+#include "../../include/mpship.h"
+#include "contract.h"
+#include "sweeps.h"
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+using namespace mpship;
+using cd = std::complex<double>;
+
+struct mps_ctx_s {
+  Ctx c;
+  // cached RK4 graphs, keyed by the handles and parameters baked into them
+  struct Key {
+    std::vector<const void*> h;
+    double dt;
+    int prec;
+    double tol;
+    size_t ws_gen;
+    bool operator<(const Key& o) const {
+      if (h != o.h) return h < o.h;
+      if (dt != o.dt) return dt < o.dt;
+      if (prec != o.prec) return prec < o.prec;
+      if (tol != o.tol) return tol < o.tol;
+      return ws_gen < o.ws_gen;
+    }
+  };
+  std::map<Key, hipGraphExec_t> graphs;
+  std::map<Key, int> seen;
+  size_t ws_gen = 0;
+  void clear_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+    seen.clear();
+  }
+  ~mps_ctx_s() { clear_graphs(); }
+};
+struct mps_state_s {
+  State s;
+  mps_ctx_s* owner;
+};
+struct mps_mpo_s {
+  MPO o;
+  mps_ctx_s* owner;
+};
+struct mps_env_s {
+  Env e;
+  mps_ctx_s* owner;
+};
+
+static thread_local std::string g_err;
+
+template <typename F>
+static int guard(F&& f) {
+  try {
+    f();
+    return MPS_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return MPS_E_STATE;
+  }
+}
+
+static void set_device(Ctx* c) { HIPCHK(hipSetDevice(c->device)); }
+
+// Any change of device buffers or dims invalidates captured graphs.
+static void invalidate(mps_ctx_s* x) {
+  if (x) x->clear_graphs();
+}
+
+static void track_ws(mps_ctx_s* x, size_t before) {
+  if (x->c.ws_bytes != before) {
+    x->ws_gen++;
+    x->clear_graphs();
+  }
+}
+
+extern "C" {
+
+const char* mps_last_error(void) { return g_err.c_str(); }
+int mps_version(void) { return 1; }
+
+int mps_ctx_create(int device, mps_ctx_t* out) {
+  return guard([&] {
+    REQUIRE(out != nullptr, "null output");
+    auto* x = new mps_ctx_s();
+    try {
+      x->c.device = device;
+      HIPCHK(hipSetDevice(device));
+      hipDeviceProp_t prop;
+      HIPCHK(hipGetDeviceProperties(&prop, device));
+      x->c.num_cus = prop.multiProcessorCount;
+      HIPCHK(hipStreamCreateWithFlags(&x->c.stream, hipStreamNonBlocking));
+      x->c.own_stream = true;
+      HIPCHK(hipMalloc((void**)&x->c.dscal, 256 * sizeof(double)));
+      HIPCHK(hipHostMalloc((void**)&x->c.hscal, 256 * sizeof(double), hipHostMallocDefault));
+    } catch (...) {
+      delete x;
+      throw;
+    }
+    *out = x;
+  });
+}
+
+int mps_ctx_destroy(mps_ctx_t ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+  });
+}
+
+int mps_ctx_set_stream(mps_ctx_t ctx, void* stream) {
+  return guard([&] {
+    set_device(&ctx->c);
+    HIPCHK(hipStreamSynchronize(ctx->c.stream));
+    if (ctx->c.own_stream) HIPCHK(hipStreamDestroy(ctx->c.stream));
+    if (stream) {
+      ctx->c.stream = (hipStream_t)stream;
+      ctx->c.own_stream = false;
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+      ctx->c.own_stream = true;
+    }
+    invalidate(ctx);
+  });
+}
+
+int mps_ctx_synchronize(mps_ctx_t ctx) { return guard([&] { ctx->c.sync(); }); }
+
+int mps_ctx_set_qr_rank_tol(mps_ctx_t ctx, double tol) {
+  return guard([&] {
+    REQUIRE(tol >= 0.0 && tol < 1.0, "qr rank tolerance must be in [0, 1)");
+    ctx->c.qr_tol = tol;
+  });
+}
+
+int mps_ctx_set_precision(mps_ctx_t ctx, int prec) {
+  return guard([&] {
+    REQUIRE(prec == MPS_PREC_FP64 || prec == MPS_PREC_FP32, "unknown precision");
+    ctx->c.prec = prec;
+  });
+}
+
+int mps_ctx_set_graphs(mps_ctx_t ctx, int enable) {
+  return guard([&] {
+    ctx->c.graphs = enable != 0;
+    if (!enable) invalidate(ctx);
+  });
+}
+
+int mps_ctx_set_profiling(mps_ctx_t ctx, int enable) {
+  return guard([&] {
+    ctx->c.sync();
+    ctx->c.profiling = enable != 0;
+    ctx->c.prof.clear();
+    ctx->c.ev_used = 0;
+  });
+}
+
+int mps_ctx_kernel_stats(mps_ctx_t ctx, double* out) {
+  return guard([&] {
+    Ctx& c = ctx->c;
+    c.sync();
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (auto& r : c.prof) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+      if (r.kind == 0) {
+        v[0] += ms;
+        v[1] += r.flops;
+        v[2] += r.bytes;
+        v[3] += 1;
+      } else {
+        v[4] += ms;
+        v[5] += r.flops;
+        v[6] += 1;
+      }
+    }
+    std::memcpy(out, v, sizeof(v));
+    c.prof.clear();
+    c.ev_used = 0;
+  });
+}
+
+// ------------------------------------------------------------------ states
+int mps_state_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* dims, mps_state_t* out) {
+  return guard([&] {
+    REQUIRE(ctx && out && d && dims && N >= 1, "invalid arguments");
+    set_device(&ctx->c);
+    auto* x = new mps_state_s();
+    x->owner = ctx;
+    State& s = x->s;
+    s.ctx = &ctx->c;
+    s.N = N;
+    s.d.assign(d, d + N);
+    s.dims.assign(dims, dims + N + 1);
+    s.cap_l.resize(N);
+    s.cap_r.resize(N);
+    s.off.resize(N);
+    size_t tot = 0;
+    for (int j = 0; j < N; ++j) {
+      if (!(s.d[j] >= 1 && s.dims[j] >= 1 && s.dims[j + 1] >= 1)) {
+        delete x;
+        throw Error(MPS_E_ARG, "DimensionMismatch: dims must be >= 1");
+      }
+      s.cap_l[j] = s.dims[j];
+      s.cap_r[j] = s.dims[j + 1];
+      s.off[j] = (int64_t)tot;
+      tot += (size_t)s.dims[j] * s.d[j] * s.dims[j + 1];
+    }
+    s.total = tot;
+    if (hipMalloc((void**)&s.buf, tot * sizeof(double2)) != hipSuccess) {
+      s.buf = nullptr;
+      delete x;
+      throw Error(MPS_E_ALLOC, "state allocation failed");
+    }
+    HIPCHK(hipMemsetAsync(s.buf, 0, tot * sizeof(double2), ctx->c.stream));
+    ctx->c.sync();
+    *out = x;
+  });
+}
+
+int mps_state_destroy(mps_state_t s) {
+  return guard([&] {
+    if (!s) return;
+    s->owner->c.sync();
+    invalidate(s->owner);
+    delete s;
+  });
+}
+
+int mps_state_dims(mps_state_t s, int64_t* out) {
+  return guard([&] { std::memcpy(out, s->s.dims.data(), (s->s.N + 1) * sizeof(int64_t)); });
+}
+
+int mps_state_upload_site(mps_state_t s, int site, const double* host, int64_t dl, int64_t d, int64_t dr) {
+  return guard([&] {
+    State& st = s->s;
+    REQUIRE(site >= 0 && site < st.N, "site out of range");
+    REQUIRE(dl == st.Dl(site) && d == st.d[site] && dr == st.Dr(site), "site shape mismatch");
+    set_device(st.ctx);
+    HIPCHK(hipMemcpyAsync(st.site(site), host, dl * d * dr * sizeof(double2), hipMemcpyHostToDevice,
+                          st.ctx->stream));
+    st.ctx->sync();
+  });
+}
+
+int mps_state_download_site(mps_state_t s, int site, double* host, int64_t dl, int64_t d, int64_t dr) {
+  return guard([&] {
+    State& st = s->s;
+    REQUIRE(site >= 0 && site < st.N, "site out of range");
+    REQUIRE(dl == st.Dl(site) && d == st.d[site] && dr == st.Dr(site), "site shape mismatch");
+    set_device(st.ctx);
+    HIPCHK(hipMemcpyAsync(host, st.site(site), dl * d * dr * sizeof(double2), hipMemcpyDeviceToHost,
+                          st.ctx->stream));
+    st.ctx->sync();
+  });
+}
+
+int mps_state_copy(mps_state_t dst, mps_state_t src) {
+  return guard([&] {
+    State &a = dst->s, &b = src->s;
+    REQUIRE(a.N == b.N, "chain lengths differ");
+    for (int j = 0; j < a.N; ++j) {
+      REQUIRE(a.d[j] == b.d[j], "local dims differ");
+      REQUIRE(b.dims[j] <= a.cap_l[j] && b.dims[j + 1] <= a.cap_r[j], "destination capacity too small");
+    }
+    set_device(a.ctx);
+    bool dims_changed = a.dims != b.dims;
+    a.dims = b.dims;
+    for (int j = 0; j < a.N; ++j) launch_copy_c(a.site(j), b.site(j), b.size(j), a.ctx->stream);
+    if (dims_changed) invalidate(dst->owner);
+  });
+}
+
+int mps_state_product(mps_state_t s, const double* states) {
+  return guard([&] {
+    State& st = s->s;
+    set_device(st.ctx);
+    // raw zero-padded sites with [0, :, 0] = state (constructors.jl:19-20), then
+    // the left QR sweep that mpsproductstate interleaves with their construction
+    for (int j = 0; j < st.N; ++j) {
+      std::vector<double2> h((size_t)st.size(j), make_double2(0.0, 0.0));
+      const int64_t Dl = st.Dl(j);
+      for (int64_t a = 0; a < st.d[j]; ++a) {
+        int64_t off = 0;
+        for (int k = 0; k < j; ++k) off += st.d[k];
+        h[(size_t)(a * Dl)] = make_double2(states[2 * (off + a)], states[2 * (off + a) + 1]);
+      }
+      HIPCHK(hipMemcpyAsync(st.site(j), h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice,
+                            st.ctx->stream));
+      st.ctx->sync();
+    }
+    size_t before = st.ctx->ws_bytes;
+    Plan p(st.ctx, {&st}, {}, 0);
+    track_ws(s->owner, before);
+    left_orthogonalize(p, &st, 0);
+    st.ctx->sync();
+  });
+}
+
+// -------------------------------------------------------------------- MPOs
+int mps_mpo_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* wdims, mps_mpo_t* out) {
+  return guard([&] {
+    REQUIRE(ctx && out && d && wdims && N >= 1, "invalid arguments");
+    set_device(&ctx->c);
+    auto* x = new mps_mpo_s();
+    x->owner = ctx;
+    MPO& o = x->o;
+    o.ctx = &ctx->c;
+    o.N = N;
+    o.d.assign(d, d + N);
+    o.w.assign(wdims, wdims + N + 1);
+    o.off.resize(N);
+    size_t tot = 0;
+    for (int j = 0; j < N; ++j) {
+      o.off[j] = (int64_t)tot;
+      tot += (size_t)o.w[j] * o.d[j] * o.w[j + 1] * o.d[j];
+    }
+    o.total = tot;
+    if (hipMalloc((void**)&o.buf, tot * sizeof(double2)) != hipSuccess) {
+      o.buf = nullptr;
+      delete x;
+      throw Error(MPS_E_ALLOC, "MPO allocation failed");
+    }
+    HIPCHK(hipMemsetAsync(o.buf, 0, tot * sizeof(double2), ctx->c.stream));
+    ctx->c.sync();
+    *out = x;
+  });
+}
+
+int mps_mpo_destroy(mps_mpo_t o) {
+  return guard([&] {
+    if (!o) return;
+    o->owner->c.sync();
+    invalidate(o->owner);
+    delete o;
+  });
+}
+
+int mps_mpo_upload_site(mps_mpo_t o, int site, const double* host, int64_t wl, int64_t d, int64_t wr) {
+  return guard([&] {
+    MPO& m = o->o;
+    REQUIRE(site >= 0 && site < m.N, "site out of range");
+    REQUIRE(wl == m.wl(site) && d == m.d[site] && wr == m.wr(site), "MPO site shape mismatch");
+    set_device(m.ctx);
+    HIPCHK(hipMemcpyAsync(m.site(site), host, wl * d * wr * d * sizeof(double2), hipMemcpyHostToDevice,
+                          m.ctx->stream));
+    m.ctx->sync();
+  });
+}
+
+int mps_mpo_set_block(mps_mpo_t o, int site, int64_t wl_idx, int64_t wr_idx, const double* op) {
+  return guard([&] {
+    MPO& m = o->o;
+    REQUIRE(site >= 0 && site < m.N, "site out of range");
+    const int64_t wl = m.wl(site), wr = m.wr(site), d = m.d[site];
+    REQUIRE(wl_idx >= 0 && wl_idx < wl && wr_idx >= 0 && wr_idx < wr, "MPO block index out of range");
+    set_device(m.ctx);
+    // O[wl_idx, s, wr_idx, t] at wl_idx + wl*s + wl*d*wr_idx + wl*d*wr*t
+    for (int64_t t = 0; t < d; ++t)
+      HIPCHK(hipMemcpy2DAsync(m.site(site) + wl_idx + wl * d * wr_idx + wl * d * wr * t,
+                              wl * sizeof(double2), op + 2 * d * t, sizeof(double2), sizeof(double2), d,
+                              hipMemcpyHostToDevice, m.ctx->stream));
+    m.ctx->sync();
+  });
+}
+
+// ------------------------------------------------------------ environments
+static int env_create(mps_ctx_t ctx, int N, const int64_t* dims1, const int64_t* dims2, const int64_t* w,
+                      bool op, mps_env_t* out) {
+  return guard([&] {
+    REQUIRE(ctx && out && dims1 && dims2 && N >= 1, "invalid arguments");
+    set_device(&ctx->c);
+    auto* x = new mps_env_s();
+    x->owner = ctx;
+    Env& e = x->e;
+    e.ctx = &ctx->c;
+    e.N = N;
+    e.op = op;
+    e.d1.resize(N + 1);
+    e.wd.resize(N + 1);
+    e.d2.resize(N + 1);
+    e.off.resize(N + 1);
+    size_t tot = 0;
+    for (int j = 0; j <= N; ++j) {
+      const bool edge = (j == 0 || j == N);
+      e.d1[j] = edge ? 1 : dims1[j];
+      e.d2[j] = edge ? 1 : dims2[j];
+      e.wd[j] = (edge || !op) ? 1 : w[j];
+      e.off[j] = (int64_t)tot;
+      // slots hold left or right environments of either bond orientation
+      const int64_t big = std::max(e.d1[j], e.d2[j]);
+      tot += (size_t)big * e.wd[j] * big;
+    }
+    e.total = tot;
+    if (hipMalloc((void**)&e.buf, tot * sizeof(double2)) != hipSuccess) {
+      e.buf = nullptr;
+      delete x;
+      throw Error(MPS_E_ALLOC, "environment allocation failed");
+    }
+    HIPCHK(hipMemsetAsync(e.buf, 0, tot * sizeof(double2), ctx->c.stream));
+    // build_env: ones at both boundaries
+    launch_set_scalar(e.slot(0), 1.0, 0.0, ctx->c.stream);
+    launch_set_scalar(e.slot(N), 1.0, 0.0, ctx->c.stream);
+    ctx->c.sync();
+    *out = x;
+  });
+}
+
+int mps_env_create_op(mps_ctx_t ctx, int N, const int64_t* dims1, const int64_t* dims2,
+                      const int64_t* wdims, mps_env_t* out) {
+  if (!wdims) {
+    g_err = "null wdims";
+    return MPS_E_ARG;
+  }
+  return env_create(ctx, N, dims1, dims2, wdims, true, out);
+}
+
+int mps_env_create(mps_ctx_t ctx, int N, const int64_t* dims1, const int64_t* dims2, mps_env_t* out) {
+  return env_create(ctx, N, dims1, dims2, nullptr, false, out);
+}
+
+int mps_env_destroy(mps_env_t e) {
+  return guard([&] {
+    if (!e) return;
+    e->owner->c.sync();
+    invalidate(e->owner);
+    delete e;
+  });
+}
+
+int mps_env_download_slot(mps_env_t e, int slot, double* host, int64_t n) {
+  return guard([&] {
+    Env& v = e->e;
+    REQUIRE(slot >= 0 && slot <= v.N, "slot out of range");
+    const int64_t cap = (slot < v.N ? v.off[slot + 1] : (int64_t)v.total) - v.off[slot];
+    REQUIRE(n >= 0 && n <= cap, "slot size mismatch");
+    set_device(v.ctx);
+    HIPCHK(hipMemcpyAsync(host, v.slot(slot), n * sizeof(double2), hipMemcpyDeviceToHost, v.ctx->stream));
+    v.ctx->sync();
+  });
+}
+
+// ------------------------------------------------- raw L1 contractions
+#define RAW_PLAN(ctx, Dm, dm, wm)            \
+  set_device(&ctx->c);                       \
+  size_t before_ = ctx->c.ws_bytes;          \
+  Plan p(&ctx->c, (Dm), (dm), (wm), 0);      \
+  track_ws(ctx, before_);
+
+static i64 mx(std::initializer_list<i64> v) {
+  i64 m = 1;
+  for (i64 x : v) m = std::max(m, x);
+  return m;
+}
+
+int mps_dev_prod_AOR(mps_ctx_t ctx, void* out, const void* A, int64_t Dl, int64_t d, int64_t Dr,
+                     const void* O, int64_t wl, int64_t wr, const void* R, int64_t Dr0) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({Dl, Dr, Dr0}), d, mx({wl, wr}));
+    prod_AOR(p.work, p.op, out, A, Dl, d, Dr, O, wl, wr, R, Dr0);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_prod_AOL(mps_ctx_t ctx, void* out, const void* A, int64_t Dl, int64_t d, int64_t Dr,
+                     const void* O, int64_t wl, int64_t wr, const void* L, int64_t Dl0) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({Dl, Dr, Dl0}), d, mx({wl, wr}));
+    prod_AOL(p.work, p.op, out, A, Dl, d, Dr, O, wl, wr, L, Dl0);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_prod_LR_34(mps_ctx_t ctx, void* out, const void* L, const void* R, int64_t r1, int64_t r2,
+                       int64_t l3, int64_t r3, int64_t r4) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({r1, l3, r4}), r3, r2);
+    prod_LR_34(p.work, p.op, out, L, R, r1, r2, l3, r3, r4, false);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_prod_LR_43(mps_ctx_t ctx, void* out, const void* L, const void* R, int64_t l1, int64_t l2,
+                       int64_t l3, int64_t l4, int64_t r3) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({l1, l3, r3}), l2, l4);
+    prod_LR_43(p.work, p.op, out, L, R, l1, l2, l3, l4, r3, false);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_prod_LR_23(mps_ctx_t ctx, void* out, const void* L, const void* R, int64_t l1, int64_t r1,
+                       int64_t r2, int64_t r3, int transpose_L) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({l1, r1, r3}), r2, 1);
+    prod_LR_23(p.work, out, L, R, l1, r1, r2, r3, transpose_L != 0, false);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_prod_LR_32(mps_ctx_t ctx, void* out, const void* L, const void* R, int64_t l1, int64_t l2,
+                       int64_t l3, int64_t r2) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({l1, l3, r2}), l2, 1);
+    prod_LR_32(p.work, out, L, R, l1, l2, l3, r2, false);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_update_re(mps_ctx_t ctx, void* out, const void* A0, const void* AOR, int64_t r1, int64_t r2,
+                      int64_t r3, int64_t r4, int64_t a1) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({r1, r4, a1}), r3, r2);
+    update_re(p.work, p.op, out, AOR, A0, r1, r2, r3, r4, a1);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_update_le(mps_ctx_t ctx, void* out, const void* A0, const void* AOL, int64_t l1, int64_t l2,
+                      int64_t l3, int64_t l4, int64_t a3) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({l1, l3, a3}), l2, l4);
+    update_le(p.work, p.op, out, AOL, A0, l1, l2, l3, l4, a3);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_update_renv(mps_ctx_t ctx, void* out, const void* A0, const void* AR, int64_t a1, int64_t d,
+                        int64_t a2, int64_t ar1) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({a1, a2, ar1}), d, 1);
+    update_renv(p.work, out, A0, AR, a1, d, a2, ar1);
+    ctx->c.sync();
+  });
+}
+
+int mps_dev_update_lenv(mps_ctx_t ctx, void* out, const void* A0, const void* AL, int64_t a1, int64_t d,
+                        int64_t a3, int64_t al3, int mode) {
+  return guard([&] {
+    REQUIRE(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    RAW_PLAN(ctx, mx({a1, a3, al3}), d, 1);
+    update_lenv(p.work, out, A0, AL, a1, d, a3, al3, mode);
+    ctx->c.sync();
+  });
+}
+
+// M[s, s'] = sum_{a,b} A[a,s,b] conj(A0[a,s',b])  (local_exp's Ap * A0p^H)
+static void local_exp_matrix(Work& w, void* M, const void* A0, const void* A, i64 a1, i64 d, i64 a3) {
+  Gemm g;
+  g.M = d;
+  g.N = d;
+  g.K = a1 * a3;
+  g.A = A; g.am = im1(a1); g.ak = im2(a1, 1, a1 * d);
+  g.B = A0; g.bk = im2(a1, 1, a1 * d); g.bn = im1(a1); g.conjB = true;
+  g.C = M; g.cm = im1(1); g.cn = im1(d);
+  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes);
+}
+
+int mps_dev_local_exp(mps_ctx_t ctx, double* out, const void* A0, const void* A, int64_t a1, int64_t d,
+                      int64_t a3, const double* O_host) {
+  return guard([&] {
+    RAW_PLAN(ctx, mx({a1, a3}), d, 1);
+    local_exp_matrix(p.work, p.tmp, A0, A, a1, d, a3);
+    std::vector<double2> M((size_t)d * d);
+    HIPCHK(hipMemcpyAsync(M.data(), p.tmp, d * d * sizeof(double2), hipMemcpyDeviceToHost, ctx->c.stream));
+    ctx->c.sync();
+    // trace(O * M) (host: d x d)
+    cd t = 0.0;
+    for (int64_t s = 0; s < d; ++s)
+      for (int64_t q = 0; q < d; ++q)
+        t += cd(O_host[2 * (s + d * q)], O_host[2 * (s + d * q) + 1]) * cd(M[q + d * s].x, M[q + d * s].y);
+    out[0] = t.real();
+    out[1] = t.imag();
+  });
+}
+
+// ------------------------------------------------------------ gauge moves
+int mps_dev_qrq(mps_ctx_t ctx, void* dev, int64_t m, int64_t n, int64_t rs, int64_t cs, void* R_out) {
+  return guard([&] {
+    REQUIRE(m >= n && n >= 1, "qrq!: m >= n >= 1 required");
+    RAW_PLAN(ctx, mx({m, n}), 1, 1);
+    launch_qr(dev, m, n, rs, cs, ctx->c.qr_tol, p.tau, nullptr, R_out, ctx->c.stream);
+    ctx->c.sync();
+  });
+}
+
+#define STATE_PLAN(st, owner, n_ovl)                      \
+  set_device(st.ctx);                                     \
+  size_t before_ = st.ctx->ws_bytes;                      \
+  Plan p(st.ctx, {&st}, {}, n_ovl);                       \
+  track_ws(owner, before_);
+
+int mps_leftorth_site(mps_state_t s, int site) {
+  return guard([&] {
+    State& st = s->s;
+    REQUIRE(site >= 0 && site < st.N, "site out of range");
+    STATE_PLAN(st, s->owner, 0);
+    qr_left(p, &st, site, nullptr);
+    st.ctx->sync();
+  });
+}
+
+int mps_rightorth_site(mps_state_t s, int site) {
+  return guard([&] {
+    State& st = s->s;
+    REQUIRE(site >= 0 && site < st.N, "site out of range");
+    STATE_PLAN(st, s->owner, 0);
+    qr_right(p, &st, site, nullptr);
+    st.ctx->sync();
+  });
+}
+
+int mps_leftorthogonalize_qr(mps_state_t s, double* norm2) {
+  return guard([&] {
+    State& st = s->s;
+    STATE_PLAN(st, s->owner, 0);
+    left_orthogonalize(p, &st, 0);
+    const double no = read_scalar(st.ctx, 0);
+    if (norm2) *norm2 = no;
+  });
+}
+
+int mps_leftorthonormalize_qr(mps_state_t s, double* norm2) {
+  return guard([&] {
+    State& st = s->s;
+    STATE_PLAN(st, s->owner, 0);
+    left_orthogonalize(p, &st, 0);
+    launch_scale_rsqrt(st.site(st.N - 1), st.size(st.N - 1), st.ctx->dscal, st.ctx->stream);
+    const double no = read_scalar(st.ctx, 0);
+    if (norm2) *norm2 = no;
+  });
+}
+
+int mps_rightorthogonalize_qr(mps_state_t s, double* norm2) {
+  return guard([&] {
+    State& st = s->s;
+    STATE_PLAN(st, s->owner, 0);
+    right_orthogonalize(p, &st, 0);
+    const double no = read_scalar(st.ctx, 0);
+    if (norm2) *norm2 = no;
+  });
+}
+
+int mps_rightorthonormalize_qr(mps_state_t s, double* norm2) {
+  return guard([&] {
+    State& st = s->s;
+    STATE_PLAN(st, s->owner, 0);
+    right_orthogonalize(p, &st, 0);
+    launch_scale_rsqrt(st.site(0), st.size(0), st.ctx->dscal, st.ctx->stream);
+    const double no = read_scalar(st.ctx, 0);
+    if (norm2) *norm2 = no;
+  });
+}
+
+int mps_normalize_lo(mps_state_t s, double* norm2) {
+  return guard([&] {
+    State& st = s->s;
+    set_device(st.ctx);
+    const int n = st.N - 1;
+    launch_sum_abs2(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream);
+    launch_scale_rsqrt(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream);
+    const double no = read_scalar(st.ctx, 0);
+    if (norm2) *norm2 = no;
+  });
+}
+
+int mps_norm_lo(mps_state_t s, double* norm2) {
+  return guard([&] {
+    State& st = s->s;
+    set_device(st.ctx);
+    const int n = st.N - 1;
+    launch_sum_abs2(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream);
+    const double no = read_scalar(st.ctx, 0);
+    if (norm2) *norm2 = no;
+  });
+}
+
+// ------------------------------------------------------------------ sweeps
+static void check_states(std::initializer_list<const State*> v) {
+  const State* f = *v.begin();
+  for (const State* s : v) {
+    REQUIRE(s->N == f->N, "chain lengths differ");
+    REQUIRE(s->ctx == f->ctx, "states belong to different contexts");
+    for (int j = 0; j < s->N; ++j) REQUIRE(s->d[j] == f->d[j], "local dims differ");
+  }
+}
+
+static void check_op_env(const Env& e, const State& s) {
+  REQUIRE(e.op, "operator environment expected (build_env with mpo_dims)");
+  REQUIRE(e.N == s.N, "environment length differs from chain length");
+}
+
+static void check_ovl_env(const Env& e, const State& s) {
+  REQUIRE(!e.op, "overlap environment expected (build_env without mpo_dims)");
+  REQUIRE(e.N == s.N, "environment length differs from chain length");
+}
+
+static void check_mpo(const MPO& o, const State& s) {
+  REQUIRE(o.N == s.N, "MPO length differs from chain length");
+  for (int j = 0; j < s.N; ++j) REQUIRE(o.d[j] == s.d[j], "MPO local dims differ");
+}
+
+int mps_compress_var_apply_H_left(mps_state_t A0, mps_env_t envop, mps_state_t Ain, mps_state_t A,
+                                  mps_mpo_t H) {
+  return guard([&] {
+    check_states({&A0->s, &Ain->s, &A->s});
+    check_op_env(envop->e, A0->s);
+    check_mpo(H->o, A0->s);
+    Ctx* c = A0->s.ctx;
+    set_device(c);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&A0->s, &Ain->s, &A->s}, {&H->o}, 0);
+    track_ws(A0->owner, before);
+    compress_var_apply_H_left(p, &A0->s, &envop->e, &Ain->s, &A->s, &H->o);
+    c->sync();
+  });
+}
+
+static void sum_var_common(mps_state_t A0, const mps_env_t* env, mps_env_t envop, mps_state_t Ain, int numv,
+                           const double* coef, const mps_state_t* A, mps_mpo_t H, bool right) {
+  // the reference always forms the first overlap term (compression.jl:918, 1036): numv >= 2
+  REQUIRE(numv >= 2 && A && coef, "numv >= 2 states and coefficients required (BoundsError)");
+  std::vector<State*> As(numv);
+  std::vector<Env*> Es(numv - 1);
+  std::vector<cd> cf(numv);
+  for (int l = 0; l < numv; ++l) {
+    As[l] = &A[l]->s;
+    cf[l] = cd(coef[2 * l], coef[2 * l + 1]);
+    check_states({&A0->s, As[l]});
+  }
+  for (int l = 0; l + 1 < numv; ++l) {
+    Es[l] = &env[l]->e;
+    check_ovl_env(*Es[l], A0->s);
+  }
+  check_states({&A0->s, &Ain->s});
+  check_op_env(envop->e, A0->s);
+  check_mpo(H->o, A0->s);
+  Ctx* c = A0->s.ctx;
+  set_device(c);
+  i64 Dm = std::max(A0->s.max_dim(), Ain->s.max_dim()), dm = 1;
+  for (int l = 0; l < numv; ++l) Dm = std::max(Dm, As[l]->max_dim());
+  for (int j = 0; j < A0->s.N; ++j) dm = std::max(dm, A0->s.d[j]);
+  size_t before = c->ws_bytes;
+  Plan pp(c, Dm, dm, H->o.max_w(), numv);
+  track_ws(A0->owner, before);
+  if (right)
+    compress_sum_var_apply_H_right(pp, &A0->s, Es.data(), &envop->e, &Ain->s, numv, cf.data(), As.data(),
+                                   &H->o);
+  else
+    compress_sum_var_apply_H_left(pp, &A0->s, Es.data(), &envop->e, &Ain->s, numv, cf.data(), As.data(),
+                                  &H->o);
+  c->sync();
+}
+
+int mps_compress_sum_var_apply_H_left(mps_state_t A0, const mps_env_t* env, mps_env_t envop,
+                                      mps_state_t Ain, int numv, const double* coef, const mps_state_t* A,
+                                      mps_mpo_t H) {
+  return guard([&] { sum_var_common(A0, env, envop, Ain, numv, coef, A, H, false); });
+}
+
+int mps_compress_sum_var_apply_H_right(mps_state_t A0, const mps_env_t* env, mps_env_t envop,
+                                       mps_state_t Ain, int numv, const double* coef, const mps_state_t* A,
+                                       mps_mpo_t H) {
+  return guard([&] { sum_var_common(A0, env, envop, Ain, numv, coef, A, H, true); });
+}
+
+int mps_compress_var_apply_H(mps_state_t A0, mps_env_t envop, mps_state_t Ain, mps_state_t A, mps_mpo_t H,
+                             int maxsweeps) {
+  return guard([&] {
+    check_states({&A0->s, &Ain->s, &A->s});
+    check_op_env(envop->e, A0->s);
+    check_mpo(H->o, A0->s);
+    REQUIRE(maxsweeps >= 0, "maxsweeps >= 0");
+    Ctx* c = A0->s.ctx;
+    set_device(c);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&A0->s, &Ain->s, &A->s}, {&H->o}, 0);
+    track_ws(A0->owner, before);
+    compress_var_apply_H(p, &A0->s, &envop->e, &Ain->s, &A->s, &H->o, maxsweeps);
+    c->sync();
+  });
+}
+
+int mps_compress_var(mps_state_t A0, mps_state_t A, mps_env_t env, int maxsweeps) {
+  return guard([&] {
+    check_states({&A0->s, &A->s});
+    check_ovl_env(env->e, A0->s);
+    REQUIRE(maxsweeps >= 0, "maxsweeps >= 0");
+    Ctx* c = A0->s.ctx;
+    set_device(c);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&A0->s, &A->s}, {}, 0);
+    track_ws(A0->owner, before);
+    compress_var(p, &A0->s, &A->s, &env->e, maxsweeps);
+    c->sync();
+  });
+}
+
+// Temporary overlap environments for the allocating reference routines.
+struct TmpEnvs {
+  std::vector<mps_env_t> v;
+  ~TmpEnvs() {
+    for (auto e : v) mps_env_destroy(e);
+  }
+};
+
+static void make_tmp_envs(mps_ctx_s* ctx, const State& s, int n, TmpEnvs& t, std::vector<Env*>& out) {
+  std::vector<int64_t> big(s.N + 1);
+  for (int j = 0; j <= s.N; ++j) big[j] = s.dims[j];
+  for (int i = 0; i < n; ++i) {
+    mps_env_t e = nullptr;
+    int rc = mps_env_create(ctx, s.N, big.data(), big.data(), &e);
+    if (rc != MPS_OK) throw Error(rc, g_err);
+    t.v.push_back(e);
+    out.push_back(&e->e);
+  }
+}
+
+int mps_compress_sum_var(mps_state_t A0out, mps_state_t Ain, int numv, const double* coef,
+                         const mps_state_t* A, int maxsweeps) {
+  return guard([&] {
+    REQUIRE(numv >= 1 && A && coef, "numv >= 1 states and coefficients required");
+    check_states({&A0out->s, &Ain->s});
+    std::vector<State*> As(numv);
+    std::vector<cd> cf(numv);
+    bool alias = false;
+    for (int l = 0; l < numv; ++l) {
+      As[l] = &A[l]->s;
+      cf[l] = cd(coef[2 * l], coef[2 * l + 1]);
+      check_states({&A0out->s, As[l]});
+      if (A[l] == A0out) alias = true;
+    }
+    Ctx* c = A0out->s.ctx;
+    set_device(c);
+    TmpEnvs te;
+    std::vector<Env*> FL;
+    make_tmp_envs(A0out->owner, Ain->s, numv, te, FL);
+    // the reference returns a fresh A0: write to a temporary when A0out is one of A
+    mps_state_t out = A0out;
+    mps_state_t tmp = nullptr;
+    if (alias) {
+      int rc = mps_state_create(A0out->owner, Ain->s.N, Ain->s.d.data(), Ain->s.dims.data(), &tmp);
+      if (rc != MPS_OK) throw Error(rc, g_err);
+      out = tmp;
+    }
+    try {
+      size_t before = c->ws_bytes;
+      Plan p(c, {&out->s, &Ain->s}, {}, numv);
+      track_ws(A0out->owner, before);
+      compress_sum_var(p, &out->s, &Ain->s, numv, cf.data(), As.data(), maxsweeps, FL);
+      if (alias) {
+        A0out->s.dims = out->s.dims;
+        for (int j = 0; j < out->s.N; ++j) launch_copy_c(A0out->s.site(j), out->s.site(j), out->s.size(j), c->stream);
+      }
+      c->sync();
+    } catch (...) {
+      if (tmp) mps_state_destroy(tmp);
+      throw;
+    }
+    if (tmp) mps_state_destroy(tmp);
+  });
+}
+
+int mps_compress_sum_var_apply_H(mps_state_t A0, mps_env_t envop, int numv, const double* coef,
+                                 const mps_state_t* A, mps_mpo_t H, int maxsweeps) {
+  return guard([&] {
+    REQUIRE(numv >= 2 && A && coef, "numv >= 2 states and coefficients required (BoundsError)");
+    std::vector<State*> As(numv);
+    std::vector<cd> cf(numv);
+    for (int l = 0; l < numv; ++l) {
+      As[l] = &A[l]->s;
+      cf[l] = cd(coef[2 * l], coef[2 * l + 1]);
+      check_states({&A0->s, As[l]});
+      REQUIRE(A[l] != A0, "compress_sum_var_apply_H!: A0 must not be one of A");
+    }
+    check_op_env(envop->e, A0->s);
+    check_mpo(H->o, A0->s);
+    Ctx* c = A0->s.ctx;
+    set_device(c);
+    TmpEnvs te;
+    std::vector<Env*> env;
+    make_tmp_envs(A0->owner, A0->s, numv - 1, te, env);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&A0->s}, {&H->o}, numv);
+    track_ws(A0->owner, before);
+    compress_sum_var_apply_H(p, &A0->s, &envop->e, numv, cf.data(), As.data(), &H->o, maxsweeps, env);
+    c->sync();
+  });
+}
+
+int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glob, double* err_trim,
+                        double* ent_ent) {
+  return guard([&] { throw Error(MPS_E_STATE, "compressionSVD!: device SVD not built yet"); });
+}
+
+// ----------------------------------------------------------------- RK4
+int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3, mps_state_t psi0,
+                            mps_state_t psi, mps_state_t psi1, mps_state_t psi2, mps_state_t psi3,
+                            mps_env_t env1, mps_env_t env2, mps_env_t env3, mps_env_t envop) {
+  return guard([&] {
+    mps_state_t out = psi0 ? psi0 : psi;
+    check_states({&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s});
+    for (auto e : {env1, env2, env3}) check_ovl_env(e->e, psi->s);
+    check_op_env(envop->e, psi->s);
+    for (auto h : {H1, H2, H3}) check_mpo(h->o, psi->s);
+    mps_ctx_s* x = psi->owner;
+    Ctx* c = &x->c;
+    set_device(c);
+    mps_ctx_s::Key key{{H1, H2, H3, psi0, psi, psi1, psi2, psi3, env1, env2, env3, envop}, dt, c->prec,
+                       c->qr_tol, x->ws_gen};
+    auto run = [&](Plan& p) {
+      rk4_half(p, dt, &H1->o, &H2->o, &H3->o, &out->s, &psi->s, &psi1->s, &psi2->s, &psi3->s, &env1->e,
+               &env2->e, &env3->e, &envop->e);
+    };
+    size_t before = c->ws_bytes;
+    Plan p(c, {&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s}, {&H1->o, &H2->o, &H3->o}, 4);
+    track_ws(x, before);
+    key.ws_gen = x->ws_gen;
+    if (!c->graphs || c->profiling) {
+      run(p);
+      c->sync();
+      return;
+    }
+    auto it = x->graphs.find(key);
+    if (it == x->graphs.end()) {
+      int& n = x->seen[key];
+      if (n++ == 0) {   // first call: eager (sizes the workspace)
+        run(p);
+        c->sync();
+        return;
+      }
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      c->capturing = true;
+      try {
+        run(p);
+      } catch (...) {
+        c->capturing = false;
+        hipGraph_t dummy = nullptr;
+        (void)hipStreamEndCapture(c->stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        throw;
+      }
+      c->capturing = false;
+      HIPCHK(hipStreamEndCapture(c->stream, &g));
+      hipGraphExec_t ex = nullptr;
+      HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      it = x->graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, c->stream));
+    c->sync();
+  });
+}
+
+// ------------------------------------------------------------ scalar products
+static void scal_common(mps_state_t A, mps_state_t B, int mode, double* out) {
+  check_states({&A->s, &B->s});
+  Ctx* c = A->s.ctx;
+  set_device(c);
+  size_t before = c->ws_bytes;
+  Plan p(c, {&A->s, &B->s}, {}, 0);
+  track_ws(A->owner, before);
+  scal_prod(p, &A->s, &B->s, mode, 0);
+  HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  out[0] = c->hscal[0];
+  out[1] = c->hscal[1];
+}
+
+int mps_scal_prod(mps_state_t A, mps_state_t B, double* out) {
+  return guard([&] { scal_common(A, B, 0, out); });
+}
+
+int mps_scal_prod_no_conj(mps_state_t A, mps_state_t B, double* out) {
+  return guard([&] { scal_common(A, B, 2, out); });
+}
+
+int mps_scal_op_prod(mps_state_t A, mps_mpo_t O, mps_state_t B, double* out) {
+  return guard([&] {
+    check_states({&A->s, &B->s});
+    check_mpo(O->o, A->s);
+    Ctx* c = A->s.ctx;
+    set_device(c);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&A->s, &B->s}, {&O->o}, 0);
+    track_ws(A->owner, before);
+    scal_op_prod(p, &A->s, &O->o, &B->s, 0);
+    HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    out[0] = c->hscal[0];
+    out[1] = c->hscal[1];
+  });
+}
+
+// measure_excitations! (spin_model_quantum_trajectory_rk4.jl:220-231)
+int mps_measure_local(mps_state_t A, const double* op, double* out) {
+  return guard([&] {
+    State& s = A->s;
+    Ctx* c = s.ctx;
+    set_device(c);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&s}, {}, 0);
+    track_ws(A->owner, before);
+    i64 dmax = 1;
+    for (int j = 0; j < s.N; ++j) dmax = std::max(dmax, s.d[j]);
+    // per-site d x d matrices M_n in T3 (N * dmax^2 complex)
+    double2* Ms = (double2*)p.T3;
+    double2* cur = (double2*)p.envA;
+    double2* nxt = (double2*)p.envB;
+    launch_set_scalar(cur, 1.0, 0.0, c->stream);
+    for (int n = s.N - 1; n >= 0; --n) {
+      const i64 Dl = s.Dl(n), d = s.d[n], Dr = s.Dr(n);
+      // Aenv = prod_LR(A[n], env)
+      prod_LR_32(p.work, p.AL, s.site(n), cur, Dl, d, Dr, Dr, false);
+      local_exp_matrix(p.work, Ms + (size_t)n * dmax * dmax, p.AL, s.site(n), Dl, d, Dr);
+      // env = update_renv(A[n], Aenv)
+      update_renv(p.work, nxt, s.site(n), p.AL, Dl, d, Dr, Dl);
+      std::swap(cur, nxt);
+    }
+    std::vector<double2> h((size_t)s.N * dmax * dmax);
+    HIPCHK(hipMemcpyAsync(h.data(), Ms, h.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    for (int n = 0; n < s.N; ++n) {
+      const i64 d = s.d[n];
+      const double2* M = h.data() + (size_t)n * dmax * dmax;
+      cd t = 0.0;
+      for (i64 a = 0; a < d; ++a)
+        for (i64 b = 0; b < d; ++b)
+          t += cd(op[2 * (a + d * b)], op[2 * (a + d * b) + 1]) * cd(M[b + d * a].x, M[b + d * a].y);
+      out[n] = t.real();
+    }
+  });
+}
+
+// apply_site_operator! (MatrixProductStates.jl:135-143)
+int mps_apply_site_operator(mps_state_t A, int site, const double* op) {
+  return guard([&] {
+    State& s = A->s;
+    REQUIRE(site >= 0 && site < s.N, "site out of range");
+    Ctx* c = s.ctx;
+    set_device(c);
+    size_t before = c->ws_bytes;
+    Plan p(c, {&s}, {}, 0);
+    track_ws(A->owner, before);
+    const i64 Dl = s.Dl(site), d = s.d[site], Dr = s.Dr(site);
+    HIPCHK(hipMemcpyAsync(p.R, op, d * d * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    // tmp[a, s, b] = sum_t A[a, t, b] op[s, t]   (batch over b)
+    Gemm g;
+    g.M = Dl;
+    g.N = d;
+    g.K = d;
+    g.batch = Dr;
+    g.A = s.site(site); g.am = im1(1); g.ak = im1(Dl); g.a_bs = Dl * d;
+    g.B = p.R; g.bk = im1(d); g.bn = im1(1);
+    g.C = p.tmp; g.cm = im1(1); g.cn = im1(Dl); g.c_bs = Dl * d;
+    run_gemm(c, g, CMP_F64, p.splitk, p.splitk_bytes);
+    launch_copy_c(s.site(site), p.tmp, Dl * d * Dr, c->stream);
+    c->sync();
+  });
+}
+
+// ----------------------------------------------------------------- Arnoldi
+// Host dense helpers for the numv x numv Krylov matrices (the reference
+// evaluates expm(N \ H) on the host too, timeevolution.jl:114).
+static std::vector<cd> solve(std::vector<cd> A, std::vector<cd> B, int n) {
+  // column-major A (n x n), B (n x n): returns A \ B (partial pivoting)
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < n; ++i)
+      if (std::abs(A[i + n * k]) > std::abs(A[piv + n * k])) piv = i;
+    REQUIRE(std::abs(A[piv + n * k]) > 0.0, "singular Krylov overlap matrix");
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k + n * j], A[piv + n * j]);
+      for (int j = 0; j < n; ++j) std::swap(B[k + n * j], B[piv + n * j]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const cd f = A[i + n * k] / A[k + n * k];
+      for (int j = k; j < n; ++j) A[i + n * j] -= f * A[k + n * j];
+      for (int j = 0; j < n; ++j) B[i + n * j] -= f * B[k + n * j];
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    for (int i = n - 1; i >= 0; --i) {
+      cd s = B[i + n * j];
+      for (int k = i + 1; k < n; ++k) s -= A[i + n * k] * B[k + n * j];
+      B[i + n * j] = s / A[i + n * i];
+    }
+  return B;
+}
+
+static std::vector<cd> matmul(const std::vector<cd>& A, const std::vector<cd>& B, int n) {
+  std::vector<cd> C((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k)
+      for (int i = 0; i < n; ++i) C[i + n * j] += A[i + n * k] * B[k + n * j];
+  return C;
+}
+
+// exp(M) by scaling and squaring of a Taylor series (truncation below 1e-17 relative).
+static std::vector<cd> expm(std::vector<cd> M, int n) {
+  double nrm = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += std::abs(M[i + n * j]);
+    nrm = std::max(nrm, s);
+  }
+  int sq = 0;
+  while (nrm > 0.25) {
+    nrm *= 0.5;
+    sq++;
+  }
+  const double f = std::ldexp(1.0, -sq);
+  for (auto& x : M) x *= f;
+  std::vector<cd> E((size_t)n * n, 0.0), T((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) E[i + n * i] = T[i + n * i] = 1.0;
+  for (int k = 1; k <= 30; ++k) {
+    T = matmul(T, M, n);
+    for (auto& x : T) x /= (double)k;
+    for (size_t i = 0; i < E.size(); ++i) E[i] += T[i];
+  }
+  for (int s = 0; s < sq; ++s) E = matmul(E, E, n);
+  return E;
+}
+
+int mps_arnoldi_expm(mps_mpo_t A, int numv, const mps_state_t* rhov, mps_env_t /*env*/, mps_env_t envop,
+                     double* coef_out) {
+  return guard([&] {
+    REQUIRE(numv >= 1 && rhov, "numv >= 1 Krylov states required");
+    for (int l = 0; l < numv; ++l) check_states({&rhov[0]->s, &rhov[l]->s});
+    check_mpo(A->o, rhov[0]->s);
+    check_op_env(envop->e, rhov[0]->s);
+    std::vector<cd> N((size_t)numv * numv, 0.0), H((size_t)numv * numv, 0.0);
+    for (int i = 0; i < numv; ++i) N[i + numv * i] = 1.0;
+    auto sop = [&](mps_state_t a, mps_state_t b) {
+      double o[2];
+      int rc = mps_scal_op_prod(a, A, b, o);
+      if (rc != MPS_OK) throw Error(rc, g_err);
+      return cd(o[0], o[1]);
+    };
+    auto sp = [&](mps_state_t a, mps_state_t b) {
+      double o[2];
+      int rc = mps_scal_prod(a, b, o);
+      if (rc != MPS_OK) throw Error(rc, g_err);
+      return cd(o[0], o[1]);
+    };
+    H[0] = sop(rhov[0], rhov[0]);
+    for (int j = 1; j < numv; ++j) {
+      std::vector<double> cf(2 * (j + 1));
+      std::vector<mps_state_t> vs(j + 1);
+      for (int l = 0; l < j; ++l) {
+        const cd v = -H[l + numv * (j - 1)];
+        cf[2 * l] = v.real();
+        cf[2 * l + 1] = v.imag();
+        vs[l] = rhov[l];
+      }
+      cf[2 * j] = 1.0;
+      cf[2 * j + 1] = 0.0;
+      vs[j] = rhov[j - 1];
+      int rc = mps_compress_sum_var_apply_H(rhov[j], envop, j + 1, cf.data(), vs.data(), A, 1);
+      if (rc != MPS_OK) throw Error(rc, g_err);
+      double no;
+      rc = mps_normalize_lo(rhov[j], &no);
+      if (rc != MPS_OK) throw Error(rc, g_err);
+      H[j + numv * j] = sop(rhov[j], rhov[j]);
+      for (int l = 0; l < j; ++l) {
+        H[j + numv * l] = sop(rhov[j], rhov[l]);
+        H[l + numv * j] = sop(rhov[l], rhov[j]);
+        N[j + numv * l] = sp(rhov[j], rhov[l]);
+        N[l + numv * j] = std::conj(N[j + numv * l]);
+      }
+    }
+    std::vector<cd> E = expm(solve(N, H, numv), numv);
+    std::vector<double> cf(2 * numv);
+    for (int l = 0; l < numv; ++l) {
+      cf[2 * l] = E[l].real();
+      cf[2 * l + 1] = E[l].imag();
+    }
+    if (coef_out) std::memcpy(coef_out, cf.data(), cf.size() * sizeof(double));
+    int rc = mps_compress_sum_var(rhov[0], rhov[0], numv, cf.data(), rhov, 1);
+    if (rc != MPS_OK) throw Error(rc, g_err);
+    double no;
+    rc = mps_normalize_lo(rhov[0], &no);
+    if (rc != MPS_OK) throw Error(rc, g_err);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Internal declarations shared by the HIP kernels and the host-side sweep code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <stdexcept>
+#include <vector>
+
+namespace mpship {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      throw ::mpship::Error(-2, std::string(#x) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define REQUIRE(cond, msg)                                               \
+  do {                                                                   \
+    if (!(cond)) throw ::mpship::Error(-1, std::string("DimensionMismatch: ") + (msg)); \
+  } while (0)
+
+// ------------------------------------------------------------- index maps
+// A tensor index i (a flattened group of reference indices) maps to the
+// element offset (i % n0) * s0 + (i / n0) * s1.  n0 = 0 means a single
+// stride (offset i * s0).  Every reshape + permutedims of the reference
+// becomes a choice of maps; nothing is materialised.
+struct IdxMap {
+  int32_t n0;
+  int64_t s0, s1;
+};
+inline IdxMap im1(int64_t s) { return IdxMap{0, s, 0}; }
+inline IdxMap im2(int64_t n0, int64_t s0, int64_t s1) { return IdxMap{(int32_t)n0, s0, s1}; }
+
+enum DType { F64 = 0, F32 = 1 };   // storage of complex elements: double2 / float2
+
+// C[m,n] = alpha * sum_k opA(A)[m,k] opB(B)[k,n] + beta * C[m,n], batched.
+struct Gemm {
+  int64_t M = 0, N = 0, K = 0, batch = 1;
+  const void* A = nullptr; IdxMap am{}, ak{}; int64_t a_bs = 0; bool conjA = false; DType ta = F64;
+  const void* B = nullptr; IdxMap bk{}, bn{}; int64_t b_bs = 0; bool conjB = false; DType tb = F64;
+  void* C = nullptr; IdxMap cm{}, cn{}; int64_t c_bs = 0; DType tc = F64;
+  double alpha_re = 1, alpha_im = 0, beta_re = 0, beta_im = 0;
+};
+
+enum Compute { CMP_F64 = 0, CMP_F32_3M = 1 };
+
+// Workspace for split-K partial sums (provided by the caller, grown on demand).
+struct GemmWorkspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus);
+void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_cus, hipStream_t s);
+
+// Householder QR (zgeqrf + zungqr semantics, optional rank cutoff); in place Q.
+// Element (r, c) at base + r*rs + c*cs; m >= n.  tau_ws: n complex doubles.
+// scal_ws: 8 doubles.  R_out (optional): n x n column-major upper triangle.
+void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol,
+               void* tau_ws, void* scal_ws, void* R_out, hipStream_t s);
+
+// small kernels (misc.hip)
+void launch_set_scalar(void* p, double re, double im, hipStream_t s);
+void launch_fill_zero(void* p, int64_t n_complex, hipStream_t s);
+void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s);
+void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s);
+void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s);
+void launch_convert(void* dst, DType td, const void* src, DType ts, int64_t n, hipStream_t s);
+void launch_trace_small(const void* M, int64_t d, void* out, hipStream_t s);
+void launch_jacobi_svd(void* A, int64_t m, int64_t n, void* V, double* S, void* work, int* info,
+                       hipStream_t s);
+
+}  // namespace mpship

@@ -1,0 +1,131 @@
+// Context, device containers and the L1 contraction chain of mps_base.jl.
+#pragma once
+#include "common.h"
+#include <memory>
+
+namespace mpship {
+
+using i64 = int64_t;
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cus = 256;
+  double qr_tol = 0.0;
+  int prec = 0;          // 0 FP64, 1 FP32 operator terms
+  bool graphs = true;
+  // workspace arena (grow only, outside graph capture)
+  char* ws = nullptr;
+  size_t ws_bytes = 0;
+  double* dscal = nullptr;      // 256 doubles of device scalars
+  double* hscal = nullptr;      // pinned host mirror
+  bool capturing = false;
+  // live kernel timing (HIP events on this stream around every GEMM / QR launch)
+  bool profiling = false;
+  struct ProfRec {
+    hipEvent_t a, b;
+    int kind;          // 0 GEMM, 1 QR
+    double flops;      // algorithmic flops of the launch (8 per complex MAC)
+    double bytes;      // algorithmic HBM bytes (operands read once, result written once)
+  };
+  std::vector<ProfRec> prof;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t next_event();
+  void prof_begin(hipEvent_t& a);
+  void prof_end(hipEvent_t a, int kind, double flops, double bytes);
+
+  ~Ctx();
+  void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+  void ensure_ws(size_t bytes);
+};
+
+// N sites, per-site capacity fixed at creation (dims may only shrink).
+struct State {
+  Ctx* ctx = nullptr;
+  int N = 0;
+  std::vector<i64> d, dims, cap_l, cap_r, off;
+  double2* buf = nullptr;
+  size_t total = 0;
+  ~State();
+  double2* site(int j) const { return buf + off[j]; }
+  i64 Dl(int j) const { return dims[j]; }
+  i64 Dr(int j) const { return dims[j + 1]; }
+  i64 size(int j) const { return dims[j] * d[j] * dims[j + 1]; }
+  i64 max_site() const;
+  i64 max_dim() const;
+};
+
+struct MPO {
+  Ctx* ctx = nullptr;
+  int N = 0;
+  std::vector<i64> d, w, off;   // w: N+1 bond dims
+  double2* buf = nullptr;
+  size_t total = 0;
+  ~MPO();
+  double2* site(int j) const { return buf + off[j]; }
+  i64 wl(int j) const { return w[j]; }
+  i64 wr(int j) const { return w[j + 1]; }
+  i64 max_w() const;
+};
+
+struct Env {
+  Ctx* ctx = nullptr;
+  int N = 0;
+  bool op = false;
+  std::vector<i64> d1, wd, d2, off;   // N+1 slots
+  double2* buf = nullptr;
+  size_t total = 0;
+  ~Env();
+  double2* slot(int j) const { return buf + off[j]; }
+};
+
+// ----------------------------------------------------------- workspace
+// Carves named scratch regions out of the context arena for one call.
+struct Scratch {
+  Ctx* c;
+  size_t used = 0;
+  std::vector<std::pair<void**, size_t>> req;
+  explicit Scratch(Ctx* ctx) : c(ctx) {}
+  void add(void** p, size_t bytes) { req.emplace_back(p, (bytes + 255) & ~size_t(255)); }
+  void commit();
+};
+
+// compute options for one contraction
+struct Op {
+  Compute cmp = CMP_F64;
+  DType tmp = F64;   // storage of T1/T2 intermediates
+};
+
+void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes);
+
+// --------------------------------------------- L1 contractions (mps_base.jl)
+struct Work {
+  Ctx* c;
+  void* splitk = nullptr;
+  size_t splitk_bytes = 0;
+  void* T1 = nullptr;
+};
+
+void prod_AOR(Work& w, const Op& op, void* T2, const void* A, i64 Dl, i64 d, i64 Dr, const void* O,
+              i64 wl, i64 wr, const void* R, i64 Dr0);
+void prod_AOL(Work& w, const Op& op, void* AOL, const void* A, i64 Dl, i64 d, i64 Dr, const void* O,
+              i64 wl, i64 wr, const void* L, i64 Dl0);
+void prod_LR_34(Work& w, const Op& op, void* out, const void* L, const void* T2, i64 Dl, i64 wl, i64 Dl0,
+                i64 d, i64 Dr0, bool accumulate);
+void prod_LR_43(Work& w, const Op& op, void* out, const void* AOL, const void* R, i64 Dl0, i64 d, i64 Dr,
+                i64 wr, i64 Dr0, bool accumulate);
+void update_re(Work& w, const Op& op, void* out, const void* T2, const void* A0, i64 Dl, i64 wl, i64 d,
+               i64 Dr0, i64 Dl0);
+void update_le(Work& w, const Op& op, void* out, const void* AOL, const void* A0, i64 Dl0, i64 d, i64 Dr,
+               i64 wr, i64 Dr0);
+void prod_LR_23(Work& w, void* out, const void* L, const void* R, i64 l1, i64 r1, i64 r2, i64 r3,
+                bool transposeL, bool accumulate);
+void prod_LR_32(Work& w, void* out, const void* L, const void* R, i64 l1, i64 l2, i64 l3, i64 r2,
+                bool accumulate, bool transposeR = false);
+void update_renv(Work& w, void* out, const void* A0, const void* AR, i64 a1, i64 d, i64 a2, i64 ar1);
+void update_lenv(Work& w, void* out, const void* A0, const void* AL, i64 a1, i64 d, i64 a3, i64 al3,
+                 int mode);
+
+}  // namespace mpship

@@ -1,0 +1,427 @@
+// Complex GEMM on CDNA4 MFMA with two-level index maps.
+//
+// One kernel serves every contraction of the hot path: each operand index
+// (m, n, k) is a flattened group of reference tensor indices whose memory
+// offset is given by an IdxMap, so the permutedims/reshape chains of the
+// reference (mps_base.jl) cost nothing.
+//
+//   CMP_F64     complex FP64, 4 real MFMA per complex MAC
+//               (v_mfma_f64_16x16x4_f64, FP64 accumulation)
+//   CMP_F32_3M  complex FP32, 3 real MFMA per complex MAC (Gauss/3M:
+//               P1 = Ar Br, P2 = Ai Bi, P3 = (Ar+Ai)(Br+Bi);
+//               Re = P1 - P2, Im = P3 - P1 - P2), v_mfma_f32_16x16x4_f32,
+//               FP32 accumulation.
+//
+// Workgroup tile 64x64 (complex), K step 16, 256 threads = 4 waves of 64,
+// each wave a 32x32 block = 2x2 MFMA 16x16 tiles.  Deep-K products are split
+// over blockIdx.y (split-K) into a partial buffer and reduced by a second
+// kernel in a fixed order, so results are bitwise reproducible run to run.
+#include "common.h"
+
+namespace mpship {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+
+struct DevMap {
+  int32_t n0;
+  int64_t s0, s1;
+};
+
+struct GemmParams {
+  int M, N, K;
+  int tiles_m, tiles_n, splits, kchunk;
+  const void* A; DevMap am, ak; int64_t a_bs; int conjA; int a_mfast;
+  const void* B; DevMap bk, bn; int64_t b_bs; int conjB; int b_nfast;
+  void* C; DevMap cm, cn; int64_t c_bs;
+  double ar, ai, br, bi;
+  void* P;        // split-K partials: [(split*batch + b)][n][m]
+  int batch;
+};
+
+__device__ __forceinline__ int64_t moff(const DevMap& mp, int i) {
+  if (mp.n0 == 0) return (int64_t)i * mp.s0;
+  const int q = i / mp.n0;
+  return (int64_t)(i - q * mp.n0) * mp.s0 + (int64_t)q * mp.s1;
+}
+
+__device__ __forceinline__ void ldc(const double2* p, double& re, double& im) {
+  const double2 v = *p;
+  re = v.x;
+  im = v.y;
+}
+__device__ __forceinline__ void ldc(const float2* p, double& re, double& im) {
+  const float2 v = *p;
+  re = v.x;
+  im = v.y;
+}
+__device__ __forceinline__ void stc(double2* p, double re, double im) { *p = make_double2(re, im); }
+__device__ __forceinline__ void stc(float2* p, double re, double im) {
+  *p = make_float2((float)re, (float)im);
+}
+
+// ------------------------------------------------------------ tile loaders
+// A tile: BM x BK, written to S[plane][k][m]; B tile: BK x BN -> S[plane][k][n].
+template <typename T, typename S, int NPL>
+__device__ __forceinline__ void load_a_tile(const GemmParams& p, const T* A, int m0, int k0, int kend,
+                                            S (*As)[BK][BM + 1]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < (BM * BK) / NT; ++j) {
+    int mm, kk;
+    if (p.a_mfast) {
+      mm = tid % BM;
+      kk = tid / BM + j * (NT / BM);
+    } else {
+      kk = tid % BK;
+      mm = tid / BK + j * (NT / BK);
+    }
+    const int gm = m0 + mm, gk = k0 + kk;
+    double re = 0.0, im = 0.0;
+    if (gm < p.M && gk < kend) {
+      ldc(A + moff(p.am, gm) + moff(p.ak, gk), re, im);
+      if (p.conjA) im = -im;
+    }
+    As[0][kk][mm] = (S)re;
+    As[1][kk][mm] = (S)im;
+    if constexpr (NPL == 3) As[2][kk][mm] = (S)re + (S)im;
+  }
+}
+
+template <typename T, typename S, int NPL>
+__device__ __forceinline__ void load_b_tile(const GemmParams& p, const T* B, int n0, int k0, int kend,
+                                            S (*Bs)[BK][BN + 1]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < (BN * BK) / NT; ++j) {
+    int nn, kk;
+    if (p.b_nfast) {
+      nn = tid % BN;
+      kk = tid / BN + j * (NT / BN);
+    } else {
+      kk = tid % BK;
+      nn = tid / BK + j * (NT / BK);
+    }
+    const int gn = n0 + nn, gk = k0 + kk;
+    double re = 0.0, im = 0.0;
+    if (gn < p.N && gk < kend) {
+      ldc(B + moff(p.bk, gk) + moff(p.bn, gn), re, im);
+      if (p.conjB) im = -im;
+    }
+    Bs[0][kk][nn] = (S)re;
+    Bs[1][kk][nn] = (S)im;
+    if constexpr (NPL == 3) Bs[2][kk][nn] = (S)re + (S)im;
+  }
+}
+
+// --------------------------------------------------------------- epilogue
+// MFMA 16x16 output layouts (measured on gfx950 with a lane probe):
+//   v_mfma_f32_16x16x4_f32: lane l, element r -> row 4*(l/16) + r, col l%16
+//   v_mfma_f64_16x16x4_f64: lane l, element r -> row (l/16) + 4*r, col l%16
+// Operands in both: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k.
+template <typename TC, bool F64LAYOUT>
+__device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split, int row0, int col0,
+                                           const double (&cr)[4], const double (&ci)[4]) {
+  const int lane = threadIdx.x & 63;
+  const int col = col0 + (lane & 15);
+  if (col >= p.N) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = F64LAYOUT ? row0 + (lane >> 4) + 4 * r : row0 + 4 * (lane >> 4) + r;
+    if (row >= p.M) continue;
+    if (p.splits > 1) {
+      double2* P = (double2*)p.P + ((int64_t)split * p.batch + b) * (int64_t)p.M * p.N;
+      P[(int64_t)col * p.M + row] = make_double2(cr[r], ci[r]);
+    } else {
+      TC* C = (TC*)p.C + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
+      double re = p.ar * cr[r] - p.ai * ci[r];
+      double im = p.ar * ci[r] + p.ai * cr[r];
+      if (p.br != 0.0 || p.bi != 0.0) {
+        double orr, oi;
+        ldc(C, orr, oi);
+        re += p.br * orr - p.bi * oi;
+        im += p.br * oi + p.bi * orr;
+      }
+      stc(C, re, im);
+    }
+  }
+}
+
+// ------------------------------------------------------------ FP64 kernel
+template <typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(NT) void zgemm_f64_kernel(GemmParams p) {
+  __shared__ double As[2][BK][BM + 1];
+  __shared__ double Bs[2][BK][BN + 1];
+  const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
+  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kbeg = split * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
+  const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  v4d accr[2][2], acci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accr[i][j] = v4d{0, 0, 0, 0};
+      acci[i][j] = v4d{0, 0, 0, 0};
+    }
+
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    load_a_tile<TA, double, 2>(p, A, m0, k0, kend, As);
+    load_b_tile<TB, double, 2>(p, B, n0, k0, kend, Bs);
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      const int k = kq * 4 + lk;
+      double a_r[2], a_i[2], b_r[2], b_i[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a_r[i] = As[0][k][wm + i * 16 + lr];
+        a_i[i] = As[1][k][wm + i * 16 + lr];
+        b_r[i] = Bs[0][k][wn + i * 16 + lr];
+        b_i[i] = Bs[1][k][wn + i * 16 + lr];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[i], b_i[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_i[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], acci[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      double cr[4], ci[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cr[r] = accr[i][j][r];
+        ci[r] = acci[i][j][r];
+      }
+      store_tile<TC, true>(p, b, split, m0 + wm + i * 16, n0 + wn + j * 16, cr, ci);
+    }
+}
+
+// --------------------------------------------------------- FP32 3M kernel
+template <typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(NT) void zgemm_f32_3m_kernel(GemmParams p) {
+  __shared__ float As[3][BK][BM + 1];
+  __shared__ float Bs[3][BK][BN + 1];
+  const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
+  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int kbeg = split * p.kchunk;
+  const int kend = min(p.K, kbeg + p.kchunk);
+  const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
+  const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  v4f p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      p1[i][j] = v4f{0, 0, 0, 0};
+      p2[i][j] = v4f{0, 0, 0, 0};
+      p3[i][j] = v4f{0, 0, 0, 0};
+    }
+
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    load_a_tile<TA, float, 3>(p, A, m0, k0, kend, As);
+    load_b_tile<TB, float, 3>(p, B, n0, k0, kend, Bs);
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < BK / 4; ++kq) {
+      const int k = kq * 4 + lk;
+      float a_r[2], a_i[2], a_s[2], b_r[2], b_i[2], b_s[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a_r[i] = As[0][k][wm + i * 16 + lr];
+        a_i[i] = As[1][k][wm + i * 16 + lr];
+        a_s[i] = As[2][k][wm + i * 16 + lr];
+        b_r[i] = Bs[0][k][wn + i * 16 + lr];
+        b_i[i] = Bs[1][k][wn + i * 16 + lr];
+        b_s[i] = Bs[2][k][wn + i * 16 + lr];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[i], b_r[j], p1[i][j], 0, 0, 0);
+          p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[i], b_i[j], p2[i][j], 0, 0, 0);
+          p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_s[i], b_s[j], p3[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      double cr[4], ci[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
+        cr[r] = (double)(x1 - x2);
+        ci[r] = (double)(x3 - x1 - x2);
+      }
+      store_tile<TC, false>(p, b, split, m0 + wm + i * 16, n0 + wn + j * 16, cr, ci);
+    }
+}
+
+// ------------------------------------------------------- split-K reduction
+template <typename TC>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmParams p) {
+  const int64_t MN = (int64_t)p.M * p.N;
+  const int64_t total = MN * p.batch;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(idx / MN);
+    const int64_t e = idx - (int64_t)b * MN;
+    const int col = (int)(e / p.M), row = (int)(e - (int64_t)col * p.M);
+    double sr = 0.0, si = 0.0;
+    const double2* P = (const double2*)p.P;
+    for (int s = 0; s < p.splits; ++s) {
+      const double2 v = P[((int64_t)s * p.batch + b) * MN + e];
+      sr += v.x;
+      si += v.y;
+    }
+    TC* C = (TC*)p.C + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
+    double re = p.ar * sr - p.ai * si;
+    double im = p.ar * si + p.ai * sr;
+    if (p.br != 0.0 || p.bi != 0.0) {
+      double orr, oi;
+      ldc(C, orr, oi);
+      re += p.br * orr - p.bi * oi;
+      im += p.br * oi + p.bi * orr;
+    }
+    stc(C, re, im);
+  }
+}
+
+// ------------------------------------------------------------------ host
+static DevMap dm(const IdxMap& m) { return DevMap{m.n0, m.s0, m.s1}; }
+
+static bool single_unit(const IdxMap& m) { return m.n0 == 0 && m.s0 == 1; }
+
+struct GemmPlan {
+  int tiles_m, tiles_n, splits, kchunk;
+};
+
+static GemmPlan make_plan(const Gemm& g, int num_cus) {
+  GemmPlan pl;
+  pl.tiles_m = (int)((g.M + BM - 1) / BM);
+  pl.tiles_n = (int)((g.N + BN - 1) / BN);
+  const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch;
+  const int64_t target = 2LL * num_cus;
+  int splits = 1;
+  if (blocks < target && g.K > 4 * BK) {
+    splits = (int)((target + blocks - 1) / blocks);
+    const int64_t max_splits = (g.K + 4 * BK - 1) / (4 * BK);   // >= 64-deep chunks
+    if (splits > max_splits) splits = (int)max_splits;
+    if (splits > 32) splits = 32;
+    if (splits < 1) splits = 1;
+  }
+  int64_t kchunk = (g.K + splits - 1) / splits;
+  kchunk = ((kchunk + BK - 1) / BK) * BK;
+  splits = (int)((g.K + kchunk - 1) / kchunk);
+  if (splits < 1) splits = 1;
+  pl.splits = splits;
+  pl.kchunk = (int)(kchunk > 0 ? kchunk : BK);
+  return pl;
+}
+
+size_t gemm_workspace_bytes(const Gemm& g, Compute, int num_cus) {
+  if (g.M == 0 || g.N == 0) return 0;
+  GemmPlan pl = make_plan(g, num_cus);
+  if (pl.splits <= 1) return 0;
+  return (size_t)pl.splits * g.batch * g.M * g.N * sizeof(double2);
+}
+
+template <typename TA, typename TB, typename TC>
+static void launch_typed(const GemmParams& p, Compute cmp, dim3 grid, hipStream_t s) {
+  if (cmp == CMP_F64)
+    hipLaunchKernelGGL((zgemm_f64_kernel<TA, TB, TC>), grid, dim3(NT), 0, s, p);
+  else
+    hipLaunchKernelGGL((zgemm_f32_3m_kernel<TA, TB, TC>), grid, dim3(NT), 0, s, p);
+}
+
+template <typename TA, typename TB>
+static void launch_tc(const GemmParams& p, DType tc, Compute cmp, dim3 grid, hipStream_t s) {
+  if (tc == F64)
+    launch_typed<TA, TB, double2>(p, cmp, grid, s);
+  else
+    launch_typed<TA, TB, float2>(p, cmp, grid, s);
+}
+
+void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_cus, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  REQUIRE(g.M < (1LL << 31) && g.N < (1LL << 31) && g.K < (1LL << 31), "GEMM too large");
+  GemmPlan pl = make_plan(g, num_cus);
+  GemmParams p;
+  p.M = (int)g.M;
+  p.N = (int)g.N;
+  p.K = (int)g.K;
+  p.tiles_m = pl.tiles_m;
+  p.tiles_n = pl.tiles_n;
+  p.splits = pl.splits;
+  p.kchunk = pl.kchunk;
+  p.A = g.A; p.am = dm(g.am); p.ak = dm(g.ak); p.a_bs = g.a_bs; p.conjA = g.conjA;
+  p.a_mfast = single_unit(g.am) ? 1 : 0;
+  p.B = g.B; p.bk = dm(g.bk); p.bn = dm(g.bn); p.b_bs = g.b_bs; p.conjB = g.conjB;
+  p.b_nfast = single_unit(g.bn) ? 1 : 0;
+  p.C = g.C; p.cm = dm(g.cm); p.cn = dm(g.cn); p.c_bs = g.c_bs;
+  p.ar = g.alpha_re; p.ai = g.alpha_im; p.br = g.beta_re; p.bi = g.beta_im;
+  p.batch = (int)g.batch;
+  p.P = nullptr;
+  if (g.K == 0) {
+    // C = beta * C (alpha term vanishes): run the reducer with zero splits.
+    p.splits = 0;
+    dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536));
+    if (g.tc == F64)
+      hipLaunchKernelGGL(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL(splitk_reduce_kernel<float2>, rg, dim3(256), 0, s, p);
+    return;
+  }
+  if (pl.splits > 1) {
+    const size_t need = (size_t)pl.splits * g.batch * g.M * g.N * sizeof(double2);
+    REQUIRE(ws != nullptr && ws_bytes >= need, "GEMM workspace too small");
+    p.P = ws;
+  }
+  dim3 grid((unsigned)(pl.tiles_m * pl.tiles_n), (unsigned)pl.splits, (unsigned)g.batch);
+  if (g.ta == F64 && g.tb == F64)
+    launch_tc<double2, double2>(p, g.tc, cmp, grid, s);
+  else if (g.ta == F64 && g.tb == F32)
+    launch_tc<double2, float2>(p, g.tc, cmp, grid, s);
+  else if (g.ta == F32 && g.tb == F64)
+    launch_tc<float2, double2>(p, g.tc, cmp, grid, s);
+  else
+    launch_tc<float2, float2>(p, g.tc, cmp, grid, s);
+  HIPCHK(hipGetLastError());
+  if (pl.splits > 1) {
+    dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536));
+    if (g.tc == F64)
+      hipLaunchKernelGGL(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
+    else
+      hipLaunchKernelGGL(splitk_reduce_kernel<float2>, rg, dim3(256), 0, s, p);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+}  // namespace mpship

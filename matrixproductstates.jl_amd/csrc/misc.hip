@@ -1,0 +1,125 @@
+// Small device kernels: boundary-coefficient writes, norms, rescaling, copies.
+#include "common.h"
+
+namespace mpship {
+
+__global__ void set_scalar_kernel(double2* p, double re, double im) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = make_double2(re, im);
+}
+
+void launch_set_scalar(void* p, double re, double im, hipStream_t s) {
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, s, (double2*)p, re, im);
+  HIPCHK(hipGetLastError());
+}
+
+__global__ void fill_zero_kernel(double2* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+
+void launch_fill_zero(void* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(g), dim3(256), 0, s, (double2*)p, n);
+  HIPCHK(hipGetLastError());
+}
+
+// sum |p_i|^2 with a fixed reduction order (one workgroup) -> out[0]
+__global__ __launch_bounds__(1024) void sum_abs2_kernel(const double2* p, int64_t n, double* out) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) {
+    const double2 v = p[i];
+    s += v.x * v.x + v.y * v.y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += sh[w];
+    out[0] = t;
+  }
+}
+
+void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(sum_abs2_kernel, dim3(1), dim3(1024), 0, s, (const double2*)p, n, out);
+  HIPCHK(hipGetLastError());
+}
+
+// p <- p / sqrt(norm2[0])   (normalize_lo!, orthonormalisation.jl:97-103)
+__global__ void scale_rsqrt_kernel(double2* p, int64_t n, const double* norm2) {
+  const double f = 1.0 / sqrt(norm2[0]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = p[i];
+    p[i] = make_double2(v.x * f, v.y * f);
+  }
+}
+
+void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s) {
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(scale_rsqrt_kernel, dim3(g), dim3(256), 0, s, (double2*)p, n, norm2);
+  HIPCHK(hipGetLastError());
+}
+
+__global__ void copy_c_kernel(double2* d, const double2* s, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(copy_c_kernel, dim3(g), dim3(256), 0, s, (double2*)dst, (const double2*)src, n);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename TD, typename TS>
+__global__ void convert_kernel(TD* d, const TS* s, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    d[i].x = s[i].x;
+    d[i].y = s[i].y;
+  }
+}
+
+void launch_convert(void* dst, DType td, const void* src, DType ts, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  if (td == F32 && ts == F64)
+    hipLaunchKernelGGL((convert_kernel<float2, double2>), dim3(g), dim3(256), 0, s, (float2*)dst,
+                       (const double2*)src, n);
+  else if (td == F64 && ts == F32)
+    hipLaunchKernelGGL((convert_kernel<double2, float2>), dim3(g), dim3(256), 0, s, (double2*)dst,
+                       (const float2*)src, n);
+  else if (td == F64)
+    hipLaunchKernelGGL((convert_kernel<double2, double2>), dim3(g), dim3(256), 0, s, (double2*)dst,
+                       (const double2*)src, n);
+  else
+    hipLaunchKernelGGL((convert_kernel<float2, float2>), dim3(g), dim3(256), 0, s, (float2*)dst,
+                       (const float2*)src, n);
+  HIPCHK(hipGetLastError());
+}
+
+// out = trace(M) for a d x d column-major complex matrix
+__global__ void trace_kernel(const double2* M, int64_t d, double2* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double r = 0.0, i = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+      r += M[k * d + k].x;
+      i += M[k * d + k].y;
+    }
+    *out = make_double2(r, i);
+  }
+}
+
+void launch_trace_small(const void* M, int64_t d, void* out, hipStream_t s) {
+  hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(64), 0, s, (const double2*)M, d, (double2*)out);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace mpship

@@ -1,0 +1,49 @@
+// L2/L3 algorithms on device-resident containers.
+#pragma once
+#include "contract.h"
+#include <complex>
+#include <initializer_list>
+
+namespace mpship {
+
+// Scratch carved for one high-level call, sized from the largest bond,
+// local and MPO dimensions of the containers involved.
+struct Plan {
+  Ctx* ctx;
+  Scratch scr;
+  void *T1 = nullptr, *T2 = nullptr, *T3 = nullptr, *AL = nullptr, *tmp = nullptr;
+  std::vector<void*> ovl;
+  void *tau = nullptr, *R = nullptr, *envA = nullptr, *envB = nullptr;
+  void* splitk = nullptr;
+  size_t splitk_bytes = 0;
+  Work work{};
+  Op op{};
+  Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
+       int n_ovl);
+  Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl); }
+  void init(i64 Dm, i64 dm, i64 wm, int n_ovl);
+};
+
+void qr_left(Plan& p, State* s, int j, void* Rout);
+void qr_right(Plan& p, State* s, int j, void* Rout);
+void left_orthogonalize(Plan& p, State* s, int slot);
+void right_orthogonalize(Plan& p, State* s, int slot);
+double read_scalar(Ctx* c, int slot);
+
+void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H);
+void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
+                                   const std::complex<double>* coef, State** A, MPO* H);
+void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
+                                    const std::complex<double>* coef, State** A, MPO* H);
+void compress_var_apply_H(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H, int maxsweeps);
+void compress_var(Plan& p, State* A0, State* A, Env* env, int maxsweeps);
+void compress_sum_var(Plan& p, State* out, State* Ain, int numv, const std::complex<double>* coef,
+                      State** A, int maxsweeps, std::vector<Env*>& FL);
+void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const std::complex<double>* coef,
+                              State** A, MPO* H, int maxsweeps, std::vector<Env*>& env);
+void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
+              State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop);
+void scal_prod(Plan& p, State* A, State* B, int mode, int slot);
+void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot);
+
+}  // namespace mpship

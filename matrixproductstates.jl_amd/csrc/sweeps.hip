@@ -1,0 +1,506 @@
+// L2 gauge moves and L3 sweeps / time steppers on device-resident states.
+//
+// Each routine follows the reference's schedule line by line (cited per
+// function); every contraction is a contract.hip GEMM launch, every QR the
+// in-place Householder kernel, and boundary coefficients are written by a
+// one-thread kernel, so a whole RK4 step is a fixed sequence of launches on
+// the context stream (capturable as one hipGraph).
+#include "contract.h"
+#include "sweeps.h"
+#include <cmath>
+#include <complex>
+
+namespace mpship {
+
+using cd = std::complex<double>;
+
+// -------------------------------------------------------------- planning
+Plan::Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
+           int n_ovl)
+    : ctx(c), scr(c) {
+  i64 Dm = 1, dm = 1, wm = 1;
+  for (const State* s : states)
+    if (s) {
+      Dm = std::max(Dm, s->max_dim());
+      for (int j = 0; j < s->N; ++j) dm = std::max(dm, s->d[j]);
+    }
+  for (const MPO* o : mpos)
+    if (o) wm = std::max(wm, o->max_w());
+  init(Dm, dm, wm, n_ovl);
+}
+
+void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl) {
+  Ctx* c = ctx;
+  const size_t cz = sizeof(double2);
+  const size_t site = (size_t)Dm * dm * Dm;
+  const size_t big = site * wm;
+  scr.add(&T1, big * cz);
+  scr.add(&T2, big * cz);
+  scr.add(&T3, big * cz);
+  scr.add(&AL, site * cz);
+  scr.add(&tmp, site * cz);
+  ovl.assign(std::max(n_ovl, 0), nullptr);
+  for (auto& p : ovl) scr.add(&p, site * cz);
+  scr.add(&tau, (size_t)Dm * dm * cz);
+  scr.add(&R, (size_t)Dm * Dm * dm * cz);
+  scr.add(&envA, big * cz);
+  scr.add(&envB, big * cz);
+  // split-K partials: splits * blocks <= 2*CUs + blocks, blocks < 2*CUs when split
+  const size_t sk = (size_t)(4 * c->num_cus + 64) * 64 * 64 * cz;
+  scr.add(&splitk, sk);
+  splitk_bytes = sk;
+  scr.commit();
+  work.c = c;
+  work.splitk = splitk;
+  work.splitk_bytes = splitk_bytes;
+  work.T1 = T1;
+  op.cmp = (c->prec == 1) ? CMP_F32_3M : CMP_F64;
+  op.tmp = (c->prec == 1) ? F32 : F64;
+}
+
+// ------------------------------------------------------------ gauge moves
+// leftorth! (orthonormalisation.jl:164-171): QR of reshape(A, Dl*d, Dr)
+static void run_qr(Plan& p, void* base, i64 m, i64 n, i64 rs, i64 cs, void* Rout) {
+  Ctx* c = p.ctx;
+  const bool prof = c->profiling && !c->capturing;
+  hipEvent_t a = nullptr;
+  if (prof) c->prof_begin(a);
+  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, nullptr, Rout, c->stream);
+  // zgeqrf + zungqr: 2 x 8 (m n^2 - n^3 / 3) flops; panel read and written once
+  if (prof) c->prof_end(a, 1, 16.0 * ((double)m * n * n - (double)n * n * n / 3.0), 32.0 * (double)m * n);
+}
+
+void qr_left(Plan& p, State* s, int j, void* Rout) {
+  const i64 Dl = s->Dl(j), d = s->d[j], Dr = s->Dr(j);
+  REQUIRE(Dl * d >= Dr, "leftorth!: Dl*d < Dr");
+  run_qr(p, s->site(j), Dl * d, Dr, 1, Dl * d, Rout);
+}
+
+// rightorth! (orthonormalisation.jl:227-234): QR of the plain transpose of
+// reshape(A, Dl, d*Dr); element (r, c) of the transpose is A[c + Dl*r], and
+// Q written through the same view is reshape(Q.', Dl, d, Dr).
+void qr_right(Plan& p, State* s, int j, void* Rout) {
+  const i64 Dl = s->Dl(j), d = s->d[j], Dr = s->Dr(j);
+  REQUIRE(d * Dr >= Dl, "rightorth!: d*Dr < Dl");
+  run_qr(p, s->site(j), d * Dr, Dl, Dl, 1, Rout);
+}
+
+// leftorthogonalizeQR! (orthonormalisation.jl:43-56); norm^2 left in dscal[slot]
+void left_orthogonalize(Plan& p, State* s, int slot) {
+  for (int i = 0; i + 1 < s->N; ++i) {
+    qr_left(p, s, i, p.R);
+    // mps[i+1] = prod_LR(r, mps[i+1]) : r is Dr x Dr
+    const i64 Dr = s->Dr(i), d1 = s->d[i + 1], Dr1 = s->Dr(i + 1);
+    prod_LR_23(p.work, p.tmp, p.R, s->site(i + 1), Dr, Dr, d1, Dr1, false, false);
+    launch_copy_c(s->site(i + 1), p.tmp, Dr * d1 * Dr1, p.ctx->stream);
+  }
+  const int n = s->N - 1;
+  launch_sum_abs2(s->site(n), s->size(n), p.ctx->dscal + slot, p.ctx->stream);
+}
+
+// rightorthogonalizeQR! (orthonormalisation.jl:77-90)
+void right_orthogonalize(Plan& p, State* s, int slot) {
+  for (int i = s->N - 1; i >= 1; --i) {
+    qr_right(p, s, i, p.R);
+    // mps[i-1] = prod_LR(mps[i-1], C.') with C = R (Dl x Dl) -> multiply by R^T
+    const i64 Dl = s->Dl(i), Dlm = s->Dl(i - 1), dm = s->d[i - 1];
+    prod_LR_32(p.work, p.tmp, s->site(i - 1), p.R, Dlm, dm, Dl, Dl, false, true);
+    launch_copy_c(s->site(i - 1), p.tmp, Dlm * dm * Dl, p.ctx->stream);
+  }
+  launch_sum_abs2(s->site(0), s->size(0), p.ctx->dscal + slot, p.ctx->stream);
+}
+
+double read_scalar(Ctx* c, int slot) {
+  HIPCHK(hipMemcpyAsync(c->hscal, c->dscal + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  return c->hscal[0];
+}
+
+// ---------------------------------------------------------------- helpers
+static void set1(Ctx* c, double2* p, cd v) { launch_set_scalar(p, v.real(), v.imag(), c->stream); }
+
+static void check_same_dims(const State* a, const State* b, const char* what) {
+  REQUIRE(a->N == b->N, std::string(what) + ": chain lengths differ");
+  for (int j = 0; j < a->N; ++j) REQUIRE(a->d[j] == b->d[j], std::string(what) + ": local dims differ");
+}
+
+static void check_env(const Env* e, const State* s, bool op, const char* what) {
+  REQUIRE(e->op == op, std::string(what) + ": wrong environment kind");
+  REQUIRE(e->N == s->N, std::string(what) + ": environment length mismatch");
+}
+
+// ------------------------------------------------------------ L3 sweeps
+// op-env left initialisation shared by the _left sweeps:
+//   envop[1][1] = 1; update_le!(envop[jj+1], Ain[jj], H[jj], A[jj], envop[jj])
+static void init_left_op(Plan& p, Env* envop, State* Ain, State* A, MPO* H, cd right_coef) {
+  Ctx* c = p.ctx;
+  const int N = Ain->N;
+  set1(c, envop->slot(0), 1.0);
+  for (int j = 0; j + 1 < N; ++j) {
+    // AOL = prod_AOL(A[j], H[j], envop[j]); envop[j+1] = update_le(Ain[j], AOL)
+    prod_AOL(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
+             H->wr(j), envop->slot(j), Ain->Dl(j));
+    update_le(p.work, p.op, envop->slot(j + 1), p.T2, Ain->site(j), Ain->Dl(j), Ain->d[j], A->Dr(j),
+              H->wr(j), Ain->Dr(j));
+  }
+  set1(c, envop->slot(N), right_coef);
+}
+
+// op-env right initialisation shared by the _right sweeps:
+//   envop[na+1][1] = 1; update_re!(envop[jj], Ain[jj], H[jj], A[jj], envop[jj+1])
+static void init_right_op(Plan& p, Env* envop, State* Ain, State* A, MPO* H, cd left_coef) {
+  Ctx* c = p.ctx;
+  const int N = Ain->N;
+  set1(c, envop->slot(N), 1.0);
+  for (int j = N - 1; j >= 1; --j) {
+    prod_AOR(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
+             H->wr(j), envop->slot(j + 1), Ain->Dr(j));
+    update_re(p.work, p.op, envop->slot(j), p.T2, Ain->site(j), A->Dl(j), H->wl(j), A->d[j],
+              Ain->Dr(j), Ain->Dl(j));
+  }
+  set1(c, envop->slot(0), left_coef);
+}
+
+// compress_var_apply_H_left! (compression.jl:321-347)
+void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H) {
+  const int N = Ain->N;
+  init_left_op(p, envop, Ain, A, H, 1.0);
+  for (int j = N - 1; j >= 0; --j) {
+    // AOR = prod_AOR(A[jj], H[jj], envop[jj+1]); prod_LR!(A0[jj], envop[jj], AOR)
+    prod_AOR(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
+             H->wr(j), envop->slot(j + 1), A0->Dr(j));
+    prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, A->Dl(j), H->wl(j), A0->Dl(j), A->d[j],
+               A0->Dr(j), false);
+    if (j == 0) break;
+    qr_right(p, A0, j, nullptr);
+    update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), A->Dl(j), H->wl(j), A->d[j], A0->Dr(j),
+              A0->Dl(j));
+  }
+}
+
+// compress_sum_var_apply_H_left! (compression.jl:889-944)
+void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
+                                   const cd* coef, State** A, MPO* H) {
+  Ctx* c = p.ctx;
+  const int N = Ain->N;
+  for (int ll = 0; ll + 1 < numv; ++ll) {
+    Env* e = env[ll];
+    set1(c, e->slot(0), 1.0);
+    for (int j = 0; j + 1 < N; ++j) {
+      // update_lenv!(env[ll][jj+1], Ain[jj], A[ll][jj], env[ll][jj])
+      prod_LR_23(p.work, p.AL, e->slot(j), A[ll]->site(j), Ain->Dl(j), A[ll]->Dl(j), A[ll]->d[j],
+                 A[ll]->Dr(j), false, false);
+      update_lenv(p.work, e->slot(j + 1), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
+                  A[ll]->Dr(j), 0);
+    }
+    set1(c, e->slot(N), coef[ll]);
+  }
+  State* Ah = A[numv - 1];
+  init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  for (int j = N - 1; j >= 0; --j) {
+    const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+    // overlap terms: AR = prod_LR(A[ll][jj], env[ll][jj+1]); A0 (+)= prod_LR(env[ll][jj], AR)
+    for (int ll = 0; ll + 1 < numv; ++ll) {
+      prod_LR_32(p.work, p.ovl[ll], A[ll]->site(j), env[ll]->slot(j + 1), A[ll]->Dl(j), d, A[ll]->Dr(j),
+                 Dr0, false);
+      prod_LR_23(p.work, A0->site(j), env[ll]->slot(j), p.ovl[ll], Dl0, A[ll]->Dl(j), d, Dr0, false,
+                 ll > 0);
+    }
+    // operator term: AOR = prod_AOR(A[numv][jj], H[jj], envop[jj+1]); A0 += prod_LR(envop[jj], AOR)
+    prod_AOR(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
+             envop->slot(j + 1), Dr0);
+    prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, Ah->Dl(j), H->wl(j), Dl0, d, Dr0,
+               numv > 1);
+    if (j == 0) break;
+    qr_right(p, A0, j, nullptr);
+    for (int ll = 0; ll + 1 < numv; ++ll)
+      update_renv(p.work, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
+    update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
+  }
+}
+
+// compress_sum_var_apply_H_right! (compression.jl:1007-1063).  A0 may alias A[0].
+void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
+                                    const cd* coef, State** A, MPO* H) {
+  Ctx* c = p.ctx;
+  const int N = Ain->N;
+  for (int ll = 0; ll + 1 < numv; ++ll) {
+    Env* e = env[ll];
+    set1(c, e->slot(N), 1.0);
+    for (int j = N - 1; j >= 1; --j) {
+      // update_renv!(env[ll][jj], Ain[jj], A[ll][jj], env[ll][jj+1])
+      prod_LR_32(p.work, p.AL, A[ll]->site(j), e->slot(j + 1), A[ll]->Dl(j), A[ll]->d[j], A[ll]->Dr(j),
+                 Ain->Dr(j), false);
+      update_renv(p.work, e->slot(j), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
+                  A[ll]->Dl(j));
+    }
+    set1(c, e->slot(0), coef[ll]);
+  }
+  State* Ah = A[numv - 1];
+  init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  for (int j = 0; j < N; ++j) {
+    const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+    // overlap terms first: AL[ll] reads A[ll][jj] before A0[jj] is written
+    for (int ll = 0; ll + 1 < numv; ++ll)
+      prod_LR_23(p.work, p.ovl[ll], env[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
+                 false, false);
+    for (int ll = 0; ll + 1 < numv; ++ll)
+      prod_LR_32(p.work, A0->site(j), p.ovl[ll], env[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0, ll > 0);
+    // operator term: AOL = prod_AOL(A[numv][jj], H[jj], envop[jj]); A0 += prod_LR(AOL, envop[jj+1])
+    prod_AOL(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
+             envop->slot(j), Dl0);
+    prod_LR_43(p.work, p.op, A0->site(j), p.T2, envop->slot(j + 1), Dl0, d, Ah->Dr(j), H->wr(j), Dr0,
+               numv > 1);
+    if (j == N - 1) break;
+    qr_left(p, A0, j, nullptr);
+    for (int ll = 0; ll + 1 < numv; ++ll)
+      update_lenv(p.work, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
+    update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
+  }
+}
+
+// compress_var_apply_H! (compression.jl:217-258)
+void compress_var_apply_H(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H, int maxsweeps) {
+  const int N = Ain->N;
+  init_left_op(p, envop, Ain, A, H, 1.0);
+  for (int sw = 0; sw < maxsweeps; ++sw) {
+    for (int j = N - 1; j >= 1; --j) {
+      prod_AOR(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
+               H->wr(j), envop->slot(j + 1), A0->Dr(j));
+      prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, A->Dl(j), H->wl(j), A0->Dl(j), A->d[j],
+                 A0->Dr(j), false);
+      qr_right(p, A0, j, nullptr);
+      update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), A->Dl(j), H->wl(j), A->d[j],
+                A0->Dr(j), A0->Dl(j));
+    }
+    for (int j = 0; j < N; ++j) {
+      prod_AOL(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
+               H->wr(j), envop->slot(j), A0->Dl(j));
+      prod_LR_43(p.work, p.op, A0->site(j), p.T2, envop->slot(j + 1), A0->Dl(j), A->d[j], A->Dr(j),
+                 H->wr(j), A0->Dr(j), false);
+      if (j == N - 1) break;
+      qr_left(p, A0, j, nullptr);
+      update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), A0->Dl(j), A->d[j], A->Dr(j),
+                H->wr(j), A0->Dr(j));
+    }
+  }
+}
+
+// compress_var! (compression.jl:65-103); env holds L[ket,bra] on the left
+// (update_lenv_co!) and R[ket,bra] on the right.
+void compress_var(Plan& p, State* A0, State* A, Env* env, int maxsweeps) {
+  Ctx* c = p.ctx;
+  const int N = A0->N;
+  set1(c, env->slot(0), 1.0);
+  for (int j = 0; j + 1 < N; ++j) {
+    // update_lenv_co!(env[jj+1], A0[jj], A[jj], env[jj]): AL = prod_LR_co(env[jj], A[jj])
+    prod_LR_23(p.work, p.AL, env->slot(j), A->site(j), A0->Dl(j), A->Dl(j), A->d[j], A->Dr(j), true,
+               false);
+    update_lenv(p.work, env->slot(j + 1), A0->site(j), p.AL, A0->Dl(j), A0->d[j], A0->Dr(j), A->Dr(j), 1);
+  }
+  set1(c, env->slot(N), 1.0);
+  for (int sw = 0; sw < maxsweeps; ++sw) {
+    for (int j = N - 1; j >= 0; --j) {
+      const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+      // AR = prod_LR(A[jj], env[jj+1]); prod_LR_co!(A0[jj], env[jj], AR)  (jj=1: prod_LR!)
+      prod_LR_32(p.work, p.AL, A->site(j), env->slot(j + 1), A->Dl(j), d, A->Dr(j), Dr0, false);
+      prod_LR_23(p.work, A0->site(j), env->slot(j), p.AL, Dl0, A->Dl(j), d, Dr0, j > 0, false);
+      if (j == 0) break;
+      qr_right(p, A0, j, nullptr);
+      update_renv(p.work, env->slot(j), A0->site(j), p.AL, Dl0, d, Dr0, A->Dl(j));
+    }
+    for (int j = 0; j < N; ++j) {
+      const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+      // AL = prod_LR_co(env[jj], A[jj]); prod_LR!(A0[jj], AL, env[jj+1])
+      prod_LR_23(p.work, p.AL, env->slot(j), A->site(j), Dl0, A->Dl(j), d, A->Dr(j), true, false);
+      prod_LR_32(p.work, A0->site(j), p.AL, env->slot(j + 1), Dl0, d, A->Dr(j), Dr0, false);
+      if (j == N - 1) break;
+      qr_left(p, A0, j, nullptr);
+      update_lenv(p.work, env->slot(j + 1), A0->site(j), p.AL, Dl0, d, Dr0, A->Dr(j), 1);
+    }
+  }
+}
+
+// compress_sum_var (compression.jl:496-565).  out must not alias any A[ll]
+// (the reference allocates a fresh A0); Ain may alias out.
+void compress_sum_var(Plan& p, State* out, State* Ain, int numv, const cd* coef, State** A,
+                      int maxsweeps, std::vector<Env*>& FL) {
+  Ctx* c = p.ctx;
+  const int N = Ain->N;
+  // FL[ll] slots 0..N hold the left envs (slot j for site j) and, from the
+  // right sweep on, the right envs FR[ll][j] in slot j+1 -- as the reference's
+  // FL/FR pair, but sharing storage the way the !-variants do.
+  for (int ll = 0; ll < numv; ++ll) {
+    Env* e = FL[ll];
+    set1(c, e->slot(0), 1.0);
+    for (int j = 0; j + 1 < N; ++j) {
+      prod_LR_23(p.work, p.AL, e->slot(j), A[ll]->site(j), Ain->Dl(j), A[ll]->Dl(j), A[ll]->d[j],
+                 A[ll]->Dr(j), false, false);
+      update_lenv(p.work, e->slot(j + 1), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
+                  A[ll]->Dr(j), 0);
+    }
+    set1(c, e->slot(N), coef[ll]);
+  }
+  // Ain (if distinct) is only read during initialisation; copy it into out.
+  if (out != Ain)
+    for (int j = 0; j < N; ++j) launch_copy_c(out->site(j), Ain->site(j), Ain->size(j), c->stream);
+  for (int sw = 0; sw < maxsweeps; ++sw) {
+    for (int j = N - 1; j >= 1; --j) {
+      const i64 Dl0 = out->Dl(j), Dr0 = out->Dr(j), d = out->d[j];
+      for (int ll = 0; ll < numv; ++ll) {
+        prod_LR_32(p.work, p.ovl[ll], A[ll]->site(j), FL[ll]->slot(j + 1), A[ll]->Dl(j), d, A[ll]->Dr(j),
+                   Dr0, false);
+        prod_LR_23(p.work, out->site(j), FL[ll]->slot(j), p.ovl[ll], Dl0, A[ll]->Dl(j), d, Dr0, false,
+                   ll > 0);
+      }
+      qr_right(p, out, j, nullptr);
+      for (int ll = 0; ll < numv; ++ll)
+        update_renv(p.work, FL[ll]->slot(j), out->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
+    }
+    for (int j = 0; j + 1 < N; ++j) {
+      const i64 Dl0 = out->Dl(j), Dr0 = out->Dr(j), d = out->d[j];
+      for (int ll = 0; ll < numv; ++ll) {
+        prod_LR_23(p.work, p.ovl[ll], FL[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
+                   false, false);
+        prod_LR_32(p.work, out->site(j), p.ovl[ll], FL[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0,
+                   ll > 0);
+      }
+      qr_left(p, out, j, nullptr);
+      for (int ll = 0; ll < numv; ++ll)
+        update_lenv(p.work, FL[ll]->slot(j + 1), out->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
+    }
+  }
+  const int j = N - 1;
+  const i64 Dl0 = out->Dl(j), Dr0 = out->Dr(j), d = out->d[j];
+  for (int ll = 0; ll < numv; ++ll) {
+    prod_LR_23(p.work, p.ovl[ll], FL[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
+               false, false);
+    prod_LR_32(p.work, out->site(j), p.ovl[ll], FL[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0, ll > 0);
+  }
+}
+
+// compress_sum_var_apply_H! (compression.jl:805-887), by intent: overlap envs
+// start at 1 on the left and coef[ll] on the right (as in _left!).
+void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const cd* coef, State** A,
+                              MPO* H, int maxsweeps, std::vector<Env*>& env) {
+  Ctx* c = p.ctx;
+  const int N = A0->N;
+  for (int ll = 0; ll + 1 < numv; ++ll) {
+    Env* e = env[ll];
+    set1(c, e->slot(0), 1.0);
+    for (int j = 0; j + 1 < N; ++j) {
+      prod_LR_23(p.work, p.AL, e->slot(j), A[ll]->site(j), A0->Dl(j), A[ll]->Dl(j), A[ll]->d[j],
+                 A[ll]->Dr(j), false, false);
+      update_lenv(p.work, e->slot(j + 1), A0->site(j), p.AL, A0->Dl(j), A0->d[j], A0->Dr(j), A[ll]->Dr(j),
+                  0);
+    }
+    set1(c, e->slot(N), coef[ll]);
+  }
+  State* Ah = A[numv - 1];
+  init_left_op(p, envop, A0, Ah, H, coef[numv - 1]);
+  for (int sw = 0; sw < maxsweeps; ++sw) {
+    for (int j = N - 1; j >= 1; --j) {
+      const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+      for (int ll = 0; ll + 1 < numv; ++ll) {
+        prod_LR_32(p.work, p.ovl[ll], A[ll]->site(j), env[ll]->slot(j + 1), A[ll]->Dl(j), d,
+                   A[ll]->Dr(j), Dr0, false);
+        prod_LR_23(p.work, A0->site(j), env[ll]->slot(j), p.ovl[ll], Dl0, A[ll]->Dl(j), d, Dr0, false,
+                   ll > 0);
+      }
+      prod_AOR(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
+               envop->slot(j + 1), Dr0);
+      prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, Ah->Dl(j), H->wl(j), Dl0, d, Dr0,
+                 numv > 1);
+      qr_right(p, A0, j, nullptr);
+      for (int ll = 0; ll + 1 < numv; ++ll)
+        update_renv(p.work, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
+      update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
+    }
+    for (int j = 0; j + 1 < N; ++j) {
+      const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+      for (int ll = 0; ll + 1 < numv; ++ll)
+        prod_LR_23(p.work, p.ovl[ll], env[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
+                   false, false);
+      for (int ll = 0; ll + 1 < numv; ++ll)
+        prod_LR_32(p.work, A0->site(j), p.ovl[ll], env[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0,
+                   ll > 0);
+      prod_AOL(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
+               envop->slot(j), Dl0);
+      prod_LR_43(p.work, p.op, A0->site(j), p.T2, envop->slot(j + 1), Dl0, d, Ah->Dr(j), H->wr(j), Dr0,
+                 numv > 1);
+      qr_left(p, A0, j, nullptr);
+      for (int ll = 0; ll + 1 < numv; ++ll)
+        update_lenv(p.work, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
+      update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
+    }
+  }
+  const int j = N - 1;
+  const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
+  for (int ll = 0; ll + 1 < numv; ++ll)
+    prod_LR_23(p.work, p.ovl[ll], env[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
+               false, false);
+  for (int ll = 0; ll + 1 < numv; ++ll)
+    prod_LR_32(p.work, A0->site(j), p.ovl[ll], env[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0, ll > 0);
+  prod_AOL(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
+           envop->slot(j), Dl0);
+  prod_LR_43(p.work, p.op, A0->site(j), p.T2, envop->slot(j + 1), Dl0, d, Ah->Dr(j), H->wr(j), Dr0,
+             numv > 1);
+}
+
+// RK4stp_apply_H_half! (timeevolution.jl:178-196)
+void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
+              State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop) {
+  compress_var_apply_H_left(p, psi1, envop, psi, psi, H1);
+  {
+    Env* e[1] = {env1};
+    State* A[2] = {psi, psi1};
+    cd coef[2] = {1.0, dt / 2};
+    compress_sum_var_apply_H_right(p, psi2, e, envop, psi1, 2, coef, A, H2);
+  }
+  {
+    Env* e[1] = {env1};
+    State* A[2] = {psi, psi2};
+    cd coef[2] = {1.0, dt};
+    compress_sum_var_apply_H_left(p, psi3, e, envop, psi2, 2, coef, A, H2);
+  }
+  {
+    Env* e[3] = {env1, env2, env3};
+    State* A[4] = {psi, psi1, psi2, psi3};
+    cd coef[4] = {-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3};
+    compress_sum_var_apply_H_right(p, out, e, envop, psi3, 4, coef, A, H3);
+  }
+}
+
+// ------------------------------------------------------ scalar products
+// scal_prod / scal_prod_no_conj (MatrixProductStates.jl:345-379) -> dscal[slot..slot+1]
+void scal_prod(Plan& p, State* A, State* B, int mode, int slot) {
+  Ctx* c = p.ctx;
+  double2* cur = (double2*)p.envA;
+  double2* nxt = (double2*)p.envB;
+  set1(c, cur, 1.0);
+  for (int j = 0; j < A->N; ++j) {
+    // AL = prod_LR(lenv, B[j]); lenv = update_lenv(A[j], AL)
+    prod_LR_23(p.work, p.AL, cur, B->site(j), A->Dl(j), B->Dl(j), B->d[j], B->Dr(j), false, false);
+    update_lenv(p.work, nxt, A->site(j), p.AL, A->Dl(j), A->d[j], A->Dr(j), B->Dr(j), mode);
+    std::swap(cur, nxt);
+  }
+  launch_copy_c(c->dscal + slot, cur, 1, c->stream);
+}
+
+// scal_op_prod (MatrixProductStates.jl:327-340): conj(<A|O|B>) -> dscal[slot..]
+void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot) {
+  Ctx* c = p.ctx;
+  double2* cur = (double2*)p.envA;
+  double2* nxt = (double2*)p.envB;
+  set1(c, cur, 1.0);
+  for (int j = 0; j < A->N; ++j) {
+    // lenv = update_le(A[j], O[j], B[j], lenv)
+    prod_AOL(p.work, p.op, p.T2, B->site(j), B->Dl(j), B->d[j], B->Dr(j), O->site(j), O->wl(j),
+             O->wr(j), cur, A->Dl(j));
+    update_le(p.work, p.op, nxt, p.T2, A->site(j), A->Dl(j), A->d[j], B->Dr(j), O->wr(j), A->Dr(j));
+    std::swap(cur, nxt);
+  }
+  launch_copy_c(c->dscal + slot, cur, 1, c->stream);
+}
+
+}  // namespace mpship

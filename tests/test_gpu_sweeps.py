@@ -1,0 +1,319 @@
+"""GPU parity of the sweeps, RK4, gauge moves and scalar products against the oracle.
+
+Inputs are seeded random left-canonical states (mpsrandom), product states and
+the spin / Rydberg model MPOs; the device result must match the FP64 oracle
+to 1e-10 relative (observables, dense states) on the same inputs.
+"""
+import numpy as np
+import pytest
+
+import mpsamd as M
+from mpsamd import models
+from oracle import mps_oracle as O
+from oracle import dense as Dn
+from helpers import rel, copy_sites, random_mpo, crandn
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def dev_state(sites):
+    return M.MPS.from_host(sites)
+
+
+def dev_sites(m):
+    return m.to_host()
+
+
+def env_dims(dims):
+    return list(dims)
+
+
+@pytest.mark.parametrize("N,D,d", [(6, 8, 2), (8, 16, 2), (5, 9, 3)])
+def test_compress_var_apply_H_left(N, D, d, rng):
+    print("case", N, D, d, flush=True)
+    psi, dims = O.mpsrandom(N, D, d, rng)
+    H = random_mpo(rng, N, d, 3)
+    A0 = copy_sites(psi)
+    envop = O.build_env_op(dims, dims, [1] + [3] * (N - 1) + [1])
+    O.compress_var_apply_H_left_(A0, envop, psi, psi, H)
+
+    g_psi = dev_state(psi)
+    g_A0 = dev_state(psi)
+    g_H = M.MPO(H)
+    g_env = M.build_env(dims, dims, [1] + [3] * (N - 1) + [1])
+    M.compress_var_apply_H_left_(g_A0, g_env, g_psi, g_psi, g_H)
+    got = dev_sites(g_A0)
+    for j in range(N):
+        assert rel(got[j], A0[j]) < TOL, j
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("numv", [2, 3, 4])
+def test_compress_sum_var_apply_H(side, numv, rng):
+    N, D, d, w = 7, 8, 2, 4
+    print("case", side, numv, flush=True)
+    states = [O.mpsrandom(N, D, d, rng)[0] for _ in range(numv)]
+    Ain, dims = O.mpsrandom(N, D, d, rng)
+    if side == "right":   # Ain right canonical for the _right sweep
+        O.rightorthonormalizationQR_(Ain)
+    H = random_mpo(rng, N, d, w)
+    coef = list(crandn(rng, numv))
+    wd = [1] + [w] * (N - 1) + [1]
+    A0 = copy_sites(Ain)
+    env = [O.build_env(dims, dims) for _ in range(numv - 1)]
+    envop = O.build_env_op(dims, dims, wd)
+    fn = O.compress_sum_var_apply_H_left_ if side == "left" else O.compress_sum_var_apply_H_right_
+    fn(A0, env, envop, Ain, coef, states, H)
+
+    g_states = [dev_state(s) for s in states]
+    g_Ain = dev_state(Ain)
+    g_A0 = dev_state(Ain)
+    g_env = [M.build_env(dims, dims) for _ in range(numv - 1)]
+    g_envop = M.build_env(dims, dims, wd)
+    gfn = M.compress_sum_var_apply_H_left_ if side == "left" else M.compress_sum_var_apply_H_right_
+    gfn(g_A0, g_env, g_envop, g_Ain, coef, g_states, M.MPO(H))
+    got = dev_sites(g_A0)
+    for j in range(N):
+        assert rel(got[j], A0[j]) < TOL, j
+
+
+def test_sum_var_requires_an_overlap_term(rng):
+    # the reference indexes A[1] unconditionally (compression.jl:918, 1036): numv = 1 is a BoundsError
+    N, D, d = 5, 4, 2
+    s, dims = O.mpsrandom(N, D, d, rng)
+    H = random_mpo(rng, N, d, 2)
+    wd = [1] + [2] * (N - 1) + [1]
+    with pytest.raises(IndexError):
+        O.compress_sum_var_apply_H_left_(copy_sites(s), [], O.build_env_op(dims, dims, wd), s, [1.0], [s], H)
+    g = dev_state(s)
+    with pytest.raises(M.DimensionMismatch):
+        M.compress_sum_var_apply_H_left_(dev_state(s), [], M.build_env(dims, dims, wd), g, [1.0], [g], M.MPO(H))
+
+
+def _spin_setup(N, D, dt, init, seed=1):
+    z = models.positions(N, 20.0, seed)
+    H = models.spin_rk4_mpos(z, 0.0, dt)
+    if init == "product":
+        psi, dims = O.mpsproductstate(N, D, 2, np.array([1, 0], dtype=complex))
+    else:
+        psi, dims = O.mpsrandom(N, D, 2, np.random.default_rng(seed + 100))
+    return z, H, psi, dims
+
+
+def _oracle_rk4(H, psi, dims, dt, steps, wop, eleven=False):
+    psi = copy_sites(psi)
+    envop = O.build_env_op(dims, dims, wop)
+    e1, e2, e3 = (O.build_env(dims, dims) for _ in range(3))
+    p1, p2, p3 = copy_sites(psi), copy_sites(psi), copy_sites(psi)
+    norms = []
+    for _ in range(steps):
+        if eleven:
+            out = copy_sites(psi)
+            O.RK4stp_apply_H_half_(dt, H, psi, p1, p2, p3, e1, e2, e3, envop, psi0=out)
+            psi = out
+        else:
+            O.RK4stp_apply_H_half_(dt, H, psi, p1, p2, p3, e1, e2, e3, envop)
+        norms.append(O.normalize_lo_(psi))
+    return psi, norms
+
+
+def _device_rk4(H, psi, dims, dt, steps, wop, eleven=False, graphs=True):
+    ctx = M.default_context()
+    ctx.set_graphs(graphs)
+    gH = [M.MPO(h) for h in H]
+    g = dev_state(psi)
+    g1, g2, g3 = dev_state(psi), dev_state(psi), dev_state(psi)
+    e1, e2, e3 = (M.build_env(dims, dims) for _ in range(3))
+    eop = M.build_env(dims, dims, wop)
+    out = dev_state(psi) if eleven else None
+    norms = []
+    for _ in range(steps):
+        if eleven:
+            M.RK4stp_apply_H_half_(dt, gH, out, g, g1, g2, g3, e1, e2, e3, eop)
+            g.copy_from(out)
+        else:
+            M.RK4stp_apply_H_half_(dt, gH, g, g1, g2, g3, e1, e2, e3, eop)
+        norms.append(M.normalize_lo_(g))
+    ctx.set_graphs(True)
+    return g, norms
+
+
+@pytest.mark.parametrize("eleven", [False, True])
+@pytest.mark.parametrize("graphs", [False, True])
+def test_rk4_random_state(eleven, graphs):
+    N, D, dt, steps = 8, 8, 0.01, 4
+    print("case", eleven, graphs, flush=True)
+    z, H, psi, dims = _spin_setup(N, D, dt, "random")
+    wop = [1] + [4] * (N - 1) + [1]
+    ref, rn = _oracle_rk4(H, psi, dims, dt, steps, wop, eleven)
+    g, gn = _device_rk4(H, psi, dims, dt, steps, wop, eleven, graphs)
+    assert np.allclose(gn, rn, rtol=TOL, atol=0)
+    got = g.to_host()
+    assert rel(Dn.mps_to_dense(got), Dn.mps_to_dense(ref)) < TOL
+
+
+def test_rk4_product_state_full_bond_dimension():
+    # N=8, D=16 = full dimension: the one-site projection is exact and the
+    # zero-padded QR completions are immaterial.
+    N, D, dt, steps = 8, 16, 0.01, 5
+    z, H, psi, dims = _spin_setup(N, D, dt, "product")
+    wop = [1] + [4] * (N - 1) + [1]
+    ref, rn = _oracle_rk4(H, psi, dims, dt, steps, wop)
+    g, gn = _device_rk4(H, psi, dims, dt, steps, wop)
+    assert np.allclose(gn, rn, rtol=TOL, atol=0)
+    assert rel(Dn.mps_to_dense(g.to_host()), Dn.mps_to_dense(ref)) < TOL
+
+
+def test_rk4_product_state_truncated_with_rank_cutoff():
+    # Truncated bond dimension from a zero-padded product state: with the
+    # rank cutoff 1e-13 both sides complete Q identically (tau = 0 on
+    # rounding-level columns) and stay within 1e-10 over 30 steps.
+    N, D, dt, steps = 12, 8, 0.01, 30
+    z, H, psi, dims = _spin_setup(N, D, dt, "product")
+    wop = [1] + [4] * (N - 1) + [1]
+    O.set_qr_rank_tol(1e-13)
+    ctx = M.default_context()
+    ctx.set_qr_rank_tol(1e-13)
+    try:
+        ref, rn = _oracle_rk4(H, psi, dims, dt, steps, wop)
+        g, gn = _device_rk4(H, psi, dims, dt, steps, wop)
+    finally:
+        ctx.set_qr_rank_tol(0.0)
+    assert np.allclose(gn, rn, rtol=TOL, atol=0)
+    ez_ref = _sz(ref)
+    ez_dev = M.measure_excitations_(None, g, models.HEE2)
+    assert np.abs(ez_dev - ez_ref).max() < TOL
+
+
+def _sz(psi):
+    N = len(psi)
+    env = np.ones((1, 1), dtype=complex)
+    ep = np.zeros(N)
+    for n in range(N - 1, -1, -1):
+        Aenv = O.prod_LR_32(psi[n], env)
+        ep[n] = O.local_exp(Aenv, psi[n], models.HEE2).real
+        env = O.update_renv(psi[n], Aenv)
+    return ep
+
+
+def test_rk4_rydberg_density_matrix():
+    N, D, dt, steps = 5, 12, 0.005, 3
+    p = models.RydbergParams(n_exp=2)
+    z = models.positions(N, 5.0, 3)
+    L = models.rydberg_rk4_mpos(p, z, dt)
+    rho, dims = O.mpsrandom(N, D, 9, np.random.default_rng(5))
+    wop = [1] + [p.w] * (N - 1) + [1]
+    ref, rn = _oracle_rk4(L, rho, dims, dt, steps, wop)
+    g, gn = _device_rk4(L, rho, dims, dt, steps, wop)
+    assert np.allclose(gn, rn, rtol=TOL, atol=0)
+    I = models.identity_vec_mps(N)
+    gI = dev_state(I)
+    assert abs(M.scal_prod_no_conj(gI, g) - O.scal_prod_no_conj(I, ref)) < TOL
+
+
+def test_scalar_products_and_observables(rng):
+    N, D, d = 7, 6, 3
+    a, dims = O.mpsrandom(N, D, d, rng)
+    b, _ = O.mpsrandom(N, D, d, rng)
+    H = random_mpo(rng, N, d, 3)
+    ga, gb, gH = dev_state(a), dev_state(b), M.MPO(H)
+    assert abs(M.scal_prod(ga, gb) - O.scal_prod(a, b)) < TOL
+    assert abs(M.scal_prod_no_conj(ga, gb) - O.scal_prod_no_conj(a, b)) < TOL
+    assert abs(M.scal_op_prod(ga, gH, gb) - O.scal_op_prod(a, H, b)) < TOL
+    op = crandn(rng, (d, d))
+    op = op + op.conj().T
+    assert np.abs(M.measure_excitations_(None, ga, op) - _local(a, op)).max() < TOL
+    M.apply_site_operator_(ga, 3, op)
+    O.apply_site_operator_(a[3], op)
+    assert rel(ga.site(3), a[3]) < TOL
+
+
+def _local(psi, op):
+    N = len(psi)
+    env = np.ones((1, 1), dtype=complex)
+    ep = np.zeros(N)
+    for n in range(N - 1, -1, -1):
+        Aenv = O.prod_LR_32(psi[n], env)
+        ep[n] = O.local_exp(Aenv, psi[n], op).real
+        env = O.update_renv(psi[n], Aenv)
+    return ep
+
+
+def test_gauge_moves(rng):
+    N, D, d = 6, 8, 2
+    a, dims = O.mpsrandom(N, D, d, rng)
+    a[-1] = a[-1] * 1.7
+    # scramble the gauge: non-canonical input
+    a = [s + 0.1 * crandn(rng, s.shape) for s in a]
+    g = dev_state(a)
+    no_ref = O.leftorthonormalizationQR_(copy_sites(a))
+    assert abs(M.leftorthonormalizationQR_(g) - no_ref) < TOL * no_ref
+    ref = copy_sites(a)
+    O.leftorthonormalizationQR_(ref)
+    assert rel(Dn.mps_to_dense(g.to_host()), Dn.mps_to_dense(ref)) < TOL
+    g2 = dev_state(a)
+    no_r = O.rightorthonormalizationQR_(copy_sites(a))
+    assert abs(M.rightorthonormalizationQR_(g2) - no_r) < TOL * no_r
+    g3 = dev_state(a)
+    assert abs(M.norm_lo(g3) - O.norm_lo(a)) < TOL
+    assert abs(M.normalize_lo_(g3) - O.normalize_lo_(copy_sites(a))) < TOL
+
+
+def test_product_state_constructor():
+    N, D, d = 9, 8, 3
+    st = np.array([0.6, 0.8j, 0.0])
+    ref, rdims = O.mpsproductstate(N, D, d, st)
+    g, gdims = M.mpsproductstate(N, D, d, st)
+    assert list(gdims) == list(rdims)
+    got = g.to_host()
+    for j in range(N):
+        assert rel(got[j], ref[j]) < 1e-14
+
+
+def test_compress_var_and_sum_var(rng):
+    N, D, d = 7, 8, 2
+    target, dims = O.mpsrandom(N, D, d, rng)
+    guess, _ = O.mpsrandom(N, D, d, rng)
+    A0 = copy_sites(guess)
+    env = O.build_env(dims, dims)
+    O.compress_var_(A0, target, env, 2)
+    gA0, gt = dev_state(guess), dev_state(target)
+    M.compress_var_(gA0, gt, M.build_env(dims, dims), 2)
+    assert rel(Dn.mps_to_dense(gA0.to_host()), Dn.mps_to_dense(A0)) < TOL
+
+    vecs = [O.mpsrandom(N, D, d, rng)[0] for _ in range(3)]
+    coef = [0.3, -0.2 + 0.1j, 1.1]
+    ref = O.compress_sum_var(guess, coef, vecs, 2)
+    got = M.compress_sum_var(dev_state(guess), coef, [dev_state(v) for v in vecs], 2)
+    assert rel(Dn.mps_to_dense(got.to_host()), Dn.mps_to_dense(ref)) < TOL
+
+
+def test_compress_var_apply_H_full_sweeps(rng):
+    N, D, d = 6, 8, 2
+    psi, dims = O.mpsrandom(N, D, d, rng)
+    H = random_mpo(rng, N, d, 2)
+    wd = [1] + [2] * (N - 1) + [1]
+    A0 = copy_sites(psi)
+    O.compress_var_apply_H_(A0, O.build_env_op(dims, dims, wd), psi, psi, H, 2)
+    g = dev_state(psi)
+    gA0 = dev_state(psi)
+    M.compress_var_apply_H_(gA0, M.build_env(dims, dims, wd), g, g, M.MPO(H), 2)
+    assert rel(Dn.mps_to_dense(gA0.to_host()), Dn.mps_to_dense(A0)) < TOL
+
+
+def test_arnoldi_expm(rng):
+    # real MPO and real state: exp(A) psi by Krylov on the MPS manifold
+    N, D, d, numv = 5, 4, 2, 4
+    psi, dims = O.mpsrandom(N, D, d, rng)
+    psi = [s.real.astype(complex) for s in psi]
+    O.leftorthonormalizationQR_(psi)
+    H = [np.real(h).astype(complex) * 0.3 for h in random_mpo(rng, N, d, 2)]
+    wd = [1] + [2] * (N - 1) + [1]
+    rhov = [copy_sites(psi) for _ in range(numv)]
+    envop = O.build_env_op(dims, dims, wd)
+    cref = O.Arnoldi_expm_(H, rhov, numv, None, envop)
+    grhov = [dev_state(psi) for _ in range(numv)]
+    cgot = M.Arnoldi_expm_(M.MPO(H), grhov, numv, None, M.build_env(dims, dims, wd))
+    assert np.abs(cgot - cref).max() < 1e-9
+    assert rel(Dn.mps_to_dense(grhov[0].to_host()), Dn.mps_to_dense(rhov[0])) < 1e-9

@@ -1,0 +1,121 @@
+"""Model builders against independently built dense operators; C-ABI load and exports (CPU only)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from oracle import dense as Dn
+from mpsamd import models
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_spin_mpo_dense():
+    N = 5
+    z = models.positions(N, 5.0, 3)
+    Om, G, k, delt = 0.5, 1.0, 2 * np.pi, 0.03
+    Hd = Dn.mpo_to_dense(models.spin_mpo(z, Om, G, k, con=1.0, delt=delt))
+    Heff = np.zeros((2 ** N, 2 ** N), dtype=complex)
+    for i in range(N):
+        Heff += -Om / 2 * Dn.site_op(N, 2, i, models.HEG2 + models.HGE2)
+        for j in range(N):
+            Heff += -0.5j * G * np.exp(1j * k * abs(z[i] - z[j])) * (
+                Dn.site_op(N, 2, i, models.HEG2) @ Dn.site_op(N, 2, j, models.HGE2))
+    assert np.abs(Hd - (np.eye(2 ** N) + delt * (-1j * Heff))).max() < 1e-14
+
+
+def _dense_lindbladian(p, z):
+    """L acting on vec(rho): site index s = bra + 3*ket (kron(A,id) acts as A rho)."""
+    N = len(z)
+    D = 3 ** N
+    I = np.eye(D)
+
+    def op(j, o):
+        return Dn.site_op(N, 3, j, o)
+
+    H = np.zeros((D, D), dtype=complex)
+    for j in range(N):
+        H += -(p.Omega_p / 2) * op(j, models.HEG3 + models.HGE3) - (p.Omega_c / 2) * op(j, models.HSE3 + models.HES3)
+    V = 0.0
+    for i in range(N):
+        for j in range(i + 1, N):
+            Vij = np.sum(p.u * np.exp(-p.kappa * abs(z[j] - z[i])))
+            H += Vij * op(i, models.HSS3) @ op(j, models.HSS3)
+    Heff = H.copy()
+    jumps = []
+    for i in range(N):
+        for j in range(N):
+            g = p.Gamma_1d * np.exp(1j * p.k * abs(z[i] - z[j]))
+            Heff += -0.5j * g * op(i, models.HEG3) @ op(j, models.HGE3)
+    for j in range(N):
+        Heff += -0.5j * p.gamma_e * op(j, models.HEE3) - 0.5j * p.gamma_r * op(j, models.HSS3)
+    # superoperator on X = rho^T stored column-major: (A rho) <-> X A^T <-> kron(A, I) vec(X)
+    def left(A):    # rho -> A rho
+        return np.kron(A, I)
+
+    def right(B):   # rho -> rho B
+        return np.kron(I, B.T)
+
+    Lsup = left(-1j * Heff) + right(1j * Heff.conj().T)
+    for i in range(N):
+        for j in range(N):
+            g = p.Gamma_1d * np.cos(p.k * abs(z[i] - z[j]))
+            Lsup += g * left(op(i, models.HGE3)) @ right(op(j, models.HEG3))
+    for j in range(N):
+        Lsup += p.gamma_e * left(op(j, models.HGE3)) @ right(op(j, models.HEG3))
+        Lsup += p.gamma_r * left(op(j, models.HSS3)) @ right(op(j, models.HSS3))
+    return Lsup
+
+
+def _site_major_perm(N):
+    """Map the (ket-block, bra-block) global index to the MPS order (site-local bra + 3 ket)."""
+    D = 3 ** N
+    perm = np.zeros(D * D, dtype=np.int64)
+    for ket in range(D):
+        for bra in range(D):
+            kd = [(ket // 3 ** n) % 3 for n in range(N)]
+            bd = [(bra // 3 ** n) % 3 for n in range(N)]
+            s = sum((bd[n] + 3 * kd[n]) * 9 ** n for n in range(N))
+            perm[s] = bra + D * ket
+    return perm
+
+
+def test_rydberg_lindblad_mpo_dense():
+    N = 3
+    p = models.RydbergParams(n_exp=2)
+    z = models.positions(N, 3.0, 4)
+    Lm = Dn.mpo_to_dense(models.rydberg_lindblad_mpo(p, z, 0.0, 1.0))
+    Ld = _dense_lindbladian(p, z)
+    perm = _site_major_perm(N)
+    Ld = Ld[np.ix_(perm, perm)]
+    assert np.abs(Lm - Ld).max() < 1e-12 * np.abs(Ld).max()
+
+
+def test_rydberg_trace_preserving():
+    N = 3
+    p = models.RydbergParams(n_exp=2)
+    Lm = Dn.mpo_to_dense(models.rydberg_lindblad_mpo(p, models.positions(N, 3.0, 2), 0.0, 1.0))
+    trace_vec = Dn.mps_to_dense(models.identity_vec_mps(N))
+    assert np.abs(trace_vec @ Lm).max() < 1e-12
+
+
+def test_c6_fit():
+    u, kap, err = models.fit_c6_exponentials(4)
+    assert err < 0.05 and np.all(u > 0) and np.all(kap > 0)
+    r = np.geomspace(1, 10, 50)
+    fit = np.exp(-np.outer(r, kap)) @ u
+    assert np.max(np.abs(fit * r ** 6 - 1)) < 0.05
+
+
+def test_library_exports_every_declared_symbol():
+    lib_path = os.path.join(REPO, "matrixproductstates.jl_amd", "lib", "libmpship.so")
+    assert os.path.exists(lib_path), "libmpship.so not built"
+    lib = ctypes.CDLL(lib_path)
+    hdr = open(os.path.join(REPO, "include", "mpship.h")).read()
+    names = re.findall(r"^\s*(?:int|const char\*)\s+(mps_\w+)\s*\(", hdr, re.M)
+    assert len(names) > 50
+    for n in names:
+        assert hasattr(lib, n), n
+    from mpsamd import _lib
+    assert set(names) == set(_lib.SIGNATURES) | {"mps_last_error"}
