@@ -61,10 +61,31 @@ size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus);
 void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_cus, hipStream_t s);
 
 // Householder QR (zgeqrf + zungqr semantics, optional rank cutoff); in place Q.
-// Element (r, c) at base + r*rs + c*cs; m >= n.  tau_ws: n complex doubles.
-// scal_ws: 8 doubles.  R_out (optional): n x n column-major upper triangle.
+// Element (r, c) at base + r*rs + c*cs; m >= n.  ws: qr_workspace_bytes(m, n).
+// R_out (optional): n x n column-major upper triangle.
+size_t qr_workspace_bytes(int64_t m, int64_t n);
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol,
                void* tau_ws, void* scal_ws, void* R_out, hipStream_t s);
+
+// ------------------------------------------------------- device helpers
+// DPP move of a double (two 32-bit halves), bound_ctrl: out-of-row sources read 0
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+// Sum over the 64 lanes of a wave, result in every lane: quad_perm and row_ror
+// DPP steps inside each 16-lane row, two cross-row exchanges.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_d<0xB1>(v);    // quad_perm(1,0,3,2)
+  v += dpp_d<0x4E>(v);    // quad_perm(2,3,0,1)
+  v += dpp_d<0x124>(v);   // row_ror:4
+  v += dpp_d<0x128>(v);   // row_ror:8
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
 
 // small kernels (misc.hip)
 void launch_set_scalar(void* p, double re, double im, hipStream_t s);

@@ -47,6 +47,13 @@ __device__ __forceinline__ int64_t moff(const DevMap& mp, int i) {
   return (int64_t)(i - q * mp.n0) * mp.s0 + (int64_t)q * mp.s1;
 }
 
+// loader offsets: without two-level maps (TWO = false) every map is a single stride
+template <bool TWO>
+__device__ __forceinline__ int64_t loff(const DevMap& mp, int i) {
+  if constexpr (TWO) return moff(mp, i);
+  return (int64_t)i * mp.s0;
+}
+
 __device__ __forceinline__ void ldc(const double2* p, double& re, double& im) {
   const double2 v = *p;
   re = v.x;
@@ -63,53 +70,67 @@ __device__ __forceinline__ void stc(float2* p, double re, double im) {
 }
 
 // ------------------------------------------------------------ tile loaders
-// A tile: BM x BK, written to S[plane][k][m]; B tile: BK x BN -> S[plane][k][n].
-template <typename T, typename S, int NPL>
-__device__ __forceinline__ void load_a_tile(const GemmParams& p, const T* A, int m0, int k0, int kend,
-                                            S (*As)[BK][BM + 1]) {
+// Each thread owns 4 elements of the A tile (BM x BK) and 4 of the B tile
+// (BK x BN).  fetch_* issues the tile's global loads into registers (clamped
+// addresses, no branches around the loads), put_* masks and stores them to
+// LDS as split planes S[plane][k][m|n]; the main loop fetches tile k+1 while
+// the MFMAs consume tile k.
+struct Frag {
+  double re[4], im[4];
+};
+
+template <bool TWO, typename T>
+__device__ __forceinline__ void fetch_a(const GemmParams& p, const T* A, int m0, int k0, int kend, Frag& f) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < (BM * BK) / NT; ++j) {
-    int mm, kk;
-    if (p.a_mfast) {
-      mm = tid % BM;
-      kk = tid / BM + j * (NT / BM);
-    } else {
-      kk = tid % BK;
-      mm = tid / BK + j * (NT / BK);
-    }
-    const int gm = m0 + mm, gk = k0 + kk;
-    double re = 0.0, im = 0.0;
-    if (gm < p.M && gk < kend) {
-      ldc(A + moff(p.am, gm) + moff(p.ak, gk), re, im);
-      if (p.conjA) im = -im;
-    }
+    const int mm = p.a_mfast ? (tid % BM) : (tid / BK + j * (NT / BK));
+    const int kk = p.a_mfast ? (tid / BM + j * (NT / BM)) : (tid % BK);
+    const int gm = min(m0 + mm, p.M - 1), gk = min(k0 + kk, kend - 1);
+    ldc(A + loff<TWO>(p.am, gm) + loff<TWO>(p.ak, gk), f.re[j], f.im[j]);
+  }
+}
+
+template <typename S, int NPL>
+__device__ __forceinline__ void put_a(const GemmParams& p, int m0, int k0, int kend, const Frag& f,
+                                      S (*As)[BK][BM + 1]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < (BM * BK) / NT; ++j) {
+    const int mm = p.a_mfast ? (tid % BM) : (tid / BK + j * (NT / BK));
+    const int kk = p.a_mfast ? (tid / BM + j * (NT / BM)) : (tid % BK);
+    const bool ok = (m0 + mm < p.M) && (k0 + kk < kend);
+    const double re = ok ? f.re[j] : 0.0;
+    const double im = ok ? (p.conjA ? -f.im[j] : f.im[j]) : 0.0;
     As[0][kk][mm] = (S)re;
     As[1][kk][mm] = (S)im;
     if constexpr (NPL == 3) As[2][kk][mm] = (S)re + (S)im;
   }
 }
 
-template <typename T, typename S, int NPL>
-__device__ __forceinline__ void load_b_tile(const GemmParams& p, const T* B, int n0, int k0, int kend,
-                                            S (*Bs)[BK][BN + 1]) {
+template <bool TWO, typename T>
+__device__ __forceinline__ void fetch_b(const GemmParams& p, const T* B, int n0, int k0, int kend, Frag& f) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < (BN * BK) / NT; ++j) {
-    int nn, kk;
-    if (p.b_nfast) {
-      nn = tid % BN;
-      kk = tid / BN + j * (NT / BN);
-    } else {
-      kk = tid % BK;
-      nn = tid / BK + j * (NT / BK);
-    }
-    const int gn = n0 + nn, gk = k0 + kk;
-    double re = 0.0, im = 0.0;
-    if (gn < p.N && gk < kend) {
-      ldc(B + moff(p.bk, gk) + moff(p.bn, gn), re, im);
-      if (p.conjB) im = -im;
-    }
+    const int nn = p.b_nfast ? (tid % BN) : (tid / BK + j * (NT / BK));
+    const int kk = p.b_nfast ? (tid / BN + j * (NT / BN)) : (tid % BK);
+    const int gn = min(n0 + nn, p.N - 1), gk = min(k0 + kk, kend - 1);
+    ldc(B + loff<TWO>(p.bk, gk) + loff<TWO>(p.bn, gn), f.re[j], f.im[j]);
+  }
+}
+
+template <typename S, int NPL>
+__device__ __forceinline__ void put_b(const GemmParams& p, int n0, int k0, int kend, const Frag& f,
+                                      S (*Bs)[BK][BN + 1]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < (BN * BK) / NT; ++j) {
+    const int nn = p.b_nfast ? (tid % BN) : (tid / BK + j * (NT / BK));
+    const int kk = p.b_nfast ? (tid / BN + j * (NT / BN)) : (tid % BK);
+    const bool ok = (n0 + nn < p.N) && (k0 + kk < kend);
+    const double re = ok ? f.re[j] : 0.0;
+    const double im = ok ? (p.conjB ? -f.im[j] : f.im[j]) : 0.0;
     Bs[0][kk][nn] = (S)re;
     Bs[1][kk][nn] = (S)im;
     if constexpr (NPL == 3) Bs[2][kk][nn] = (S)re + (S)im;
@@ -150,7 +171,7 @@ __device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split
 }
 
 // ------------------------------------------------------------ FP64 kernel
-template <typename TA, typename TB, typename TC>
+template <typename TA, typename TB, typename TC, bool TWO>
 __global__ __launch_bounds__(NT) void zgemm_f64_kernel(GemmParams p) {
   __shared__ double As[2][BK][BM + 1];
   __shared__ double Bs[2][BK][BN + 1];
@@ -174,10 +195,17 @@ __global__ __launch_bounds__(NT) void zgemm_f64_kernel(GemmParams p) {
       acci[i][j] = v4d{0, 0, 0, 0};
     }
 
+  Frag fa, fb;
+  fetch_a<TWO>(p, A, m0, kbeg, kend, fa);
+  fetch_b<TWO>(p, B, n0, kbeg, kend, fb);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    load_a_tile<TA, double, 2>(p, A, m0, k0, kend, As);
-    load_b_tile<TB, double, 2>(p, B, n0, k0, kend, Bs);
+    put_a<double, 2>(p, m0, k0, kend, fa, As);
+    put_b<double, 2>(p, n0, k0, kend, fb, Bs);
     __syncthreads();
+    if (k0 + BK < kend) {
+      fetch_a<TWO>(p, A, m0, k0 + BK, kend, fa);
+      fetch_b<TWO>(p, B, n0, k0 + BK, kend, fb);
+    }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
       const int k = kq * 4 + lk;
@@ -216,7 +244,7 @@ __global__ __launch_bounds__(NT) void zgemm_f64_kernel(GemmParams p) {
 }
 
 // --------------------------------------------------------- FP32 3M kernel
-template <typename TA, typename TB, typename TC>
+template <typename TA, typename TB, typename TC, bool TWO>
 __global__ __launch_bounds__(NT) void zgemm_f32_3m_kernel(GemmParams p) {
   __shared__ float As[3][BK][BM + 1];
   __shared__ float Bs[3][BK][BN + 1];
@@ -241,10 +269,17 @@ __global__ __launch_bounds__(NT) void zgemm_f32_3m_kernel(GemmParams p) {
       p3[i][j] = v4f{0, 0, 0, 0};
     }
 
+  Frag fa, fb;
+  fetch_a<TWO>(p, A, m0, kbeg, kend, fa);
+  fetch_b<TWO>(p, B, n0, kbeg, kend, fb);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    load_a_tile<TA, float, 3>(p, A, m0, k0, kend, As);
-    load_b_tile<TB, float, 3>(p, B, n0, k0, kend, Bs);
+    put_a<float, 3>(p, m0, k0, kend, fa, As);
+    put_b<float, 3>(p, n0, k0, kend, fb, Bs);
     __syncthreads();
+    if (k0 + BK < kend) {
+      fetch_a<TWO>(p, A, m0, k0 + BK, kend, fa);
+      fetch_b<TWO>(p, B, n0, k0 + BK, kend, fb);
+    }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
       const int k = kq * 4 + lk;
@@ -355,10 +390,18 @@ size_t gemm_workspace_bytes(const Gemm& g, Compute, int num_cus) {
 
 template <typename TA, typename TB, typename TC>
 static void launch_typed(const GemmParams& p, Compute cmp, dim3 grid, hipStream_t s) {
-  if (cmp == CMP_F64)
-    hipLaunchKernelGGL((zgemm_f64_kernel<TA, TB, TC>), grid, dim3(NT), 0, s, p);
-  else
-    hipLaunchKernelGGL((zgemm_f32_3m_kernel<TA, TB, TC>), grid, dim3(NT), 0, s, p);
+  const bool two = p.am.n0 || p.ak.n0 || p.bk.n0 || p.bn.n0;
+  if (cmp == CMP_F64) {
+    if (two)
+      hipLaunchKernelGGL((zgemm_f64_kernel<TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
+    else
+      hipLaunchKernelGGL((zgemm_f64_kernel<TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+  } else {
+    if (two)
+      hipLaunchKernelGGL((zgemm_f32_3m_kernel<TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
+    else
+      hipLaunchKernelGGL((zgemm_f32_3m_kernel<TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+  }
 }
 
 template <typename TA, typename TB>

@@ -33,8 +33,7 @@ __global__ __launch_bounds__(1024) void sum_abs2_kernel(const double2* p, int64_
     const double2 v = p[i];
     s += v.x * v.x + v.y * v.y;
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = wave_sum_dpp(s);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {

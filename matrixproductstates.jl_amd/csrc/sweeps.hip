@@ -41,7 +41,7 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl) {
   scr.add(&tmp, site * cz);
   ovl.assign(std::max(n_ovl, 0), nullptr);
   for (auto& p : ovl) scr.add(&p, site * cz);
-  scr.add(&tau, (size_t)Dm * dm * cz);
+  scr.add(&tau, qr_workspace_bytes(Dm * dm, Dm));
   scr.add(&R, (size_t)Dm * Dm * dm * cz);
   scr.add(&envA, big * cz);
   scr.add(&envB, big * cz);

@@ -81,7 +81,8 @@ def test_overlap_contractions(dev, shape, rng):
     assert abs(dev.local_exp(A0, A0, Op) - O.local_exp(A0, A0, Op)) < 1e-10 * max(1, abs(O.local_exp(A0, A0, Op)))
 
 
-@pytest.mark.parametrize("mn", [(8, 4), (64, 32), (256, 128), (1152, 128), (300, 77)])
+@pytest.mark.parametrize("mn", [(8, 4), (64, 32), (256, 128), (1152, 128), (300, 77), (1000, 50),
+                                (2304, 256), (2600, 40)])
 @pytest.mark.parametrize("view", [False, True])
 def test_qrq_full_rank(dev, mn, view, rng):
     m, n = mn
@@ -105,10 +106,11 @@ def test_qrq_zero_padded_closed_form(dev):
 
 
 @pytest.mark.parametrize("tol", [1e-13, 1e-8])
-def test_qrq_rank_cutoff(dev, tol, rng):
-    # rank-3 panel plus rounding-level noise: with the cutoff both sides take
+@pytest.mark.parametrize("mnr", [(64, 16, 3), (600, 70, 9), (1152, 128, 20), (2304, 40, 5)])
+def test_qrq_rank_cutoff(dev, tol, mnr, rng):
+    # low-rank panel plus rounding-level noise: with the cutoff both sides take
     # the identity reflector on the noise columns, so the completion agrees.
-    m, n, r = 64, 16, 3
+    m, n, r = mnr
     M = crandn(rng, (m, r)) @ crandn(rng, (r, n)) + 1e-17 * crandn(rng, (m, n))
     from mpsamd import default_context
     ctx = default_context()
