@@ -591,8 +591,8 @@ int mps_dev_qrq(mps_ctx_t ctx, void* dev, int64_t m, int64_t n, int64_t rs, int6
     RAW_PLAN(ctx, mx({m, n}), 1, 1);
     Scratch qs(&ctx->c);
     (void)qs;
-    REQUIRE(qr_workspace_bytes(m, n) <= p.scr.used, "qr workspace");
-    launch_qr(dev, m, n, rs, cs, ctx->c.qr_tol, p.tau, nullptr, R_out, ctx->c.stream);
+    REQUIRE(qr_workspace_bytes(m, n, ctx->c.num_cus) <= p.scr.used, "qr workspace");
+    launch_qr(dev, m, n, rs, cs, ctx->c.qr_tol, p.tau, ctx->c.num_cus, R_out, ctx->c.stream);
     ctx->c.sync();
   });
 }

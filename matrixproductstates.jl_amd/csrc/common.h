@@ -48,6 +48,7 @@ struct Gemm {
   const void* B = nullptr; IdxMap bk{}, bn{}; int64_t b_bs = 0; bool conjB = false; DType tb = F64;
   void* C = nullptr; IdxMap cm{}, cn{}; int64_t c_bs = 0; DType tc = F64;
   double alpha_re = 1, alpha_im = 0, beta_re = 0, beta_im = 0;
+  double diag_re = 0;   // added to C[i][i] after alpha/beta (Q = I - V W)
 };
 
 enum Compute { CMP_F64 = 0, CMP_F32_3M = 1 };
@@ -61,11 +62,11 @@ size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus);
 void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_cus, hipStream_t s);
 
 // Householder QR (zgeqrf + zungqr semantics, optional rank cutoff); in place Q.
-// Element (r, c) at base + r*rs + c*cs; m >= n.  ws: qr_workspace_bytes(m, n).
-// R_out (optional): n x n column-major upper triangle.
-size_t qr_workspace_bytes(int64_t m, int64_t n);
-void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol,
-               void* tau_ws, void* scal_ws, void* R_out, hipStream_t s);
+// Element (r, c) at base + r*rs + c*cs; m >= n.  ws: qr_workspace_bytes(m, n,
+// num_cus).  R_out (optional): n x n column-major upper triangle.
+size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus);
+void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
+               void* R_out, hipStream_t s);
 
 // ------------------------------------------------------- device helpers
 // DPP move of a double (two 32-bit halves), bound_ctrl: out-of-row sources read 0

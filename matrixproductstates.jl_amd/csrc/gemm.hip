@@ -37,6 +37,7 @@ struct GemmParams {
   const void* B; DevMap bk, bn; int64_t b_bs; int conjB; int b_nfast;
   void* C; DevMap cm, cn; int64_t c_bs;
   double ar, ai, br, bi;
+  double dg;      // added to the real part of every diagonal element C[i][i]
   void* P;        // split-K partials: [(split*batch + b)][n][m]
   int batch;
 };
@@ -165,6 +166,7 @@ __device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split
         re += p.br * orr - p.bi * oi;
         im += p.br * oi + p.bi * orr;
       }
+      re += (row == col) ? p.dg : 0.0;
       stc(C, re, im);
     }
   }
@@ -345,6 +347,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmParams p) {
       re += p.br * orr - p.bi * oi;
       im += p.br * oi + p.bi * orr;
     }
+    re += (row == col) ? p.dg : 0.0;
     stc(C, re, im);
   }
 }
@@ -430,6 +433,7 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
   p.b_nfast = single_unit(g.bn) ? 1 : 0;
   p.C = g.C; p.cm = dm(g.cm); p.cn = dm(g.cn); p.c_bs = g.c_bs;
   p.ar = g.alpha_re; p.ai = g.alpha_im; p.br = g.beta_re; p.bi = g.beta_im;
+  p.dg = g.diag_re;
   p.batch = (int)g.batch;
   p.P = nullptr;
   if (g.K == 0) {

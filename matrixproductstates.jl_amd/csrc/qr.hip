@@ -16,10 +16,12 @@
 // (orthonormalisation.jl:230-232) is a view, and Q lands directly in the
 // site tensor's own layout.
 //
-// Work split: one workgroup of 1024 threads per QR; 16 waves each own a
-// strided subset of the trailing columns (wavefront-shuffle dot products),
-// reflector norms use a workgroup reduction.
+// Work split: panel-blocked (below); a single-workgroup unblocked kernel
+// covers panels taller than the blocked variants hold in registers.
 #include "common.h"
+
+#include <algorithm>
+#include <type_traits>
 
 namespace mpship {
 
@@ -175,19 +177,17 @@ __global__ __launch_bounds__(QR_NT) void qr_householder_kernel(double2* __restri
 }
 
 // ----------------------------------------------------------------------------
-// Panel-blocked form (the default for m <= QB_MAXR * 64).
+// Panel-blocked form (the default for m <= 2560).
 //
-// Same per-column operation order as zgeqr2 / zung2r: trailing column j
-// receives reflectors 0, 1, ..., j-1 in order; Q column c receives H(c),
-// H(c-1), ..., H(0) in order.  Only the schedule differs:
+// Same reflectors as zgeqr2 / zung2r, applied through the compact-WY form
+// H(c0)...H(c0+nb-1) = I - V T V^H of each panel:
 //   qr_colnorm  one wave per column: ||a_c||^2 (rank cutoff only)
-//   qr_panel    one workgroup factors a 32-column panel held in registers
-//               (wave w owns panel columns w and w+16; one LDS barrier per
-//               reflector, double-buffered reflector in LDS)
-//   qr_apply    one wave per trailing column applies the panel's reflectors
-//   qr_formq    one wave per column of Q (reflectors read from a separate
+//   qr_panel    one workgroup factors an NB-column panel held in registers,
+//               rows distributed over threads
+//   qr_apply    trailing columns <- (I - V T V^H)^H columns, 4 per workgroup
+//   qr_formq    Q = (I - V_0 T_0 V_0^H) ... (I - V_p T_p V_p^H) I(:, 0:n),
+//               4 columns per workgroup (reflectors read from a separate
 //               buffer, so Q can overwrite the panel in place)
-constexpr int QB_NT = 1024;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -208,92 +208,76 @@ __global__ __launch_bounds__(64) void qr_colnorm_kernel(const double2* __restric
   if (lane == 0) colnrm[c] = s;
 }
 
-// zlarfg on the owner wave's register column; the reflector goes to LDS.
-template <int MAXR>
-__device__ __forceinline__ void panel_reflector(double2 (&a)[MAXR], int lane, int rk, int M, double thr,
-                                                double2* __restrict__ vb, double2& t_out) {
-  double xs = 0.0, ar = 0.0, ai = 0.0;
-#pragma unroll
-  for (int i = 0; i < MAXR; ++i) {
-    const int rr = lane + 64 * i;
-    const double n2 = a[i].x * a[i].x + a[i].y * a[i].y;
-    xs += (rr > rk && rr < M) ? n2 : 0.0;
-    ar += (rr == rk) ? a[i].x : 0.0;
-    ai += (rr == rk) ? a[i].y : 0.0;
-  }
-  xs = wave_sum(xs);
-  ar = wave_sum(ar);
-  ai = wave_sum(ai);
-  const double xnorm = sqrt(xs);
-  const bool ident = (thr > 0.0) ? (xnorm <= thr && fabs(ai) <= thr) : (xs == 0.0 && ai == 0.0);
-  double2 t = make_double2(0.0, 0.0), sc = make_double2(1.0, 0.0);
-  double beta = ar;
-  if (!ident) {
-    beta = -copysign(sqrt(ar * ar + ai * ai + xs), ar);
-    t = make_double2((beta - ar) / beta, -ai / beta);
-    const double dr = ar - beta, di = ai, den = dr * dr + di * di;
-    sc = make_double2(dr / den, -di / den);
-  }
-#pragma unroll
-  for (int i = 0; i < MAXR; ++i) {
-    const int rr = lane + 64 * i;
-    if (rr == rk) {
-      if (!ident) a[i] = make_double2(beta, 0.0);
-      vb[rr] = make_double2(1.0, 0.0);
-    } else if (rr > rk) {
-      if (!ident) a[i] = cmul(a[i], sc);
-      vb[rr] = (rr < M) ? a[i] : make_double2(0.0, 0.0);
-    }
-  }
-  t_out = t;
+// Cross-half sums with the gfx950 lane swaps (VALU, no LDS round trip):
+//   add32(x, y): lanes 0-31 get x[l] + x[l+32], lanes 32-63 get y[l-32] + y[l]
+//   add16(x, y): within each 32-lane half, the 16-lane row with (l & 16) == 0
+//                gets x[l] + x[l^16], the other row y[l^16] + y[l]
+__device__ __forceinline__ double swap_add32(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ double swap_add16(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+// value of lane l ^ 4: row_shl:4 into banks 0 and 2, row_shr:4 into banks 1 and 3
+__device__ __forceinline__ double xor4_d(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x104, 0xf, 0x5, false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x104, 0xf, 0x5, false);
+  lo = __builtin_amdgcn_update_dpp(lo, __double2loint(v), 0x114, 0xf, 0xa, false);
+  hi = __builtin_amdgcn_update_dpp(hi, __double2hiint(v), 0x114, 0xf, 0xa, false);
+  return __hiloint2double(hi, lo);
 }
 
-// a <- a - conj(tau) v (v^H a), v from LDS (rows >= rk), unconditional reads, masked use
-template <int MAXR>
-__device__ __forceinline__ void apply_reflector_lds(double2 (&a)[MAXR], int lane, int rk, int M,
-                                                    const double2* __restrict__ v, double2 t) {
-  double wr = 0.0, wi = 0.0;
+// Butterfly reduce-scatter of 8 doubles over a wave: afterwards lane l holds
+// the wave total of v[l >> 3].  The exchanges across 32 and 16 lanes halve
+// the values a lane carries (lane swaps), the one across 8 lanes picks one
+// of the remaining two (DPP row_ror:8), and xor 4, 2, 1 add within the group
+// of 8; those final adds commute, so the 8 lanes hold identical bits.
+__device__ __forceinline__ double wave_rs8(const double (&v)[8]) {
+  const int lane = threadIdx.x & 63;
+  double w4[4], w2[2];
 #pragma unroll
-  for (int i = 0; i < MAXR; ++i) {
-    const int rr = lane + 64 * i;
-    const double2 x = v[rr];
-    const bool ok = (rr >= rk && rr < M);
-    wr += ok ? (x.x * a[i].x + x.y * a[i].y) : 0.0;
-    wi += ok ? (x.x * a[i].y - x.y * a[i].x) : 0.0;
-  }
-  wr = wave_sum(wr);
-  wi = wave_sum(wi);
-  const double fr = t.x * wr + t.y * wi, fi = t.x * wi - t.y * wr;   // conj(tau) w
-  // re-read v from LDS instead of keeping MAXR values live across the reduction
-  asm volatile("" ::: "memory");
+  for (int q = 0; q < 4; ++q) w4[q] = swap_add32(v[q], v[q + 4]);
 #pragma unroll
-  for (int i = 0; i < MAXR; ++i) {
-    const int rr = lane + 64 * i;
-    const double2 x = v[rr];
-    if (rr >= rk && rr < M) {
-      a[i].x -= x.x * fr - x.y * fi;
-      a[i].y -= x.x * fi + x.y * fr;
-    }
-  }
+  for (int q = 0; q < 2; ++q) w2[q] = swap_add16(w4[q], w4[q + 2]);
+  const bool h3 = (lane & 8) != 0;
+  const double keep = h3 ? w2[1] : w2[0];
+  const double send = h3 ? w2[0] : w2[1];
+  double w = keep + dpp_d<0x128>(send);   // row_ror:8 pairs lane i with i ^ 8
+  w += xor4_d(w);
+  w += dpp_d<0x4E>(w);                     // quad_perm(2,3,0,1): i ^ 2
+  w += dpp_d<0xB1>(w);                     // quad_perm(1,0,3,2): i ^ 1
+  return w;
 }
 
-// One workgroup factors panel columns c0 .. c0+nbp-1 (nbp <= NW), rows c0..m-1;
-// wave w holds panel column w in registers.  Only LDS traffic inside the
-// column loop.  Waves whose column is already factored accumulate the Gram
-// entries G[w][k] = v_w^H v_k while the others apply reflector k, so the
-// compact-WY factor T (H(0)...H(nbp-1) = I - V T V^H, zlarft) needs no
-// further pass over V.
-template <int MAXR, int NT>
+// One workgroup factors panel columns c0 .. c0+nbp-1 (nbp <= NB), rows
+// c0..m-1.  Thread t holds rows t, t+NT, ..., t+(R-1)NT of the panel in
+// registers, so every dot product of a step is a local partial sum; only the
+// 3 zlarfg scalars and the dot products v_k^H a_j cross lanes (butterfly in
+// the wave, LDS across waves).  The column loop is not unrolled (the
+// unrolled body overflows the instruction cache): the panel shifts left by
+// one column per step, so the active column always sits in a[.][0].  Each
+// finished column goes straight to A (packed R / reflector) and Vs
+// (explicit reflector).  Then the Gram entries G[j][l] = v_j^H v_l (j < l)
+// are formed from Vs for the compact-WY factor T (H(0)...H(nbp-1) =
+// I - V T V^H, zlarft).
+template <int R, int NB, int NT>
 __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, int m, int n, int64_t rs,
                                                       int64_t cs, int c0, int nbp, double tol,
                                                       const double* __restrict__ colnrm,
                                                       double2* __restrict__ Vs, double2* __restrict__ tau,
                                                       double2* __restrict__ Tp) {
   constexpr int NW = NT / 64;
-  __shared__ double2 vbuf[2][MAXR * 64];
-  __shared__ double2 s_tau[NW];
-  __shared__ double2 s_G[NW][NW];
-  __shared__ double2 s_T[NW][NW];
+  constexpr int GMAX = (2 * (NB - 1) + 7) / 8;            // 8-value groups per column step
+  static_assert(GMAX <= 8, "panel too wide");
+  __shared__ double red[2][NW * GMAX * 64];
+  __shared__ double gram[NB * NW * 64];
+  __shared__ double2 s_G[NB][NB];
+  __shared__ double2 s_T[NB][NB + 1];
+  __shared__ double2 s_tau[NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int M = m - c0;
   double thr = 0.0;
@@ -302,87 +286,267 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     for (int c = 0; c < n; ++c) t += colnrm[c];
     thr = tol * sqrt(t);
   }
-  const int j = wave;
-  double2 a[MAXR];
+  double2 a[R][NB];
 #pragma unroll
-  for (int i = 0; i < MAXR; ++i) {
-    const int rr = lane + 64 * i;
-    const double2 x = A[(int64_t)(c0 + min(rr, M - 1)) * rs + (int64_t)(c0 + min(j, nbp - 1)) * cs];
-    a[i] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
+  for (int i = 0; i < R; ++i) {
+    const int rr = tid + NT * i;
+    const int64_t ro = (int64_t)(c0 + min(rr, M - 1)) * rs;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double2 x = A[ro + (int64_t)(c0 + min(j, nbp - 1)) * cs];
+      a[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
+    }
   }
-  for (int k = 0; k < nbp; ++k) {
-    const int buf = k & 1;
-    if (wave == k) {
-      double2 t;
-      panel_reflector<MAXR>(a, lane, k, M, thr, vbuf[buf], t);
-      if (lane == 0) s_tau[k] = t;
+  // sum over waves of the butterfly results of group g; lane l gets value l >> 3
+  auto wave_sum_slots = [&](const double* r, int g) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += r[(w * GMAX + g) * 64 + lane];
+    return s;
+  };
+  auto bcast = [&](double v, int srclane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), srclane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), srclane);
+    return __hiloint2double(hi, lo);
+  };
+  int ph = 0;
+  // steps k in [kb, ke) with the panel's live columns in a[.][0..W-1]: once
+  // at most W-1 columns trail column k, the loop body shrinks to width W
+  auto segment = [&](auto Wc, int kb, int ke) {
+    constexpr int W = decltype(Wc)::value;
+    constexpr int GW = (2 * (W - 1) + 7) / 8;
+#pragma unroll 1
+  for (int k = kb; k < ke; ++k) {
+    // zlarfg on column k (= a[.][0]): ||x||^2 below row k and alpha = a(k, k)
+    {
+      double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int rr = tid + NT * i;
+        const double2 x = a[i][0];
+        const double n2 = x.x * x.x + x.y * x.y;
+        v8[0] += (rr > k && rr < M) ? n2 : 0.0;
+        v8[1] += (rr == k) ? x.x : 0.0;
+        v8[2] += (rr == k) ? x.y : 0.0;
+      }
+      red[ph][wave * GMAX * 64 + lane] = wave_rs8(v8);
     }
     __syncthreads();
-    const double2 t = s_tau[k];
-    if (j > k && j < nbp) {
-      if (t.x != 0.0 || t.y != 0.0) apply_reflector_lds<MAXR>(a, lane, k, M, vbuf[buf], t);
-    } else if (j < k) {
-      // G[j][k] = v_j^H v_k over rows >= k (v_k vanishes above k; row j of v_j is 1 < k)
-      double gr = 0.0, gi = 0.0;
+    const double s0 = wave_sum_slots(red[ph], 0);
+    ph ^= 1;
+    const double xs = bcast(s0, 0), ar = bcast(s0, 8), ai = bcast(s0, 16);
+    const bool ident = (thr > 0.0) ? (sqrt(xs) <= thr && fabs(ai) <= thr) : (xs == 0.0 && ai == 0.0);
+    if (ident) {
+      if (tid == 0) s_tau[k] = make_double2(0.0, 0.0);
+      // column k out unchanged (packed to A; Vs: 1 at row k, zero elsewhere)
 #pragma unroll
-      for (int i = 0; i < MAXR; ++i) {
-        const int rr = lane + 64 * i;
-        const double2 x = vbuf[buf][rr];
-        const bool ok = (rr >= k && rr < M);
-        gr += ok ? (a[i].x * x.x + a[i].y * x.y) : 0.0;
-        gi += ok ? (a[i].x * x.y - a[i].y * x.x) : 0.0;
+      for (int i = 0; i < R; ++i) {
+        const int rr = tid + NT * i;
+        if (rr < M) {
+          A[(int64_t)(c0 + rr) * rs + (int64_t)(c0 + k) * cs] = a[i][0];
+          Vs[(int64_t)(c0 + k) * m + c0 + rr] = make_double2(rr == k ? 1.0 : 0.0, 0.0);
+        }
       }
-      gr = wave_sum(gr);
-      gi = wave_sum(gi);
-      if (lane == 0) s_G[j][k] = make_double2(gr, gi);
+      // shift the panel: column k+1 becomes a[.][0]
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+#pragma unroll
+        for (int j = 0; j + 1 < W; ++j) a[i][j] = a[i][j + 1];
+        a[i][W - 1] = make_double2(0.0, 0.0);
+      }
+    } else {
+      const double beta = -copysign(sqrt(ar * ar + ai * ai + xs), ar);
+      const double2 t = make_double2((beta - ar) / beta, -ai / beta);
+      const double dr = ar - beta, di = ai, den = dr * dr + di * di;
+      const double2 sc = make_double2(dr / den, -di / den);
+      if (tid == 0) s_tau[k] = t;
+      // beta on the diagonal, the reflector tail below; vm = v_k on rows
+      // k..M-1 and 0 elsewhere (masks the dots and the update).  Column k
+      // goes out packed to A and as the explicit reflector to Vs.
+      double2 vm[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int rr = tid + NT * i;
+        const double2 x = a[i][0];
+        const double2 y = cmul(x, sc);
+        vm[i] = (rr == k) ? make_double2(1.0, 0.0) : ((rr > k && rr < M) ? y : make_double2(0.0, 0.0));
+        if (rr < M) {
+          A[(int64_t)(c0 + rr) * rs + (int64_t)(c0 + k) * cs] =
+              (rr == k) ? make_double2(beta, 0.0) : ((rr > k) ? y : x);
+          Vs[(int64_t)(c0 + k) * m + c0 + rr] = vm[i];
+        }
+      }
+      // s_j = v_k^H a_j for the W-1 columns to the right (zero past the
+      // panel), four columns per butterfly
+#pragma unroll
+      for (int g = 0; g < GW; ++g) {
+        double v8[8];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const int j = 1 + 4 * g + p;
+          double sr = 0.0, si = 0.0;
+          if (j < W) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+              const double2 x = a[i][j];
+              sr += vm[i].x * x.x + vm[i].y * x.y;
+              si += vm[i].x * x.y - vm[i].y * x.x;
+            }
+          }
+          v8[2 * p] = sr;
+          v8[2 * p + 1] = si;
+        }
+        red[ph][(wave * GMAX + g) * 64 + lane] = wave_rs8(v8);
+      }
+      __syncthreads();
+      double sg[GW];
+#pragma unroll
+      for (int g = 0; g < GW; ++g) sg[g] = wave_sum_slots(red[ph], g);
+      ph ^= 1;
+      // a_j <- a_j - conj(tau) v_k s_j, written one column to the left
+#pragma unroll
+      for (int j = 1; j < W; ++j) {
+        const int f = 2 * (j - 1), g = f / 8, q = f % 8;
+        const double sr = bcast(sg[g], 8 * q), si = bcast(sg[g], 8 * q + 8);
+        const double fr = t.x * sr + t.y * si, fi = t.x * si - t.y * sr;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          a[i][j - 1].x = a[i][j].x - (vm[i].x * fr - vm[i].y * fi);
+          a[i][j - 1].y = a[i][j].y - (vm[i].x * fi + vm[i].y * fr);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < R; ++i) a[i][W - 1] = make_double2(0.0, 0.0);
     }
   }
+  };
+  if constexpr (NB == 16) {
+    segment(std::integral_constant<int, 16>{}, 0, min(nbp, 4));
+    segment(std::integral_constant<int, 12>{}, 4, min(nbp, 8));
+    segment(std::integral_constant<int, 8>{}, 8, min(nbp, 12));
+    segment(std::integral_constant<int, 4>{}, 12, nbp);
+  } else {
+    static_assert(NB == 8, "panel width 8 or 16");
+    segment(std::integral_constant<int, 8>{}, 0, min(nbp, 4));
+    segment(std::integral_constant<int, 4>{}, 4, nbp);
+  }
   __syncthreads();
-  // T: T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k]
-  if (wave == 0) {
-    for (int k = 0; k < nbp; ++k) {
-      const double2 tk = s_tau[k];
-      double2 acc = make_double2(0.0, 0.0);
-      if (lane < k)
-        for (int l = lane; l < k; ++l) {
-          const double2 p = cmul(s_T[lane][l], s_G[l][k]);
+  // Gram: G[l-1-s][l] = v_{l-1-s}^H v_l for history slot s < l; the history
+  // h[.][s] holds v_{l-1-s} (rows of this thread), shifted once per l
+  double2 h[R][NB - 1];
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int s = 0; s < NB - 1; ++s) h[i][s] = make_double2(0.0, 0.0);
+  double2 vn[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int rr = tid + NT * i;
+    const double2 x = Vs[(int64_t)c0 * m + c0 + min(rr, M - 1)];
+    vn[i] = (rr < M) ? x : make_double2(0.0, 0.0);
+  }
+  // columns l in [lb, le) need history slots 0..l-1 < S
+  auto gseg = [&](auto Sc, int lb, int le) {
+    constexpr int S = decltype(Sc)::value;
+    constexpr int GS = (2 * S + 7) / 8;
+#pragma unroll 1
+  for (int l = lb; l < le; ++l) {
+    double2 v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = vn[i];
+    // next column's rows (this thread's own earlier stores) in flight meanwhile
+    const int ln = min(l + 1, nbp - 1);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int rr = tid + NT * i;
+      const double2 x = Vs[(int64_t)(c0 + ln) * m + c0 + min(rr, M - 1)];
+      vn[i] = (rr < M) ? x : make_double2(0.0, 0.0);
+    }
+    double keep = 0.0;
+#pragma unroll
+    for (int g = 0; g < GS; ++g) {
+      double v8[8];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int s = 4 * g + p;
+        double gr = 0.0, gi = 0.0;
+        if (s < S) {
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            gr += h[i][s].x * v[i].x + h[i][s].y * v[i].y;
+            gi += h[i][s].x * v[i].y - h[i][s].y * v[i].x;
+          }
+        }
+        v8[2 * p] = gr;
+        v8[2 * p + 1] = gi;
+      }
+      const double r8 = wave_rs8(v8);
+      keep = ((lane & 7) == g) ? r8 : keep;     // value q of group g at lane 8q + g
+    }
+    gram[(l * NW + wave) * 64 + lane] = keep;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+#pragma unroll
+      for (int s = S - 1; s > 0; --s) h[i][s] = h[i][s - 1];
+      h[i][0] = v[i];
+    }
+  }
+  };
+  if constexpr (NB == 16) {
+    gseg(std::integral_constant<int, 4>{}, 0, min(nbp, 4));
+    gseg(std::integral_constant<int, 8>{}, 4, min(nbp, 8));
+    gseg(std::integral_constant<int, 12>{}, 8, min(nbp, 12));
+    gseg(std::integral_constant<int, 15>{}, 12, nbp);
+  } else {
+    gseg(std::integral_constant<int, 4>{}, 0, min(nbp, 4));
+    gseg(std::integral_constant<int, 7>{}, 4, nbp);
+  }
+  __syncthreads();
+  for (int e = tid; e < NB * NB; e += NT) {
+    const int j = e % NB, l = e / NB;
+    double2 gv = make_double2(0.0, 0.0);
+    if (j < l && l < nbp) {
+      const int s = l - 1 - j, g = s / 4, q = 2 * (s % 4);
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        gv.x += gram[(l * NW + w) * 64 + 8 * q + g];
+        gv.y += gram[(l * NW + w) * 64 + 8 * (q + 1) + g];
+      }
+    }
+    s_G[j][l] = gv;
+  }
+  __syncthreads();
+  // T: T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k]; lane j
+  // keeps row j of T in registers, G columns are broadcast reads
+  if (tid < 64) {
+    double2 trow[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) trow[c] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      if (k < nbp) {
+        const double2 tk = s_tau[k];
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int l = 0; l < k; ++l) {
+          const double2 p = cmul(trow[l], s_G[l][k]);   // trow[l] = 0 for l < lane
           acc.x += p.x;
           acc.y += p.y;
         }
-      __builtin_amdgcn_wave_barrier();
-      if (lane < k) {
         const double2 v = cmul(tk, acc);
-        s_T[lane][k] = make_double2(-v.x, -v.y);
-      } else if (lane == k) {
-        s_T[k][k] = tk;
-      } else if (lane < NW) {
-        s_T[lane][k] = make_double2(0.0, 0.0);
+        trow[k] = (lane < k) ? make_double2(-v.x, -v.y) : ((lane == k) ? tk : make_double2(0.0, 0.0));
       }
-      __builtin_amdgcn_wave_barrier();
     }
-  }
-  // packed panel back to A; explicit reflectors to Vs (column c0+j: zero rows
-  // 0..c0+j-1, 1 at row c0+j, the tail below; an identity reflector stores 0)
-  if (j < nbp) {
-    const bool id = (s_tau[j].x == 0.0 && s_tau[j].y == 0.0);
-    double2* vcol = Vs + (int64_t)(c0 + j) * m;
-    for (int r = lane; r < c0; r += 64) vcol[r] = make_double2(0.0, 0.0);
+    if (lane < NB) {
 #pragma unroll
-    for (int i = 0; i < MAXR; ++i) {
-      const int rr = lane + 64 * i;
-      if (rr < M) {
-        A[(int64_t)(c0 + rr) * rs + (int64_t)(c0 + j) * cs] = a[i];
-        vcol[c0 + rr] = (rr < j) ? make_double2(0.0, 0.0)
-                                 : (rr == j ? make_double2(1.0, 0.0) : (id ? make_double2(0.0, 0.0) : a[i]));
-      }
+      for (int c = 0; c < NB; ++c) s_T[lane][c] = (lane < nbp && c < nbp) ? trow[c] : make_double2(0.0, 0.0);
     }
   }
+  for (int r = tid; r < c0; r += NT)
+    for (int j = 0; j < nbp; ++j) Vs[(int64_t)(c0 + j) * m + r] = make_double2(0.0, 0.0);
   if (tid < nbp) tau[c0 + tid] = s_tau[tid];
   __syncthreads();
-  for (int e = tid; e < NW * NW; e += NT) {
-    const int r = e % NW, c = e / NW;
-    Tp[e] = (r < nbp && c < nbp) ? s_T[r][c] : make_double2(0.0, 0.0);
-  }
+  for (int e = tid; e < NB * NB; e += NT) Tp[e] = s_T[e % NB][e / NB];
 }
 
 // ---------------------------------------------------------------------------
@@ -542,6 +706,97 @@ __global__ __launch_bounds__(WY_NT) void qr_apply_kernel(double2* __restrict__ A
   }
 }
 
+// geqrf trailing update, one workgroup per trailing column j:
+//   a_j <- (I - V T V^H)^H a_j = a_j - V (T^H (V^H a_j))   on rows c0..m-1.
+// Thread t holds rows t, t+NT, ... (RA of them) of a_j and of the panel's NB
+// reflectors in registers for both passes; V^H a_j is one butterfly
+// reduction per wave plus an LDS sum over waves.
+template <int RA, int NB, int NT>
+__global__ __launch_bounds__(NT) void qr_apply_col_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
+                                                          int c0, int nbp, const double2* __restrict__ Vs,
+                                                          const double2* __restrict__ Tp) {
+  constexpr int NW = NT / 64;
+  constexpr int NG = (2 * NB + 7) / 8;
+  __shared__ double red[NW * NG * 64];
+  __shared__ double2 s_w[NB], s_u[NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j = c0 + nbp + blockIdx.x;
+  const int M = m - c0;
+  const double2* V = Vs + (int64_t)c0 * m + c0;   // reflector q (panel-local), row c0 + r: V[r + q m]
+  double2 x[RA], v[RA][NB];
+#pragma unroll
+  for (int i = 0; i < RA; ++i) {
+    const int rr = tid + NT * i;
+    const bool ok = rr < M;
+    const int rc = min(rr, M - 1);
+    const double2 xa = A[(int64_t)(c0 + rc) * rs + (int64_t)j * cs];
+    x[i] = ok ? xa : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const double2 va = V[rc + (int64_t)min(q, nbp - 1) * m];
+      v[i][q] = (ok && q < nbp) ? va : make_double2(0.0, 0.0);
+    }
+  }
+  // w = V^H x
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    double v8[8];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int q = 4 * g + p;
+      double wr = 0.0, wi = 0.0;
+      if (q < NB) {
+#pragma unroll
+        for (int i = 0; i < RA; ++i) {
+          wr += v[i][q].x * x[i].x + v[i][q].y * x[i].y;
+          wi += v[i][q].x * x[i].y - v[i][q].y * x[i].x;
+        }
+      }
+      v8[2 * p] = wr;
+      v8[2 * p + 1] = wi;
+    }
+    red[(wave * NG + g) * 64 + lane] = wave_rs8(v8);
+  }
+  __syncthreads();
+  if (tid < NB) {
+    const int g = tid / 4, f = 2 * (tid % 4);
+    double wr = 0.0, wi = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      wr += red[(w * NG + g) * 64 + 8 * f];
+      wi += red[(w * NG + g) * 64 + 8 * (f + 1)];
+    }
+    s_w[tid] = make_double2(wr, wi);
+  }
+  __syncthreads();
+  // u = T^H w
+  if (tid < NB) {
+    double ur = 0.0, ui = 0.0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const double2 t = Tp[i + tid * NB], w = s_w[i];
+      ur += t.x * w.x + t.y * w.y;
+      ui += t.x * w.y - t.y * w.x;
+    }
+    s_u[tid] = make_double2(ur, ui);
+  }
+  __syncthreads();
+  double2 u[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) u[q] = s_u[q];
+#pragma unroll
+  for (int i = 0; i < RA; ++i) {
+    const int rr = tid + NT * i;
+    double dr = 0.0, di = 0.0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      dr += v[i][q].x * u[q].x - v[i][q].y * u[q].y;
+      di += v[i][q].x * u[q].y + v[i][q].y * u[q].x;
+    }
+    if (rr < M) A[(int64_t)(c0 + rr) * rs + (int64_t)j * cs] = make_double2(x[i].x - dr, x[i].y - di);
+  }
+}
+
 // Q = H(0) ... H(n-1) I(:, 0:n): 4 columns per workgroup (same panel), panel
 // blocks from the columns' own panel down to 0, over all m rows (V is zero
 // above each panel).
@@ -576,53 +831,223 @@ __global__ __launch_bounds__(WY_NT) void qr_formq_kernel(double2* __restrict__ A
   }
 }
 
-template <int MAXR, int NT>
-static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double tol, double2* Vs,
-                       double2* tau, double* colnrm, double2* Tall, double2* R_out, hipStream_t s) {
-  constexpr int NB = NT / 64;
+// ---------------------------------------------------------------------------
+// Q from the panel factors with two MFMA GEMMs.  With B_p = I - V_p T_p V_p^H
+// the panel block reflectors, B_p B_{p+1} ... = I - sum_q V_q Z_q^H where
+//   Z_p^H = T_p (V_p^H - sum_{q>p} G_pq Z_q^H),  G_pq = V_p^H V_q,
+// so Q = B_0 ... B_{np-1} I(:, 0:n) = I(:, 0:n) - V W with the block rows
+//   W_p = T_p (V_p[0:n, :]^H - sum_{q>p} G_pq W_q)          (NB x n).
+// qr_formw_kernel does this block back-substitution (8 columns of W per
+// workgroup, G's block row staged in LDS) and copies R out; the GEMM
+// G = V^H V comes before it, and Q = I(:, 0:n) - V W after it (written over
+// the panel, the identity added in the GEMM's epilogue).
+constexpr int FW_CW = 8;           // W columns per workgroup
+constexpr int FW_NMAX = 256;       // n limit of the LDS layout
+
+// thread (r, c, ks): row r of the block, column c of the workgroup, slice ks
+// of the k-sum (KS = 256 / (NB * FW_CW) slices, combined through LDS)
+template <int NB>
+__global__ __launch_bounds__(256) void qr_formw_kernel(const double2* __restrict__ A, int m, int n, int64_t rs,
+                                                       int64_t cs, const double2* __restrict__ Vs,
+                                                       const double2* __restrict__ Tall,
+                                                       const double2* __restrict__ G, double2* __restrict__ W,
+                                                       double2* __restrict__ Rout) {
+  constexpr int KS = 256 / (NB * FW_CW);
+  __shared__ double2 sW[FW_NMAX][FW_CW];
+  __shared__ double2 sG[NB][FW_NMAX + 1];
+  __shared__ double2 sp[KS][FW_CW][NB + 1];
+  const int tid = threadIdx.x, r = tid % NB, cc = (tid / NB) % FW_CW, ks = tid / (NB * FW_CW);
+  const int cbase = blockIdx.x * FW_CW;
+  const int c = cbase + cc;
+  const int cl = min(c, n - 1);
+  const int np = (n + NB - 1) / NB;
+  // R (upper triangle of the packed panel) out before Q overwrites it
+  if (Rout != nullptr) {
+    for (int e = tid; e < FW_CW * n; e += 256) {
+      const int rr = e % n, q = cbase + e / n;
+      if (q < n)
+        Rout[(int64_t)q * n + rr] = (rr <= q) ? A[(int64_t)rr * rs + (int64_t)q * cs] : make_double2(0.0, 0.0);
+    }
+  }
+  for (int p = np - 1; p >= 0; --p) {
+    const int k0 = (p + 1) * NB;
+    // stage G[pNB : pNB+NB, k0 : n]
+    for (int e = tid; e < NB * (n - k0); e += 256) {
+      const int rr = e % NB, k = k0 + e / NB;
+      sG[rr][k] = G[(int64_t)k * n + min(p * NB + rr, n - 1)];
+    }
+    __syncthreads();
+    const int row = p * NB + r;
+    // slice ks of -sum_k G[row, k] W[k, c]; two accumulators
+    double2 y0 = make_double2(0.0, 0.0), y1 = make_double2(0.0, 0.0);
+    if (row < n) {
+      int k = k0 + ks;
+      for (; k + KS < n; k += 2 * KS) {
+        const double2 g0 = sG[r][k], w0 = sW[k][cc], g1 = sG[r][k + KS], w1 = sW[k + KS][cc];
+        y0.x -= g0.x * w0.x - g0.y * w0.y;
+        y0.y -= g0.x * w0.y + g0.y * w0.x;
+        y1.x -= g1.x * w1.x - g1.y * w1.y;
+        y1.y -= g1.x * w1.y + g1.y * w1.x;
+      }
+      if (k < n) {
+        const double2 g0 = sG[r][k], w0 = sW[k][cc];
+        y0.x -= g0.x * w0.x - g0.y * w0.y;
+        y0.y -= g0.x * w0.y + g0.y * w0.x;
+      }
+    }
+    sp[ks][cc][r] = make_double2(y0.x + y1.x, y0.y + y1.y);
+    __syncthreads();
+    if (ks == 0 && row < n) {
+      // y = V_p[0:n, :]^H e_c + sum of the slices; W[row, c] = (T_p y)[r]
+      double2 y[NB];
+#pragma unroll
+      for (int s = 0; s < NB; ++s) {
+        const int rs_ = p * NB + s;
+        const double2 v = Vs[(int64_t)min(rs_, n - 1) * m + cl];
+        double2 acc = (rs_ < n) ? make_double2(v.x, -v.y) : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < KS; ++q) {
+          acc.x += sp[q][cc][s].x;
+          acc.y += sp[q][cc][s].y;
+        }
+        y[s] = acc;
+      }
+      const double2* Tp = Tall + (int64_t)p * NB * NB;
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int s = 0; s < NB; ++s) {
+        const double2 t = Tp[r + s * NB];
+        acc.x += t.x * y[s].x - t.y * y[s].y;
+        acc.y += t.x * y[s].y + t.y * y[s].x;
+      }
+      sW[row][cc] = acc;
+      if (c < n) W[(int64_t)c * n + row] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+struct QrGemms {
+  Gemm gram, q;
+};
+static QrGemms qr_gemms(double2* A, int64_t m, int64_t n, int64_t rs, int64_t cs, const double2* Vs, double2* G,
+                        double2* W) {
+  QrGemms g;
+  // G = V^H V
+  g.gram.M = n; g.gram.N = n; g.gram.K = m;
+  g.gram.A = Vs; g.gram.am = im1(m); g.gram.ak = im1(1); g.gram.conjA = true;
+  g.gram.B = Vs; g.gram.bk = im1(1); g.gram.bn = im1(m);
+  g.gram.C = G; g.gram.cm = im1(1); g.gram.cn = im1(n);
+  // Q = I(:, 0:n) - V W, in place over the reset panel
+  g.q.M = m; g.q.N = n; g.q.K = n;
+  g.q.A = Vs; g.q.am = im1(1); g.q.ak = im1(m);
+  g.q.B = W; g.q.bk = im1(1); g.q.bn = im1(n);
+  g.q.C = A; g.q.cm = im1(rs); g.q.cn = im1(cs);
+  g.q.alpha_re = -1.0; g.q.diag_re = 1.0;
+  return g;
+}
+
+// R rows per thread and panel width NB in the panel kernel (at most NTMAX
+// threads); MAXR rows per lane in the WY kernels.
+struct QrWork {
+  double2 *Vs, *tau, *Tall, *G, *W;
+  double* colnrm;
+  void* gws;
+  size_t gws_bytes;
+  int num_cus;
+};
+
+template <int R, int NB, int NT, int MAXR, int RA, int NTA>
+static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double tol, const QrWork& q,
+                       double2* R_out, hipStream_t s) {
+  double2 *Vs = q.Vs, *tau = q.tau, *Tall = q.Tall;
+  double* colnrm = q.colnrm;
   if (tol > 0.0) hipLaunchKernelGGL(qr_colnorm_kernel, dim3((unsigned)n), dim3(64), 0, s, A, m, rs, cs, colnrm);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
-    hipLaunchKernelGGL((qr_panel_kernel<MAXR, NT>), dim3(1), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp, tol,
-                       colnrm, Vs, tau, Tp);
+    hipLaunchKernelGGL((qr_panel_kernel<R, NB, NT>), dim3(1), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
+                       tol, colnrm, Vs, tau, Tp);
     const int trail = n - c0 - nbp;
     if (trail > 0)
-      hipLaunchKernelGGL((qr_apply_kernel<MAXR, NB>), dim3((unsigned)((trail + 3) / 4)), dim3(WY_NT), 0, s, A,
-                         m, rs, cs, c0, nbp, trail, Vs, Tp);
+      hipLaunchKernelGGL((qr_apply_col_kernel<RA, NB, NTA>), dim3((unsigned)trail), dim3(NTA), 0, s, A, m, rs, cs,
+                         c0, nbp, Vs, Tp);
   }
-  hipLaunchKernelGGL((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4)), dim3(WY_NT), 0, s, A, m, rs,
-                     cs, Vs, Tall, R_out, n);
+  if (n <= FW_NMAX) {
+    const QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
+    launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+    hipLaunchKernelGGL((qr_formw_kernel<NB>), dim3((unsigned)((n + FW_CW - 1) / FW_CW)), dim3(256), 0, s, A, m,
+                       n, rs, cs, Vs, Tall, q.G, q.W, R_out);
+    launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+  } else {
+    hipLaunchKernelGGL((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4)), dim3(WY_NT), 0, s, A, m, rs,
+                       cs, Vs, Tall, R_out, n);
+  }
 }
 
-size_t qr_workspace_bytes(int64_t m, int64_t n) {
-  // V (m x n) | tau (n) | column norms (n) | T blocks (ceil(n/NB) x NB x NB, NB <= 16)
-  return (size_t)m * n * sizeof(double2) + (size_t)n * sizeof(double2) + (size_t)n * sizeof(double) +
-         (size_t)(n + 16) * 16 * sizeof(double2) + 1024;
+// V (m x n) | tau (n) | column norms (n) | T blocks (ceil(n/NB) x NB x NB,
+// NB <= 16) | G (n x n) | W (n x n) | split-K partials of the two GEMMs
+struct QrLayout {
+  size_t vs, tau, cn, tall, g, w, gws, total, gws_bytes;
+};
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
+  QrLayout L;
+  size_t o = 0;
+  L.vs = o; o = al256(o + (size_t)m * n * sizeof(double2));
+  L.tau = o; o = al256(o + (size_t)n * sizeof(double2));
+  L.cn = o; o = al256(o + (size_t)n * sizeof(double));
+  L.tall = o; o = al256(o + (size_t)(n + 16) * 16 * sizeof(double2));
+  L.g = o; o = al256(o + (size_t)n * n * sizeof(double2));
+  L.w = o; o = al256(o + (size_t)n * n * sizeof(double2));
+  // split-K partials: a bound that grows with (m, n) (at most 32 splits of
+  // the n x n Gram, ceil(n/64) of the m x n product), so a workspace sized
+  // for the largest panel of a state serves every smaller one
+  const QrGemms g = qr_gemms(nullptr, m, n, 1, m, nullptr, nullptr, nullptr);
+  const size_t need =
+      std::max(gemm_workspace_bytes(g.gram, CMP_F64, num_cus), gemm_workspace_bytes(g.q, CMP_F64, num_cus));
+  const size_t bound = std::max((size_t)32 * n * n, (size_t)((n + 63) / 64) * m * n) * sizeof(double2);
+  L.gws_bytes = std::max(need, bound);
+  L.gws = o; o = al256(o + L.gws_bytes);
+  L.total = o;
+  return L;
 }
 
-void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws,
-               void* /*scal_ws*/, void* R_out, hipStream_t s) {
+size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus) { return qr_layout(m, n, num_cus).total; }
+
+void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
+               void* R_out, hipStream_t s) {
   REQUIRE(m >= n && n >= 1, "qrq!: requires m >= n >= 1");
   REQUIRE(m < (1LL << 31) && m * n < (1LL << 40), "qrq!: panel too large");
   char* w = (char*)ws;
-  double2* Vs = (double2*)w;
-  double2* tau = (double2*)(w + (size_t)m * n * sizeof(double2));
-  double* colnrm = (double*)(w + (size_t)m * n * sizeof(double2) + (size_t)n * sizeof(double2));
-  double2* Tall = (double2*)(w + (size_t)m * n * sizeof(double2) + (size_t)n * sizeof(double2) +
-                             (((size_t)n * sizeof(double) + 15) & ~(size_t)15));
+  const QrLayout L = qr_layout(m, n, num_cus);
+  QrWork q;
+  q.Vs = (double2*)(w + L.vs);
+  q.tau = (double2*)(w + L.tau);
+  q.colnrm = (double*)(w + L.cn);
+  q.Tall = (double2*)(w + L.tall);
+  q.G = (double2*)(w + L.g);
+  q.W = (double2*)(w + L.w);
+  q.gws = L.gws_bytes ? (void*)(w + L.gws) : nullptr;
+  q.gws_bytes = L.gws_bytes;
+  q.num_cus = num_cus;
   double2* A = (double2*)base;
   double2* R = (double2*)R_out;
-  // (rows per lane, panel workgroup size = 64 x panel width); a complex
-  // double is 4 VGPRs and 16 waves of 1024 threads leave 128 VGPRs per lane
-  if (m <= 640)
-    qr_blocked<10, 1024>(A, (int)m, (int)n, rs, cs, tol, Vs, tau, colnrm, Tall, R, s);    // panel 16
-  else if (m <= 1280)
-    qr_blocked<20, 1024>(A, (int)m, (int)n, rs, cs, tol, Vs, tau, colnrm, Tall, R, s);    // panel 16
+  // <panel rows per thread, panel width, panel threads, WY rows per lane
+  //  (Q formation for n > 256), apply rows per thread, apply threads>; each
+  // variant needs m within rows per thread * threads and 64 * WY rows
+  if (m <= 256)
+    qr_blocked<1, 16, 256, 4, 1, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+  else if (m <= 512)
+    qr_blocked<2, 16, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+  else if (m <= 768)
+    qr_blocked<3, 16, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+  else if (m <= 1152)
+    qr_blocked<3, 16, 384, 18, 3, 384>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
   else if (m <= 2560)
-    qr_blocked<40, 512>(A, (int)m, (int)n, rs, cs, tol, Vs, tau, colnrm, Tall, R, s);     // panel 8
+    qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
   else
-    hipLaunchKernelGGL(qr_householder_kernel, dim3(1), dim3(QR_NT), 0, s, A, (int)m, (int)n, rs, cs, tol, tau,
+    hipLaunchKernelGGL(qr_householder_kernel, dim3(1), dim3(QR_NT), 0, s, A, (int)m, (int)n, rs, cs, tol, q.tau,
                        R);
   HIPCHK(hipGetLastError());
 }

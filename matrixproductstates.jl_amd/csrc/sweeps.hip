@@ -41,7 +41,7 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl) {
   scr.add(&tmp, site * cz);
   ovl.assign(std::max(n_ovl, 0), nullptr);
   for (auto& p : ovl) scr.add(&p, site * cz);
-  scr.add(&tau, qr_workspace_bytes(Dm * dm, Dm));
+  scr.add(&tau, qr_workspace_bytes(Dm * dm, Dm, c->num_cus));
   scr.add(&R, (size_t)Dm * Dm * dm * cz);
   scr.add(&envA, big * cz);
   scr.add(&envB, big * cz);
@@ -65,7 +65,7 @@ static void run_qr(Plan& p, void* base, i64 m, i64 n, i64 rs, i64 cs, void* Rout
   const bool prof = c->profiling && !c->capturing;
   hipEvent_t a = nullptr;
   if (prof) c->prof_begin(a);
-  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, nullptr, Rout, c->stream);
+  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, c->num_cus, Rout, c->stream);
   // zgeqrf + zungqr: 2 x 8 (m n^2 - n^3 / 3) flops; panel read and written once
   if (prof) c->prof_end(a, 1, 16.0 * ((double)m * n * n - (double)n * n * n / 3.0), 32.0 * (double)m * n);
 }
