@@ -231,6 +231,26 @@ __device__ __forceinline__ double xor4_d(double v) {
   return __hiloint2double(hi, lo);
 }
 
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// (j, l) of the p-th pair j < l, ordered by l then j
+__host__ __device__ constexpr int pair_l(int p) {
+  int l = 1;
+  while (p >= l) { p -= l; ++l; }
+  return l;
+}
+__host__ __device__ constexpr int pair_j(int p) {
+  int l = 1;
+  while (p >= l) { p -= l; ++l; }
+  return p;
+}
+
 // Butterfly reduce-scatter of 8 doubles over a wave: afterwards lane l holds
 // the wave total of v[l >> 3].  The exchanges across 32 and 16 lanes halve
 // the values a lane carries (lane swaps), the one across 8 lanes picks one
@@ -274,7 +294,9 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
   constexpr int GMAX = (2 * (NB - 1) + 7) / 8;            // 8-value groups per column step
   static_assert(GMAX <= 8, "panel too wide");
   __shared__ double red[2][NW * GMAX * 64];
-  __shared__ double gram[NB * NW * 64];
+  constexpr int NP = NB * (NB - 1) / 2;                   // Gram pairs
+  constexpr int NPB = (NP + 31) / 32;                     // blocks of 8 groups of 4 pairs
+  __shared__ double gram[NW * NPB * 64];
   __shared__ double2 s_G[NB][NB];
   __shared__ double2 s_T[NB][NB + 1];
   __shared__ double2 s_tau[NB];
@@ -431,86 +453,60 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     segment(std::integral_constant<int, 4>{}, 4, nbp);
   }
   __syncthreads();
-  // Gram: G[l-1-s][l] = v_{l-1-s}^H v_l for history slot s < l; the history
-  // h[.][s] holds v_{l-1-s} (rows of this thread), shifted once per l
-  double2 h[R][NB - 1];
-#pragma unroll
-  for (int i = 0; i < R; ++i)
-#pragma unroll
-    for (int s = 0; s < NB - 1; ++s) h[i][s] = make_double2(0.0, 0.0);
-  double2 vn[R];
+  // Gram entries G[j][l] = v_j^H v_l (j < l) from this thread's own rows of
+  // Vs (explicit zeros above each reflector's diagonal, so the sums run over
+  // rows >= l by themselves): 4 pairs per butterfly, 8 butterflies per
+  // stored block, lane t keeping group (t & 7)'s value (t >> 3)
+  double2 b[R][NB];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int rr = tid + NT * i;
-    const double2 x = Vs[(int64_t)c0 * m + c0 + min(rr, M - 1)];
-    vn[i] = (rr < M) ? x : make_double2(0.0, 0.0);
-  }
-  // columns l in [lb, le) need history slots 0..l-1 < S
-  auto gseg = [&](auto Sc, int lb, int le) {
-    constexpr int S = decltype(Sc)::value;
-    constexpr int GS = (2 * S + 7) / 8;
-#pragma unroll 1
-  for (int l = lb; l < le; ++l) {
-    double2 v[R];
 #pragma unroll
-    for (int i = 0; i < R; ++i) v[i] = vn[i];
-    // next column's rows (this thread's own earlier stores) in flight meanwhile
-    const int ln = min(l + 1, nbp - 1);
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const int rr = tid + NT * i;
-      const double2 x = Vs[(int64_t)(c0 + ln) * m + c0 + min(rr, M - 1)];
-      vn[i] = (rr < M) ? x : make_double2(0.0, 0.0);
+    for (int j = 0; j < NB; ++j) {
+      const double2 x = Vs[(int64_t)(c0 + min(j, nbp - 1)) * m + c0 + min(rr, M - 1)];
+      b[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
     }
+  }
+  static_for<0, NPB>([&](auto bc) {
+    constexpr int blk = decltype(bc)::value;
     double keep = 0.0;
+    static_for<0, 8>([&](auto gc) {
+      constexpr int gg = decltype(gc)::value;
+      constexpr int g = 8 * blk + gg;
+      if constexpr (4 * g < NP) {
+        double v8[8];
+        static_for<0, 4>([&](auto pc) {
+          constexpr int pq = decltype(pc)::value;
+          constexpr int p = 4 * g + pq;
+          double gr = 0.0, gi = 0.0;
+          if constexpr (p < NP) {
+            constexpr int l = pair_l(p), j = pair_j(p);
 #pragma unroll
-    for (int g = 0; g < GS; ++g) {
-      double v8[8];
-#pragma unroll
-      for (int p = 0; p < 4; ++p) {
-        const int s = 4 * g + p;
-        double gr = 0.0, gi = 0.0;
-        if (s < S) {
-#pragma unroll
-          for (int i = 0; i < R; ++i) {
-            gr += h[i][s].x * v[i].x + h[i][s].y * v[i].y;
-            gi += h[i][s].x * v[i].y - h[i][s].y * v[i].x;
+            for (int i = 0; i < R; ++i) {
+              gr += b[i][j].x * b[i][l].x + b[i][j].y * b[i][l].y;
+              gi += b[i][j].x * b[i][l].y - b[i][j].y * b[i][l].x;
+            }
           }
-        }
-        v8[2 * p] = gr;
-        v8[2 * p + 1] = gi;
+          v8[2 * pq] = gr;
+          v8[2 * pq + 1] = gi;
+        });
+        const double t = wave_rs8(v8);
+        keep = ((lane & 7) == gg) ? t : keep;
       }
-      const double r8 = wave_rs8(v8);
-      keep = ((lane & 7) == g) ? r8 : keep;     // value q of group g at lane 8q + g
-    }
-    gram[(l * NW + wave) * 64 + lane] = keep;
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-#pragma unroll
-      for (int s = S - 1; s > 0; --s) h[i][s] = h[i][s - 1];
-      h[i][0] = v[i];
-    }
-  }
-  };
-  if constexpr (NB == 16) {
-    gseg(std::integral_constant<int, 4>{}, 0, min(nbp, 4));
-    gseg(std::integral_constant<int, 8>{}, 4, min(nbp, 8));
-    gseg(std::integral_constant<int, 12>{}, 8, min(nbp, 12));
-    gseg(std::integral_constant<int, 15>{}, 12, nbp);
-  } else {
-    gseg(std::integral_constant<int, 4>{}, 0, min(nbp, 4));
-    gseg(std::integral_constant<int, 7>{}, 4, nbp);
-  }
+    });
+    gram[(wave * NPB + blk) * 64 + lane] = keep;
+  });
   __syncthreads();
   for (int e = tid; e < NB * NB; e += NT) {
     const int j = e % NB, l = e / NB;
     double2 gv = make_double2(0.0, 0.0);
     if (j < l && l < nbp) {
-      const int s = l - 1 - j, g = s / 4, q = 2 * (s % 4);
+      // pair p = l(l-1)/2 + j: group p/4 (block g/8, lane slot g%8), values 2(p%4), 2(p%4)+1
+      const int p = l * (l - 1) / 2 + j, g = p / 4, f = 2 * (p % 4);
 #pragma unroll
       for (int w = 0; w < NW; ++w) {
-        gv.x += gram[(l * NW + w) * 64 + 8 * q + g];
-        gv.y += gram[(l * NW + w) * 64 + 8 * (q + 1) + g];
+        gv.x += gram[(w * NPB + g / 8) * 64 + 8 * f + (g % 8)];
+        gv.y += gram[(w * NPB + g / 8) * 64 + 8 * (f + 1) + (g % 8)];
       }
     }
     s_G[j][l] = gv;
