@@ -100,6 +100,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch kernels eagerly instead of replaying the captured RK4 step")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run a device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
@@ -108,6 +112,17 @@ def main():
 
     import mpsamd as M
     ctx = M.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
+    if args.calib_copy_mib > 0:
+        # one-site states of side n: the library's copy kernel moves n*n*16 B each way
+        n = int((args.calib_copy_mib * (1 << 20) // 16) ** 0.5)
+        src, dst = M.MPS(1, [1], [n, n], ctx), M.MPS(1, [1], [n, n], ctx)
+        for _ in range(2):
+            dst.copy_from(src)
+        ctx.synchronize()
+        print(f"calibration copy: {n * n * 16} bytes read and written per launch", file=sys.stderr, flush=True)
+        del src, dst
     p, z, L, rho, rhos, envs, envop, dims = setup(args.N, args.D, args.n_exp, args.dt, args.seed, ctx)
     dt = args.dt
 
