@@ -901,9 +901,103 @@ int mps_compress_sum_var_apply_H(mps_state_t A0, mps_env_t envop, int numv, cons
   });
 }
 
+// compressionSVD! (compression.jl:1280-1312), right to left: site i times the
+// carried r, SVD of reshape(., dl, d*dnew), truncation by the cumulative
+// weight rule, site i <- V[:, 1:dnew]^H, r <- U[:, 1:dnew] S.  The SVD is the
+// one-sided Jacobi of svd.hip on whichever of X^H / X has fewer columns.
 int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glob, double* err_trim,
                         double* ent_ent) {
-  return guard([&] { throw Error(MPS_E_STATE, "compressionSVD!: device SVD not built yet"); });
+  return guard([&] {
+    State& st = s->s;
+    REQUIRE(dmax >= 1, "compressionSVD!: dmax >= 1 required");
+    REQUIRE(st.dims[0] == 1 && st.dims[st.N] == 1, "compressionSVD!: open boundary bonds must be 1");
+    STATE_PLAN(st, s->owner, 0);
+    Ctx* c = st.ctx;
+    const std::vector<i64> old = st.dims;
+    // small device buffers: sigma | scale | perm | flag (p.envA is at least a site large)
+    const i64 nmax = st.max_dim() * 16 + 64;
+    double* sig_d = (double*)p.envA;
+    double* scl_d = sig_d + nmax;
+    int* perm_d = (int*)(scl_d + nmax);
+    int* flag_d = perm_d + nmax;
+    int* flag_h = (int*)c->hscal;
+    launch_set_scalar(p.AL, 1.0, 0.0, c->stream);   // r = 1 (1 x 1)
+    i64 dnew = 1;
+    std::vector<double> sig;
+    for (int i = st.N - 1; i >= 0; --i) {
+      const i64 dl = old[i], d = st.d[i], dr = old[i + 1];
+      // tmp (dl d x dnew) = reshape(site, dl d, dr) * r (dr x dnew)
+      Gemm g;
+      g.M = dl * d; g.N = dnew; g.K = dr;
+      g.A = st.site(i); g.am = im1(1); g.ak = im1(dl * d);
+      g.B = p.AL; g.bk = im1(1); g.bn = im1(dr);
+      g.C = p.tmp; g.cm = im1(1); g.cn = im1(dl * d);
+      run_gemm(c, g, CMP_F64, p.splitk, p.splitk_bytes);
+      // X = reshape(tmp, dl, d dnew); Y = X^H (cols = dl) or X (cols = d dnew)
+      const i64 mc = d * dnew;
+      const bool ct = mc >= dl;
+      const i64 m = ct ? mc : dl, n = ct ? dl : mc;
+      REQUIRE(nmax > n, "compressionSVD!: bond dimension too large");
+      svd_load(p.T1, p.tmp, m, n, ct, c->stream);
+      // rotation threshold: rounding in y_p^H y_q over m rows is ~sqrt(m) eps sqrt(alpha beta)
+      svd_jacobi(p.T1, m, n, p.T2, 4.0 * std::sqrt((double)m) * 2.220446049250313e-16, flag_d, flag_h, 60,
+                 c->stream);
+      svd_colnorms(p.T1, m, n, sig_d, c->stream);
+      sig.assign(n, 0.0);
+      HIPCHK(hipMemcpyAsync(sig.data(), sig_d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      c->sync();
+      // s = s[s .> 0], descending
+      std::vector<int> order;
+      for (int k = 0; k < n; ++k)
+        if (sig[k] > 0.0) order.push_back(k);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sig[a] > sig[b]; });
+      REQUIRE(!order.empty(), "compressionSVD!: zero site");
+      double stot = 0.0;
+      for (int k : order) stot += sig[k] * sig[k];
+      i64 cnt = 0;
+      double cum = 0.0;
+      for (int k : order) {
+        cum += sig[k] * sig[k] / stot;
+        if (cum < 1.0 - tol) ++cnt;
+      }
+      const i64 dn = std::min<i64>(cnt + 1, dmax);
+      REQUIRE(dn <= (i64)order.size(), "compressionSVD!: truncation rank exceeds the number of singular values");
+      double trim = 0.0, ent = 0.0;
+      for (size_t k = 0; k < order.size(); ++k) {
+        const double s2 = sig[order[k]] * sig[order[k]];
+        if ((i64)k >= dn) trim += s2;
+        ent += s2 * std::log(sig[order[k]]);
+      }
+      err_trim[i] = trim / stot;
+      ent_ent[i] = -2.0 * ent;
+      std::vector<double> scl(dn);
+      std::vector<int> perm(order.begin(), order.begin() + dn);
+      for (i64 k = 0; k < dn; ++k) scl[k] = ct ? 1.0 / sig[perm[k]] : sig[perm[k]];
+      HIPCHK(hipMemcpyAsync(perm_d, perm.data(), dn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(scl_d, scl.data(), dn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      // site i <- reshape(V[:, 1:dn]^H, dn, d, dnew): V = Y Sigma^-1 (ct) or J
+      if (ct)
+        svd_gather(st.site(i), p.T1, m, perm_d, scl_d, dn, m, true, false, c->stream);
+      else
+        svd_gather(st.site(i), p.T2, n, perm_d, nullptr, dn, n, true, false, c->stream);
+      // r (dl x dn) = U[:, 1:dn] S: J S (ct) or Y
+      if (ct) {
+        for (i64 k = 0; k < dn; ++k) scl[k] = sig[perm[k]];
+        HIPCHK(hipMemcpyAsync(scl_d, scl.data(), dn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        svd_gather(p.AL, p.T2, n, perm_d, scl_d, dn, dl, false, true, c->stream);
+      } else {
+        svd_gather(p.AL, p.T1, m, perm_d, nullptr, dn, dl, false, true, c->stream);
+      }
+      c->sync();   // host vectors scl / perm are reused next site
+      st.dims[i] = dn;
+      dnew = dn;
+    }
+    double r0[2];
+    HIPCHK(hipMemcpyAsync(r0, p.AL, sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    *err_glob = 1.0 - (r0[0] * r0[0] + r0[1] * r0[1]);
+    invalidate(s->owner);
+  });
 }
 
 // ----------------------------------------------------------------- RK4

@@ -96,7 +96,12 @@ void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s);
 void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s);
 void launch_convert(void* dst, DType td, const void* src, DType ts, int64_t n, hipStream_t s);
 void launch_trace_small(const void* M, int64_t d, void* out, hipStream_t s);
-void launch_jacobi_svd(void* A, int64_t m, int64_t n, void* V, double* S, void* work, int* info,
-                       hipStream_t s);
+// one-sided Jacobi SVD (svd.hip)
+void svd_load(void* Y, const void* X, int64_t m, int64_t n, bool conj_transpose, hipStream_t s);
+int svd_jacobi(void* Y, int64_t m, int64_t n, void* J, double tol, int* flag_dev, int* flag_host,
+               int max_sweeps, hipStream_t s);
+void svd_colnorms(const void* Y, int64_t m, int64_t n, double* sigma, hipStream_t s);
+void svd_gather(void* out, const void* S, int64_t ld, const int* perm, const double* scale, int64_t rows,
+                int64_t cols, bool conj, bool by_col, hipStream_t s);
 
 }  // namespace mpship

@@ -317,3 +317,36 @@ def test_arnoldi_expm(rng):
     cgot = M.Arnoldi_expm_(M.MPO(H), grhov, numv, None, M.build_env(dims, dims, wd))
     assert np.abs(cgot - cref).max() < 1e-9
     assert rel(Dn.mps_to_dense(grhov[0].to_host()), Dn.mps_to_dense(rhov[0])) < 1e-9
+
+
+def _phase_aligned_rel(a, b):
+    """Relative difference of two dense states up to one global phase."""
+    ov = np.vdot(b, a)
+    ph = ov / abs(ov) if abs(ov) > 0 else 1.0
+    return rel(a, b * ph)
+
+
+@pytest.mark.parametrize("N,D,d,dmax,tol", [(6, 8, 2, 64, 1e-15), (6, 8, 2, 2, 1e-15), (8, 16, 3, 5, 1e-15),
+                                            (7, 12, 2, 64, 1e-6), (20, 32, 2, 16, 1e-15)])
+def test_compression_svd(rng, N, D, d, dmax, tol):
+    # compressionSVD! (compression.jl:1280-1312) on a left-canonical random state:
+    # same truncation (dims), same spectra (err_trim, ent_ent, err_glob) and the
+    # same truncated state up to the global phase the reference drops with r
+    print("case", N, D, d, dmax, tol, flush=True)
+    a, _ = O.mpsrandom(N, D, d, rng)
+    O.leftorthonormalizationQR_(a)
+    ref = copy_sites(a)
+    eg_r, et_r, ee_r = O.compressionSVD_(ref, dmax, tol)
+    g = dev_state(a)
+    eg, et, ee = M.compressionSVD_(g, dmax, tol)
+    got = g.to_host()
+    assert [s.shape for s in got] == [s.shape for s in ref]
+    assert abs(eg - eg_r) < 1e-10
+    assert np.abs(et - et_r).max() < 1e-10
+    assert np.abs(ee - ee_r).max() < 1e-10
+    if N <= 8:
+        assert _phase_aligned_rel(Dn.mps_to_dense(got), Dn.mps_to_dense(ref)) < 1e-10
+    # right-canonical sites: sum_{s,b} B[a,s,b] conj(B[a',s,b]) = I
+    for B in got[1:]:
+        M2 = B.reshape(B.shape[0], -1, order="F")
+        assert np.abs(M2 @ M2.conj().T - np.eye(B.shape[0])).max() < 1e-12
