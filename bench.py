@@ -100,6 +100,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 and FP32+refine forms")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch kernels eagerly instead of replaying the captured RK4 step")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -125,6 +126,7 @@ def main():
         del src, dst
     p, z, L, rho, rhos, envs, envop, dims = setup(args.N, args.D, args.n_exp, args.dt, args.seed, ctx)
     dt = args.dt
+    rho0 = rho.to_host()
 
     def step():
         M.RK4stp_apply_H_half_(dt, L, rho, rhos[0], rhos[1], rhos[2], *envs, envop)
@@ -141,7 +143,69 @@ def main():
     sweeps = args.steps / elapsed
     print(f"timed {args.steps} steps: {elapsed:.3f} s", file=sys.stderr, flush=True)
 
-    tr = M.scal_prod_no_conj(M.MPS.from_host(__import__("mpsamd").models.identity_vec_mps(args.N), ctx), rho)
+    from mpsamd import models as MD
+    idm = M.MPS.from_host(MD.identity_vec_mps(args.N), ctx)
+    nem = M.MPS.from_host(MD.sum_local_vec_mps(args.N, MD.HEE3), ctx)
+    nrm = M.MPS.from_host(MD.sum_local_vec_mps(args.N, MD.HSS3), ctx)
+
+    def observables(r):
+        tr_ = M.scal_prod_no_conj(idm, r)
+        return {"trace": tr_, "n_e": M.scal_prod_no_conj(nem, r) / tr_, "n_r": M.scal_prod_no_conj(nrm, r) / tr_}
+
+    tr = M.scal_prod_no_conj(idm, rho)
+    ref_obs = observables(rho)
+    rho64 = rho.to_host()
+
+    # Precision forms: the same start state and number of steps as the FP64 run.
+    #   fp32:        operator contractions in complex FP32 (3M) on MFMA; QR, norms,
+    #                overlap terms FP64
+    #   fp32_refine: the fp32 step (11-argument form, input kept) followed by two
+    #                FP64 half-sweeps (left, right) of the final RK4 stage's
+    #                variational compression, started from the fp32 result
+    prec_forms = {}
+    if not args.no_fp32:
+        coef4 = [-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3]
+        rnew, rtmp = rho.similar(), rho.similar()
+
+        def fidelity(a, b_host):
+            b = M.MPS.from_host(b_host, ctx)
+            ab = abs(M.scal_prod(a, b))
+            return ab / np.sqrt(abs(M.scal_prod(a, a)) * abs(M.scal_prod(b, b)))
+
+        for form, nref in (("fp32", 0), ("fp32_refine", 2)):
+            rho.upload(rho0)
+            for r in rhos:
+                r.copy_from(rho)
+
+            def step32():
+                ctx.set_precision("fp32")
+                if nref == 0:
+                    M.RK4stp_apply_H_half_(dt, L, rho, rhos[0], rhos[1], rhos[2], *envs, envop)
+                    return
+                M.RK4stp_apply_H_half_(dt, L, rnew, rho, rhos[0], rhos[1], rhos[2], *envs, envop)
+                ctx.set_precision("fp64")
+                A4 = [rho, rhos[0], rhos[1], rhos[2]]
+                M.compress_sum_var_apply_H_left_(rtmp, envs, envop, rnew, coef4, A4, L[2])
+                M.compress_sum_var_apply_H_right_(rnew, envs, envop, rtmp, coef4, A4, L[2])
+                rho.copy_from(rnew)
+
+            for i in range(args.warmup):
+                step32()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for i in range(args.steps):
+                step32()
+            ctx.synchronize()
+            el = time.perf_counter() - t0
+            ctx.set_precision("fp64")
+            ob = observables(rho)
+            prec_forms[form] = {
+                "value": round(args.steps / el, 4), "ms_per_step": round(1e3 * el / args.steps, 3),
+                "speedup_vs_fp64": round((args.steps / el) / sweeps, 3),
+                "err_n_e": abs(ob["n_e"] - ref_obs["n_e"]), "err_n_r": abs(ob["n_r"] - ref_obs["n_r"]),
+                "infidelity": 1.0 - fidelity(rho, rho64)}
+            print(f"{form}: {prec_forms[form]}", file=sys.stderr, flush=True)
+        rho.upload(rho64)
 
     roof = None
     prof = None
@@ -186,6 +250,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "profile": prof,
+        "precision_forms": prec_forms,
     }
     print(json.dumps(out))
 

@@ -305,3 +305,22 @@ def local_population_vec_mps(N, j, op, d=3):
     mps = identity_vec_mps(N, d)
     mps[j] = np.asarray(op, dtype=CDT).reshape(-1, order="F").reshape(1, d * d, 1).copy()
     return mps
+
+
+def sum_local_vec_mps(N, op, d=3):
+    """Vectorised sum_j op_j (bond dimension 2), so that one ``scal_prod_no_conj``
+    with a vectorised density matrix gives sum_j tr(op_j rho) -- the total
+    population that the reference accumulates site by site (``e_pop`` / ``s_pop``,
+    rydberg_density_matrix_gpu.jl:174-175)."""
+    i = np.eye(d, dtype=CDT).reshape(-1, order="F")
+    o = np.asarray(op, dtype=CDT).reshape(-1, order="F")
+    dd = d * d
+    first = np.zeros((1, dd, 2), dtype=CDT)
+    first[0, :, 0], first[0, :, 1] = i, o
+    last = np.zeros((2, dd, 1), dtype=CDT)
+    last[0, :, 0], last[1, :, 0] = o, i
+    bulk = np.zeros((2, dd, 2), dtype=CDT)
+    bulk[0, :, 0], bulk[0, :, 1], bulk[1, :, 1] = i, o, i
+    if N == 1:
+        return [(i * 0 + o).reshape(1, dd, 1)]
+    return [first] + [bulk.copy() for _ in range(N - 2)] + [last]

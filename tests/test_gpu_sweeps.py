@@ -350,3 +350,26 @@ def test_compression_svd(rng, N, D, d, dmax, tol):
     for B in got[1:]:
         M2 = B.reshape(B.shape[0], -1, order="F")
         assert np.abs(M2 @ M2.conj().T - np.eye(B.shape[0])).max() < 1e-12
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_rk4_fp32_operator_terms(graphs):
+    # Precision form: the operator contractions run in complex FP32 (3M MFMA,
+    # FP32 accumulation) while QR, norms and overlap terms stay FP64.  Against
+    # the FP64 oracle the error must sit at the FP32 level: above FP64 round-off
+    # (the FP32 kernel really ran) and within 1e-5 of the dense state.
+    N, D, dt, steps = 8, 8, 0.01, 4
+    print("case fp32", graphs, flush=True)
+    z, H, psi, dims = _spin_setup(N, D, dt, "random")
+    wop = [1] + [4] * (N - 1) + [1]
+    ref, rn = _oracle_rk4(H, psi, dims, dt, steps, wop)
+    ctx = M.default_context()
+    ctx.set_precision("fp32")
+    try:
+        g, gn = _device_rk4(H, psi, dims, dt, steps, wop, False, graphs)
+    finally:
+        ctx.set_precision("fp64")
+    err = rel(Dn.mps_to_dense(g.to_host()), Dn.mps_to_dense(ref))
+    print("fp32 relative state error", err, flush=True)
+    assert 1e-12 < err < 1e-5
+    assert np.allclose(gn, rn, rtol=1e-5, atol=0)
