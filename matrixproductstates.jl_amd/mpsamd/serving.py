@@ -1,0 +1,232 @@
+"""Batched trajectories and a continuously batched evolution server.
+
+Every solve (a quantum-jump trajectory or a density-matrix evolution) owns a
+library context, i.e. its own HIP stream, workspace arena and captured RK4
+graph, and keeps all of its tensors resident in HBM until it finishes.  One
+scheduler tick first queues one step of every active solve (graph launches,
+no host wait), then collects the host-side scalars (norms, jump draws) solve
+by solve.  Independent solves therefore overlap on the device: the QR panels
+and small GEMMs of one chain occupy a few CUs, and other chains fill the rest.
+
+The server admits requests as they arrive (Poisson, seeded), while the HBM
+footprint of the active set stays under a budget and a slot is free, and
+reports throughput and p50/p99 latency.  A request's result does not depend
+on what else is running: each solve has its own stream and workspace, and
+every kernel reduces in a fixed order.
+"""
+from __future__ import annotations
+
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# more hardware queues than HIP's default of 4, so that many streams run
+# concurrently; only effective before the first HIP call of the process
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
+from . import core as M  # noqa: E402
+from . import models as MD  # noqa: E402
+from .trajectory import SpinTrajectory  # noqa: E402
+
+
+class RydbergEvolution:
+    """Density-matrix RK4 evolution of the Rydberg ladder (rydberg_density_matrix_gpu.jl:211-222)
+    with the same launch/finish interface as ``SpinTrajectory``."""
+
+    def __init__(self, N, D, seed, dt=0.005, n_exp=2, ctx: M.Context | None = None):
+        self.N, self.D, self.dt = N, D, dt
+        self.ctx = ctx if ctx is not None else M.default_context()
+        p = MD.RydbergParams(n_exp=n_exp)
+        self.z = MD.positions(N, float(N), seed)
+        self.L = [M.MPO(h, self.ctx) for h in MD.rydberg_rk4_mpos(p, self.z, dt)]
+        self.rho, self.dims = M.mpsgroundstate(N, D, 9, self.ctx)
+        self.r = [self.rho.similar() for _ in range(3)]
+        for s in self.r:
+            s.copy_from(self.rho)
+        self.env = [M.build_env(self.dims, self.dims, ctx=self.ctx) for _ in range(3)]
+        self.envop = M.build_env(self.dims, self.dims, [1] + [p.w] * (N - 1) + [1], ctx=self.ctx)
+        self.idm = M.MPS.from_host(MD.identity_vec_mps(N), self.ctx)
+        self.nrm = M.MPS.from_host(MD.sum_local_vec_mps(N, MD.HSS3), self.ctx)
+        self.w = p.w
+
+    def launch_coherent(self):
+        M.RK4stp_apply_H_half_(self.dt, self.L, self.rho, *self.r, *self.env, self.envop)
+
+    def finish_step(self):
+        self.ctx.synchronize()
+        return 1.0
+
+    def result(self):
+        tr = M.scal_prod_no_conj(self.idm, self.rho)
+        return np.array([tr.real, (M.scal_prod_no_conj(self.nrm, self.rho) / tr).real])
+
+    def footprint_bytes(self):
+        site = sum(self.dims[j] * 9 * self.dims[j + 1] for j in range(self.N))
+        ovl = sum(self.dims[j] ** 2 for j in range(self.N + 1))
+        env_op = sum(self.dims[j] ** 2 * self.w for j in range(self.N + 1))
+        return 16 * (4 * site + 3 * ovl + env_op + 3 * self.N * self.w * self.w * 81)
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str            # "spin" (quantum-jump trajectory, d = 2) or "rydberg" (density matrix, d = 9)
+    N: int
+    D: int
+    steps: int
+    seed: int
+    arrival: float = 0.0
+    start: float = 0.0
+    done: float = 0.0
+    result: np.ndarray | None = None
+    footprint: int = 0
+    solve: object = field(default=None, repr=False)
+    left: int = 0
+
+
+def make_solve(req: Request, ctx: M.Context):
+    if req.kind == "spin":
+        return SpinTrajectory(req.N, req.D, req.seed, dt=0.01, ctx=ctx)
+    return RydbergEvolution(req.N, req.D, req.seed, ctx=ctx)
+
+
+def estimate_footprint(req: Request) -> int:
+    """HBM bytes of a request before it is built: states + environments + MPOs, complex FP64,
+    at maximal bond dimension, plus the workspace arena (about 4 sites of w D^2 d)."""
+    d = 2 if req.kind == "spin" else 9
+    w = 4 if req.kind == "spin" else 10
+    nst = 7 if req.kind == "spin" else 4
+    D = req.D
+    site = req.N * D * D * d
+    return 16 * (nst * site + 4 * req.N * D * D * w + 4 * D * D * d * w)
+
+
+def release(solve):
+    """Destroy a solve's device objects, then its context (objects must go first:
+    they reference the context)."""
+    for v in list(vars(solve).values()):
+        for o in (v if isinstance(v, (list, tuple)) else [v]):
+            if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                o.__del__()
+    solve.ctx.close()
+
+
+def _result(solve):
+    if isinstance(solve, SpinTrajectory):
+        return np.concatenate([solve.sigma_z(), [solve.p_coh[-1] if solve.p_coh else 1.0]])
+    return solve.result()
+
+
+def run_batch(reqs, sequential=False):
+    """Run every request to completion; returns wall seconds.  ``sequential``
+    solves them one after another on one context (the plain baseline)."""
+    t0 = time.perf_counter()
+    if sequential:
+        ctx = M.Context(0)
+        for r in reqs:
+            r.solve = make_solve(r, ctx)
+            for _ in range(r.steps):
+                r.solve.launch_coherent()
+                r.solve.finish_step()
+            r.result = _result(r.solve)
+            for v in list(vars(r.solve).values()):
+                for o in (v if isinstance(v, (list, tuple)) else [v]):
+                    if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                        o.__del__()
+            r.solve = None
+        ctx.close()
+        return time.perf_counter() - t0
+    active = []
+    for r in reqs:
+        r.solve = make_solve(r, M.Context(0))
+        r.left = r.steps
+        active.append(r)
+    while active:
+        for r in active:
+            r.solve.launch_coherent()
+        for r in active:
+            r.solve.finish_step()
+            r.left -= 1
+        for r in [r for r in active if r.left == 0]:
+            r.result = _result(r.solve)
+            release(r.solve)
+            r.solve = None
+            active.remove(r)
+    return time.perf_counter() - t0
+
+
+class Server:
+    """Continuous batching over a request stream.
+
+    ``requests`` carry arrival times (seconds from start).  Each tick admits
+    the arrived requests that fit (slots, HBM budget; first come, first
+    served), advances every active request by one step, and retires the
+    finished ones.  Latency = completion - arrival.
+    """
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 16):
+        self.budget = hbm_budget_bytes
+        self.max_active = max_active
+
+    def run(self, requests):
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active, done = [], []
+        used = 0
+        t0 = time.perf_counter()
+        busy = 0.0
+        while pending or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                r = pending[0]
+                r.footprint = estimate_footprint(r)
+                if used + r.footprint > self.budget and active:
+                    break
+                pending.pop(0)
+                r.start = now
+                r.solve = make_solve(r, M.Context(0))
+                r.left = r.steps
+                used += r.footprint
+                active.append(r)
+            if not active:
+                time.sleep(max(0.0, min(pending[0].arrival - now, 0.01)))
+                continue
+            tb = time.perf_counter()
+            for r in active:
+                r.solve.launch_coherent()
+            for r in active:
+                r.solve.finish_step()
+                r.left -= 1
+            busy += time.perf_counter() - tb
+            for r in [r for r in active if r.left == 0]:
+                r.result = _result(r.solve)
+                r.done = time.perf_counter() - t0
+                release(r.solve)
+                r.solve = None
+                used -= r.footprint
+                active.remove(r)
+                done.append(r)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.done - r.arrival for r in done])
+        return {"requests": len(done), "wall_s": wall, "busy_s": busy,
+                "throughput_req_s": len(done) / wall,
+                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+
+
+def poisson_requests(n, rate, seed, spin_frac=0.7, N_range=(20, 60), D_choices=(16, 32), steps_range=(10, 40),
+                     rydberg_N_range=(10, 30), rydberg_D_choices=(16, 32)):
+    """Seeded request stream: exponential inter-arrival times at ``rate`` per second."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < spin_frac:
+            out.append(Request(i, "spin", int(rng.integers(*N_range, endpoint=True)), int(rng.choice(D_choices)),
+                               int(rng.integers(*steps_range, endpoint=True)), int(rng.integers(1 << 30)), t))
+        else:
+            out.append(Request(i, "rydberg", int(rng.integers(*rydberg_N_range, endpoint=True)),
+                               int(rng.choice(rydberg_D_choices)), int(rng.integers(*steps_range, endpoint=True)),
+                               int(rng.integers(1 << 30)), t))
+    return out

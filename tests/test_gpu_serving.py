@@ -1,0 +1,60 @@
+"""Batched trajectories and the continuously batched server: a request's result
+is the same alone and in the mix, and matches the FP64 oracle."""
+import numpy as np
+import pytest
+
+from mpsamd import models
+from mpsamd import serving as S
+from oracle import trajectory as OT
+
+pytestmark = pytest.mark.gpu
+
+SAME = 1e-12   # alone vs in the mix (separate streams, fixed-order reductions: expected bit-identical)
+
+
+def _req(rid, kind, N, D, steps, seed, arrival=0.0):
+    return S.Request(rid, kind, N, D, steps, seed, arrival)
+
+
+def test_batch_result_same_alone_and_in_the_mix():
+    alone = [_req(0, "spin", 12, 8, 12, 7)]
+    S.run_batch(alone)
+    mix = [_req(0, "spin", 12, 8, 12, 7), _req(1, "spin", 16, 16, 9, 3), _req(2, "rydberg", 6, 9, 5, 2),
+           _req(3, "spin", 10, 4, 15, 5)]
+    S.run_batch(mix)
+    d = np.abs(mix[0].result - alone[0].result).max()
+    print("alone vs mix max difference", d, flush=True)
+    assert d <= SAME
+    seq = [_req(i, r.kind, r.N, r.D, r.steps, r.seed) for i, r in enumerate(mix)]
+    S.run_batch(seq, sequential=True)
+    for a, b in zip(seq, mix):
+        assert np.abs(a.result - b.result).max() <= SAME
+
+
+def test_spin_request_matches_oracle():
+    # full bond dimension (N = 8, D = 16): the device trajectory, the oracle
+    # trajectory and their sigma_z profiles agree within 1e-10
+    r = _req(0, "spin", 8, 16, 20, 11)
+    S.run_batch([r, _req(1, "rydberg", 5, 9, 4, 1)])
+    z = models.positions(8, 8.0, 11)
+    H = models.spin_rk4_mpos(z, 0.0, 0.01)
+    jr, jl = models.spin_jump_mpos(z)
+    ref = OT.OracleSpinTrajectory(8, 16, 11, H, jr, jl, 0.01)
+    for _ in range(20):
+        ref.step()
+    assert np.abs(r.result[:-1] - ref.sigma_z()).max() < 1e-10
+    assert abs(r.result[-1] - ref.p_coh[-1]) < 1e-10
+
+
+def test_server_poisson_stream():
+    reqs = S.poisson_requests(8, rate=50.0, seed=4, N_range=(8, 16), D_choices=(4, 8), steps_range=(4, 10),
+                              rydberg_N_range=(4, 8), rydberg_D_choices=(9,))
+    stats = S.Server(hbm_budget_bytes=2 << 30, max_active=4).run(reqs)
+    print(stats, flush=True)
+    assert stats["requests"] == len(reqs)
+    assert 0 < stats["p50_latency_s"] <= stats["p99_latency_s"]
+    # every response equals the same request solved alone
+    for r in reqs:
+        a = S.Request(r.rid, r.kind, r.N, r.D, r.steps, r.seed)
+        S.run_batch([a])
+        assert np.abs(a.result - r.result).max() <= SAME
