@@ -20,6 +20,11 @@ import os
 import sys
 import time
 
+# more hardware queues than HIP's default of 4 for the concurrent solves of the
+# batched / serving legs (read once, at the first HIP call of the process)
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "matrixproductstates.jl_amd"))
@@ -88,6 +93,36 @@ def cpu_baseline(N, D, n_exp, dt, seed, steps_cpu=1):
                       f"(numpy/OpenBLAS zgemm + LAPACK zgeqrf/zungqr), {t:.1f} s"}
 
 
+def batch_and_serve(seed):
+    """Batched quantum-jump trajectories (concurrent per-stream solves vs the same
+    solves one after another) and a Poisson request stream through the server."""
+    from mpsamd import serving as S
+    rng = np.random.default_rng(seed)
+    n_traj, steps = 16, 10
+    spec = [(int(rng.integers(40, 81)), int(rng.choice([16, 32])), int(rng.integers(1 << 30))) for _ in range(n_traj)]
+    seq = [S.Request(i, "spin", N, D, steps, sd) for i, (N, D, sd) in enumerate(spec)]
+    t_seq = S.run_batch(seq, sequential=True)
+    print(f"sequential {n_traj} trajectories: {t_seq:.2f} s", file=sys.stderr, flush=True)
+    con = [S.Request(i, "spin", N, D, steps, sd) for i, (N, D, sd) in enumerate(spec)]
+    t_con = S.run_batch(con)
+    print(f"concurrent {n_traj} trajectories: {t_con:.2f} s", file=sys.stderr, flush=True)
+    same = max(float(np.abs(a.result - b.result).max()) for a, b in zip(seq, con))
+    batched = {"trajectories": n_traj, "steps_each": steps, "N_range": [40, 80], "D_choices": [16, 32],
+               "trajectories_per_s": round(n_traj / t_con, 3),
+               "sequential_trajectories_per_s": round(n_traj / t_seq, 3),
+               "speedup_vs_sequential": round(t_seq / t_con, 3), "max_abs_diff_vs_sequential": same}
+    reqs = S.poisson_requests(40, rate=50.0, seed=seed, N_range=(20, 60), D_choices=(16, 32),
+                              steps_range=(5, 20), rydberg_N_range=(10, 30), rydberg_D_choices=(16,))
+    st = S.Server(hbm_budget_bytes=64 << 30, max_active=16).run(reqs)
+    print(f"serving: {st}", file=sys.stderr, flush=True)
+    serving = {"requests": st["requests"], "arrival_rate_req_s": 50.0,
+               "mix": "70% spin trajectories (d=2, N 20-60, D 16/32), 30% Rydberg density matrices "
+                      "(d=9, N 10-30, D 16), 5-20 steps each",
+               "throughput_req_s": round(st["throughput_req_s"], 3),
+               "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
+    return batched, serving
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +136,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 and FP32+refine forms")
+    ap.add_argument("--no-serve", action="store_true", help="skip the batched-trajectory and serving legs")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch kernels eagerly instead of replaying the captured RK4 step")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -225,6 +261,10 @@ def main():
                 "gemm_gflop_per_step": st["gemm_flops"] / args.steps / 1e9}
         print(f"profile: {prof}", file=sys.stderr, flush=True)
 
+    batched = serving = None
+    if not args.no_serve:
+        batched, serving = batch_and_serve(args.seed)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.N, args.D, args.n_exp, args.dt, args.seed)
@@ -251,6 +291,8 @@ def main():
         "cpu_baseline": cpu,
         "profile": prof,
         "precision_forms": prec_forms,
+        "batched_trajectories": batched,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -274,6 +274,8 @@ int mps_state_copy(mps_state_t dst, mps_state_t src) {
       REQUIRE(b.dims[j] <= a.cap_l[j] && b.dims[j + 1] <= a.cap_r[j], "destination capacity too small");
     }
     set_device(a.ctx);
+    // the source may still be written by work queued on another context's stream
+    if (b.ctx != a.ctx) b.ctx->sync();
     bool dims_changed = a.dims != b.dims;
     a.dims = b.dims;
     for (int j = 0; j < a.N; ++j) launch_copy_c(a.site(j), b.site(j), b.size(j), a.ctx->stream);
@@ -911,12 +913,18 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
     State& st = s->s;
     REQUIRE(dmax >= 1, "compressionSVD!: dmax >= 1 required");
     REQUIRE(st.dims[0] == 1 && st.dims[st.N] == 1, "compressionSVD!: open boundary bonds must be 1");
-    STATE_PLAN(st, s->owner, 0);
+    // small device buffers: sigma | scale | perm | flag, in the plan's extra buffer
+    // (the SVD of a site has at most max_dim * max d columns)
+    i64 dm = 1;
+    for (int j = 0; j < st.N; ++j) dm = std::max(dm, st.d[j]);
+    const i64 nmax = st.max_dim() * dm + 64;
+    set_device(st.ctx);
+    size_t before_ = st.ctx->ws_bytes;
+    Plan p(st.ctx, {&st}, {}, 0, (size_t)nmax * (2 * sizeof(double) + sizeof(int)) + 256);
+    track_ws(s->owner, before_);
     Ctx* c = st.ctx;
     const std::vector<i64> old = st.dims;
-    // small device buffers: sigma | scale | perm | flag (p.envA is at least a site large)
-    const i64 nmax = st.max_dim() * 16 + 64;
-    double* sig_d = (double*)p.envA;
+    double* sig_d = (double*)p.extra;
     double* scl_d = sig_d + nmax;
     int* perm_d = (int*)(scl_d + nmax);
     int* flag_d = perm_d + nmax;
@@ -1023,9 +1031,12 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     Plan p(c, {&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s}, {&H1->o, &H2->o, &H3->o}, 4);
     track_ws(x, before);
     key.ws_gen = x->ws_gen;
+    // Asynchronous on the context's stream (no host value is produced): the
+    // caller's next call on this context, or a scalar-returning call, orders
+    // after it; coefficients are kernel arguments, never host memory.
     if (!c->graphs || c->profiling) {
       run(p);
-      c->sync();
+      if (c->profiling) c->sync();
       return;
     }
     auto it = x->graphs.find(key);
@@ -1033,7 +1044,6 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
       int& n = x->seen[key];
       if (n++ == 0) {   // first call: eager (sizes the workspace)
         run(p);
-        c->sync();
         return;
       }
       hipGraph_t g = nullptr;
@@ -1056,7 +1066,6 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
       it = x->graphs.emplace(key, ex).first;
     }
     HIPCHK(hipGraphLaunch(it->second, c->stream));
-    c->sync();
   });
 }
 
@@ -1107,12 +1116,12 @@ int mps_measure_local(mps_state_t A, const double* op, double* out) {
     Ctx* c = s.ctx;
     set_device(c);
     size_t before = c->ws_bytes;
-    Plan p(c, {&s}, {}, 0);
-    track_ws(A->owner, before);
     i64 dmax = 1;
     for (int j = 0; j < s.N; ++j) dmax = std::max(dmax, s.d[j]);
-    // per-site d x d matrices M_n in T3 (N * dmax^2 complex)
-    double2* Ms = (double2*)p.T3;
+    // per-site d x d matrices M_n (N * dmax^2 complex) in the plan's extra buffer
+    Plan p(c, {&s}, {}, 0, (size_t)s.N * dmax * dmax * sizeof(double2));
+    track_ws(A->owner, before);
+    double2* Ms = (double2*)p.extra;
     double2* cur = (double2*)p.envA;
     double2* nxt = (double2*)p.envB;
     launch_set_scalar(cur, 1.0, 0.0, c->stream);
@@ -1148,10 +1157,10 @@ int mps_apply_site_operator(mps_state_t A, int site, const double* op) {
     Ctx* c = s.ctx;
     set_device(c);
     size_t before = c->ws_bytes;
-    Plan p(c, {&s}, {}, 0);
-    track_ws(A->owner, before);
     const i64 Dl = s.Dl(site), d = s.d[site], Dr = s.Dr(site);
-    HIPCHK(hipMemcpyAsync(p.R, op, d * d * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    Plan p(c, {&s}, {}, 0, (size_t)d * d * sizeof(double2));
+    track_ws(A->owner, before);
+    HIPCHK(hipMemcpyAsync(p.extra, op, d * d * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     // tmp[a, s, b] = sum_t A[a, t, b] op[s, t]   (batch over b)
     Gemm g;
     g.M = Dl;
@@ -1159,7 +1168,7 @@ int mps_apply_site_operator(mps_state_t A, int site, const double* op) {
     g.K = d;
     g.batch = Dr;
     g.A = s.site(site); g.am = im1(1); g.ak = im1(Dl); g.a_bs = Dl * d;
-    g.B = p.R; g.bk = im1(d); g.bn = im1(1);
+    g.B = p.extra; g.bk = im1(d); g.bn = im1(1);
     g.C = p.tmp; g.cm = im1(1); g.cn = im1(Dl); g.c_bs = Dl * d;
     run_gemm(c, g, CMP_F64, p.splitk, p.splitk_bytes);
     launch_copy_c(s.site(site), p.tmp, Dl * d * Dr, c->stream);

@@ -16,12 +16,13 @@ struct Plan {
   void *tau = nullptr, *R = nullptr, *envA = nullptr, *envB = nullptr;
   void* splitk = nullptr;
   size_t splitk_bytes = 0;
+  void* extra = nullptr;   // caller-sized buffer (extra_bytes), for per-call arrays
   Work work{};
   Op op{};
   Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
-       int n_ovl);
-  Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl); }
-  void init(i64 Dm, i64 dm, i64 wm, int n_ovl);
+       int n_ovl, size_t extra_bytes = 0);
+  Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl, 0); }
+  void init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes);
 };
 
 void qr_left(Plan& p, State* s, int j, void* Rout);

@@ -16,7 +16,7 @@ using cd = std::complex<double>;
 
 // -------------------------------------------------------------- planning
 Plan::Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
-           int n_ovl)
+           int n_ovl, size_t extra_bytes)
     : ctx(c), scr(c) {
   i64 Dm = 1, dm = 1, wm = 1;
   for (const State* s : states)
@@ -26,10 +26,10 @@ Plan::Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_
     }
   for (const MPO* o : mpos)
     if (o) wm = std::max(wm, o->max_w());
-  init(Dm, dm, wm, n_ovl);
+  init(Dm, dm, wm, n_ovl, extra_bytes);
 }
 
-void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl) {
+void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
   Ctx* c = ctx;
   const size_t cz = sizeof(double2);
   const size_t site = (size_t)Dm * dm * Dm;
@@ -49,6 +49,7 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl) {
   const size_t sk = (size_t)(4 * c->num_cus + 64) * 64 * 64 * cz;
   scr.add(&splitk, sk);
   splitk_bytes = sk;
+  if (extra_bytes) scr.add(&extra, extra_bytes);
   scr.commit();
   work.c = c;
   work.splitk = splitk;

@@ -8,6 +8,10 @@ no host wait), then collects the host-side scalars (norms, jump draws) solve
 by solve.  Independent solves therefore overlap on the device: the QR panels
 and small GEMMs of one chain occupy a few CUs, and other chains fill the rest.
 
+HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default); a
+process that serves many solves at once should raise it (bench.py sets 16)
+before its first HIP call.
+
 The server admits requests as they arrive (Poisson, seeded), while the HBM
 footprint of the active set stays under a budget and a slot is free, and
 reports throughput and p50/p99 latency.  A request's result does not depend
@@ -16,19 +20,14 @@ every kernel reduces in a fixed order.
 """
 from __future__ import annotations
 
-import os
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-# more hardware queues than HIP's default of 4, so that many streams run
-# concurrently; only effective before the first HIP call of the process
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
-
-from . import core as M  # noqa: E402
-from . import models as MD  # noqa: E402
-from .trajectory import SpinTrajectory  # noqa: E402
+from . import core as M
+from . import models as MD
+from .trajectory import SpinTrajectory
 
 
 class RydbergEvolution:
@@ -86,7 +85,15 @@ class Request:
     left: int = 0
 
 
+# Requests start from product states at truncated bond dimension, whose QR
+# panels are rank-deficient; the rank cutoff makes their Householder
+# completions exact instead of noise-seeded (a context per request, so the
+# setting stays local).
+QR_RANK_TOL = 1e-13
+
+
 def make_solve(req: Request, ctx: M.Context):
+    ctx.set_qr_rank_tol(QR_RANK_TOL)
     if req.kind == "spin":
         return SpinTrajectory(req.N, req.D, req.seed, dt=0.01, ctx=ctx)
     return RydbergEvolution(req.N, req.D, req.seed, ctx=ctx)

@@ -26,3 +26,21 @@ def _reset_oracle_tol():
     O.set_qr_rank_tol(0.0)
     yield
     O.set_qr_rank_tol(0.0)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_hip_first(request):
+    """Bring up torch's HIP runtime before the library's when GPU tests run.
+
+    The buffer-level parity helpers (mpsamd.dev) pass torch device memory to
+    the library.  torch ships its own HIP runtime next to the one the library
+    links; on the GPU hosts torch then sees no device if the library's runtime
+    opened it first, so torch goes first in every test process."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except Exception:
+            pass
+    yield

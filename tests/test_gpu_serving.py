@@ -36,6 +36,8 @@ def test_spin_request_matches_oracle():
     # trajectory and their sigma_z profiles agree within 1e-10
     r = _req(0, "spin", 8, 16, 20, 11)
     S.run_batch([r, _req(1, "rydberg", 5, 9, 4, 1)])
+    from oracle import mps_oracle as O
+    O.set_qr_rank_tol(S.QR_RANK_TOL)       # the server's per-request contexts use the cutoff
     z = models.positions(8, 8.0, 11)
     H = models.spin_rk4_mpos(z, 0.0, 0.01)
     jr, jl = models.spin_jump_mpos(z)
