@@ -71,13 +71,15 @@ __device__ __forceinline__ void stc(float2* p, double re, double im) {
 }
 
 // ------------------------------------------------------------ tile loaders
-// Each thread owns 4 elements of the A tile (BM x BK) and 4 of the B tile
-// (BK x BN).  fetch_* issues the tile's global loads into registers (clamped
-// addresses, no branches around the loads), put_* masks and stores them to
-// LDS as split planes S[plane][k][m|n]; the main loop fetches tile k+1 while
-// the MFMAs consume tile k.
+// Each thread owns BM*BK/NT elements of the A tile (BM x BK) and as many of
+// the B tile (BK x BN).  fetch_* issues the tile's global loads into
+// registers (clamped addresses, no branches around the loads), put_* masks
+// and stores them to LDS as split planes S[plane][k][m|n]; the main loop
+// fetches tile k+1 while the MFMAs consume tile k.
+constexpr int FRAG = (BM * BK) / NT;
+static_assert((BN * BK) / NT == FRAG, "A and B tiles hold the same number of elements per thread");
 struct Frag {
-  double re[4], im[4];
+  double re[FRAG], im[FRAG];
 };
 
 template <bool TWO, typename T>
@@ -174,7 +176,7 @@ __device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split
 
 // ------------------------------------------------------------ FP64 kernel
 template <typename TA, typename TB, typename TC, bool TWO>
-__global__ __launch_bounds__(NT) void zgemm_f64_kernel(GemmParams p) {
+__global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
   __shared__ double As[2][BK][BM + 1];
   __shared__ double Bs[2][BK][BN + 1];
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(NT) void zgemm_f64_kernel(GemmParams p) {
 
 // --------------------------------------------------------- FP32 3M kernel
 template <typename TA, typename TB, typename TC, bool TWO>
-__global__ __launch_bounds__(NT) void zgemm_f32_3m_kernel(GemmParams p) {
+__global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
   __shared__ float As[3][BK][BM + 1];
   __shared__ float Bs[3][BK][BN + 1];
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
@@ -365,6 +367,8 @@ static GemmPlan make_plan(const Gemm& g, int num_cus) {
   GemmPlan pl;
   pl.tiles_m = (int)((g.M + BM - 1) / BM);
   pl.tiles_n = (int)((g.N + BN - 1) / BN);
+  // Split K while the tiles fill fewer than the 2 x num_cus workgroup slots
+  // (two workgroups fit per CU).
   const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch;
   const int64_t target = 2LL * num_cus;
   int splits = 1;

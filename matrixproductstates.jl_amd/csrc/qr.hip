@@ -192,6 +192,12 @@ __global__ __launch_bounds__(QR_NT) void qr_householder_kernel(double2* __restri
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// a - v f for complex v, f = (fr, fi), with the rounding pinned by explicit fmas
+// (the panel's update and its look-ahead of the next diagonal must agree bit
+// for bit)
+__device__ __forceinline__ double2 cupd(double2 a, double2 v, double fr, double fi) {
+  return make_double2(fma(v.y, fi, fma(-v.x, fr, a.x)), fma(-v.y, fr, fma(-v.x, fi, a.y)));
+}
 __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {   // conj(a) * b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
@@ -291,7 +297,7 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
                                                       double2* __restrict__ Vs, double2* __restrict__ tau,
                                                       double2* __restrict__ Tp) {
   constexpr int NW = NT / 64;
-  constexpr int GMAX = (2 * (NB - 1) + 7) / 8;            // 8-value groups per column step
+  constexpr int GMAX = (2 * (NB - 1) + 7) / 8 + 1;        // 8-value groups per column step (+1 look-ahead)
   static_assert(GMAX <= 8, "panel too wide");
   __shared__ double red[2][NW * GMAX * 64];
   constexpr int NP = NB * (NB - 1) / 2;                   // Gram pairs
@@ -332,6 +338,10 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     return __hiloint2double(hi, lo);
   };
   int ph = 0;
+  // look-ahead: the zlarfg inputs of the next column, downdated from sums
+  // reduced together with the current column's dot products (uniform)
+  bool have_next = false;
+  double nx_xs = 0.0, nx_ar = 0.0, nx_ai = 0.0;
   // steps k in [kb, ke) with the panel's live columns in a[.][0..W-1]: once
   // at most W-1 columns trail column k, the loop body shrinks to width W
   auto segment = [&](auto Wc, int kb, int ke) {
@@ -340,7 +350,12 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
 #pragma unroll 1
   for (int k = kb; k < ke; ++k) {
     // zlarfg on column k (= a[.][0]): ||x||^2 below row k and alpha = a(k, k)
-    {
+    double xs, ar, ai;
+    if (have_next) {
+      xs = nx_xs;
+      ar = nx_ar;
+      ai = nx_ai;
+    } else {
       double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < R; ++i) {
@@ -352,11 +367,14 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         v8[2] += (rr == k) ? x.y : 0.0;
       }
       red[ph][wave * GMAX * 64 + lane] = wave_rs8(v8);
+      __syncthreads();
+      const double s0 = wave_sum_slots(red[ph], 0);
+      ph ^= 1;
+      xs = bcast(s0, 0);
+      ar = bcast(s0, 8);
+      ai = bcast(s0, 16);
     }
-    __syncthreads();
-    const double s0 = wave_sum_slots(red[ph], 0);
-    ph ^= 1;
-    const double xs = bcast(s0, 0), ar = bcast(s0, 8), ai = bcast(s0, 16);
+    have_next = false;
     const bool ident = (thr > 0.0) ? (sqrt(xs) <= thr && fabs(ai) <= thr) : (xs == 0.0 && ai == 0.0);
     if (ident) {
       if (tid == 0) s_tau[k] = make_double2(0.0, 0.0);
@@ -411,8 +429,8 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
 #pragma unroll
             for (int i = 0; i < R; ++i) {
               const double2 x = a[i][j];
-              sr += vm[i].x * x.x + vm[i].y * x.y;
-              si += vm[i].x * x.y - vm[i].y * x.x;
+              sr = fma(vm[i].x, x.x, fma(vm[i].y, x.y, sr));
+              si = fma(vm[i].x, x.y, fma(-vm[i].y, x.x, si));
             }
           }
           v8[2 * p] = sr;
@@ -420,10 +438,32 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         }
         red[ph][(wave * GMAX + g) * 64 + lane] = wave_rs8(v8);
       }
+      // look-ahead sums for column k+1 (a[.][1] before its update), rows > k+1:
+      // ||a||^2, v_k^H a, ||v_k||^2, and a(k+1), v_k(k+1) from their owner
+      {
+        double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int rr = tid + NT * i;
+          const double2 x = a[i][1], v = vm[i];
+          const bool below = (rr > k + 1 && rr < M), at = (rr == k + 1);
+          const double2 xb = below ? x : make_double2(0.0, 0.0);
+          v8[0] = fma(xb.x, xb.x, fma(xb.y, xb.y, v8[0]));
+          v8[1] = fma(v.x, xb.x, fma(v.y, xb.y, v8[1]));
+          v8[2] = fma(v.x, xb.y, fma(-v.y, xb.x, v8[2]));
+          v8[3] += below ? fma(v.x, v.x, v.y * v.y) : 0.0;
+          v8[4] += at ? x.x : 0.0;
+          v8[5] += at ? x.y : 0.0;
+          v8[6] += at ? v.x : 0.0;
+          v8[7] += at ? v.y : 0.0;
+        }
+        red[ph][(wave * GMAX + GW) * 64 + lane] = wave_rs8(v8);
+      }
       __syncthreads();
       double sg[GW];
 #pragma unroll
       for (int g = 0; g < GW; ++g) sg[g] = wave_sum_slots(red[ph], g);
+      const double la = wave_sum_slots(red[ph], GW);
       ph ^= 1;
       // a_j <- a_j - conj(tau) v_k s_j, written one column to the left
 #pragma unroll
@@ -431,11 +471,27 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         const int f = 2 * (j - 1), g = f / 8, q = f % 8;
         const double sr = bcast(sg[g], 8 * q), si = bcast(sg[g], 8 * q + 8);
         const double fr = t.x * sr + t.y * si, fi = t.x * si - t.y * sr;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          a[i][j - 1].x = a[i][j].x - (vm[i].x * fr - vm[i].y * fi);
-          a[i][j - 1].y = a[i][j].y - (vm[i].x * fi + vm[i].y * fr);
+        if (j == 1 && k + 1 < nbp) {
+          // column k+1 after the update: alpha' = a(k+1) - c v_k(k+1) (the
+          // owner's update, same operations) and ||x'||^2 =
+          // ||a||^2 - 2 Re(conj(c) v_k^H a) + |c|^2 ||v_k||^2, c = conj(tau) s;
+          // used unless cancellation could cost more than ~1e-10 relative
+          const double u1 = bcast(la, 0), u2r = bcast(la, 8), u2i = bcast(la, 16), u3 = bcast(la, 24);
+          const double2 ap = make_double2(bcast(la, 32), bcast(la, 40));
+          const double2 v1 = make_double2(bcast(la, 48), bcast(la, 56));
+          const double2 al = cupd(ap, v1, fr, fi);
+          const double c2 = fr * fr + fi * fi;
+          const double xn = u1 - 2.0 * (fr * u2r + fi * u2i) + c2 * u3;
+          const double scale = u1 + 2.0 * sqrt(c2 * u1 * u3) + c2 * u3;
+          if (xn >= 1e-6 * scale) {
+            have_next = true;
+            nx_xs = xn;
+            nx_ar = al.x;
+            nx_ai = al.y;
+          }
         }
+#pragma unroll
+        for (int i = 0; i < R; ++i) a[i][j - 1] = cupd(a[i][j], vm[i], fr, fi);
       }
 #pragma unroll
       for (int i = 0; i < R; ++i) a[i][W - 1] = make_double2(0.0, 0.0);
@@ -1032,14 +1088,19 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
   // <panel rows per thread, panel width, panel threads, WY rows per lane
   //  (Q formation for n > 256), apply rows per thread, apply threads>; each
   // variant needs m within rows per thread * threads and 64 * WY rows
+  // one wave per SIMD (256 threads) where the panel fits: the column loop is
+  // VALU-bound, and a SIMD with two waves carries twice the work; 8-column
+  // panels keep the rows of a thread in registers up to m = 1280 (measured:
+  // m = 1152 panel 3.6 us/column vs 5.7 with 16 columns on 6 waves; at m = 256
+  // the 16-column panel wins, at m = 2304 two waves per SIMD beat AGPR storage)
   if (m <= 256)
     qr_blocked<1, 16, 256, 4, 1, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
   else if (m <= 512)
-    qr_blocked<2, 16, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<2, 8, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
   else if (m <= 768)
-    qr_blocked<3, 16, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
-  else if (m <= 1152)
-    qr_blocked<3, 16, 384, 18, 3, 384>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<3, 8, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+  else if (m <= 1280)
+    qr_blocked<5, 8, 256, 20, 5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
   else if (m <= 2560)
     qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
   else
