@@ -367,12 +367,13 @@ static GemmPlan make_plan(const Gemm& g, int num_cus) {
   GemmPlan pl;
   pl.tiles_m = (int)((g.M + BM - 1) / BM);
   pl.tiles_n = (int)((g.N + BN - 1) / BN);
-  // Split K while the tiles fill fewer than the 2 x num_cus workgroup slots
-  // (two workgroups fit per CU).
+  // Split K while the tiles leave CUs idle, up to the 2 x num_cus workgroup
+  // slots (two workgroups fit per CU); a grid with a tile for every CU is
+  // not split (the partials' extra pass costs more than the second slot gains).
   const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch;
   const int64_t target = 2LL * num_cus;
   int splits = 1;
-  if (blocks < target && g.K > 4 * BK) {
+  if (blocks < num_cus && g.K > 4 * BK) {
     splits = (int)((target + blocks - 1) / blocks);
     const int64_t max_splits = (g.K + 4 * BK - 1) / (4 * BK);   // >= 64-deep chunks
     if (splits > max_splits) splits = (int)max_splits;
