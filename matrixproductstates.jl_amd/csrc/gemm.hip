@@ -383,7 +383,7 @@ static GemmPlan make_plan(const Gemm& g, int num_cus) {
   int splits = 1;
   if (blocks < num_cus && g.K > 4 * BK) {
     splits = (int)((target + blocks - 1) / blocks);
-    const int64_t max_splits = (g.K + 4 * BK - 1) / (4 * BK);   // >= 64-deep chunks
+    const int64_t max_splits = (g.K + 2 * BK - 1) / (2 * BK);   // >= 32-deep chunks
     if (splits > max_splits) splits = (int)max_splits;
     if (splits > 32) splits = 32;
     if (splits < 1) splits = 1;

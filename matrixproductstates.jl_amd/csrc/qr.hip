@@ -1053,12 +1053,12 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
   L.g = o; o = al256(o + (size_t)n * n * sizeof(double2));
   L.w = o; o = al256(o + (size_t)n * n * sizeof(double2));
   // split-K partials: a bound that grows with (m, n) (at most 32 splits of
-  // the n x n Gram, ceil(n/64) of the m x n product), so a workspace sized
-  // for the largest panel of a state serves every smaller one
+  // the n x n Gram, ceil(n/32) of the m x n product: chunks are >= 32 deep),
+  // so a workspace sized for the largest panel of a state serves every smaller one
   const QrGemms g = qr_gemms(nullptr, m, n, 1, m, nullptr, nullptr, nullptr);
   const size_t need =
       std::max(gemm_workspace_bytes(g.gram, CMP_F64, num_cus), gemm_workspace_bytes(g.q, CMP_F64, num_cus));
-  const size_t bound = std::max((size_t)32 * n * n, (size_t)((n + 63) / 64) * m * n) * sizeof(double2);
+  const size_t bound = std::max((size_t)32 * n * n, (size_t)((n + 31) / 32) * m * n) * sizeof(double2);
   L.gws_bytes = std::max(need, bound);
   L.gws = o; o = al256(o + L.gws_bytes);
   L.total = o;
