@@ -279,32 +279,42 @@ __device__ __forceinline__ double wave_rs8(const double (&v)[8]) {
   return w;
 }
 
+// 1/x from the hardware estimate and two Newton steps (x finite, nonzero)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // One workgroup factors panel columns c0 .. c0+nbp-1 (nbp <= NB), rows
 // c0..m-1.  Thread t holds rows t, t+NT, ..., t+(R-1)NT of the panel in
-// registers, so every dot product of a step is a local partial sum; only the
-// 3 zlarfg scalars and the dot products v_k^H a_j cross lanes (butterfly in
+// registers (rows past the panel are zero and stay zero through every step,
+// so only the first row slot, the one that can meet the diagonal, is
+// masked), so every dot product of a step is a local partial sum; only the
+// zlarfg scalars and the dot products v_k^H a_j cross lanes (butterfly in
 // the wave, LDS across waves).  The column loop is not unrolled (the
 // unrolled body overflows the instruction cache): the panel shifts left by
 // one column per step, so the active column always sits in a[.][0].  Each
-// finished column goes straight to A (packed R / reflector) and Vs
-// (explicit reflector).  Then the Gram entries G[j][l] = v_j^H v_l (j < l)
-// are formed from Vs for the compact-WY factor T (H(0)...H(nbp-1) =
-// I - V T V^H, zlarft).
-template <int R, int NB, int NT>
+// finished column goes to Vs as the explicit reflector and, on rows <= k,
+// to A as a column of R (the reflector tail below the diagonal is not
+// stored in A: Q is formed from Vs and overwrites the panel).
+// PT: also form the compact-WY factor T (H(0)...H(nbp-1) = I - V T V^H,
+// zlarft) from the Gram entries G[j][l] = v_j^H v_l; otherwise the apply
+// kernel works from the Gram entries itself and T is formed after the
+// factorization from the Gram GEMM.
+template <int R, int NB, int NT, bool PT>
 __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, int m, int n, int64_t rs,
                                                       int64_t cs, int c0, int nbp, double tol,
                                                       const double* __restrict__ colnrm,
                                                       double2* __restrict__ Vs, double2* __restrict__ tau,
                                                       double2* __restrict__ Tp) {
+  static_assert(NT > NB, "the first row slot must cover the panel's diagonal");
   constexpr int NW = NT / 64;
   constexpr int GMAX = (2 * (NB - 1) + 7) / 8 + 1;        // 8-value groups per column step (+1 look-ahead)
   static_assert(GMAX <= 8, "panel too wide");
   __shared__ double red[2][NW * GMAX * 64];
-  constexpr int NP = NB * (NB - 1) / 2;                   // Gram pairs
-  constexpr int NPB = (NP + 31) / 32;                     // blocks of 8 groups of 4 pairs
-  __shared__ double gram[NW * NPB * 64];
-  __shared__ double2 s_G[NB][NB];
-  __shared__ double2 s_T[NB][NB + 1];
   __shared__ double2 s_tau[NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int M = m - c0;
@@ -314,6 +324,7 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     for (int c = 0; c < n; ++c) t += colnrm[c];
     thr = tol * sqrt(t);
   }
+  const double thr2 = thr * thr;
   double2 a[R][NB];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -325,6 +336,8 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       a[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
     }
   }
+  double2* const Ak = A + (int64_t)(c0 + tid) * rs + (int64_t)c0 * cs;   // row c0 + tid (slot 0), column c0
+  double2* const Vk = Vs + (int64_t)c0 * m + c0 + tid;                    // column c0, row c0 + tid
   // sum over waves of the butterfly results of group g; lane l gets value l >> 3
   auto wave_sum_slots = [&](const double* r, int g) {
     double s = 0.0;
@@ -349,6 +362,7 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     constexpr int GW = (2 * (W - 1) + 7) / 8;
 #pragma unroll 1
   for (int k = kb; k < ke; ++k) {
+    if (k >= NB) __builtin_unreachable();
     // zlarfg on column k (= a[.][0]): ||x||^2 below row k and alpha = a(k, k)
     double xs, ar, ai;
     if (have_next) {
@@ -359,13 +373,12 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        const int rr = tid + NT * i;
         const double2 x = a[i][0];
-        const double n2 = x.x * x.x + x.y * x.y;
-        v8[0] += (rr > k && rr < M) ? n2 : 0.0;
-        v8[1] += (rr == k) ? x.x : 0.0;
-        v8[2] += (rr == k) ? x.y : 0.0;
+        const double n2 = fma(x.x, x.x, x.y * x.y);
+        v8[0] += (i > 0 || tid > k) ? n2 : 0.0;
       }
+      v8[1] = (tid == k) ? a[0][0].x : 0.0;
+      v8[2] = (tid == k) ? a[0][0].y : 0.0;
       red[ph][wave * GMAX * 64 + lane] = wave_rs8(v8);
       __syncthreads();
       const double s0 = wave_sum_slots(red[ph], 0);
@@ -375,17 +388,18 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       ai = bcast(s0, 16);
     }
     have_next = false;
-    const bool ident = (thr > 0.0) ? (sqrt(xs) <= thr && fabs(ai) <= thr) : (xs == 0.0 && ai == 0.0);
+    const bool ident = (thr > 0.0) ? (xs <= thr2 && fabs(ai) <= thr) : (xs == 0.0 && ai == 0.0);
     if (ident) {
-      if (tid == 0) s_tau[k] = make_double2(0.0, 0.0);
-      // column k out unchanged (packed to A; Vs: 1 at row k, zero elsewhere)
+      if (tid == 0) {
+        s_tau[k] = make_double2(0.0, 0.0);
+        tau[c0 + k] = make_double2(0.0, 0.0);
+      }
+      // column k out unchanged (R part to A; Vs: 1 at row k, zero elsewhere)
+      if (tid <= k) Ak[(int64_t)k * cs] = a[0][0];
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         const int rr = tid + NT * i;
-        if (rr < M) {
-          A[(int64_t)(c0 + rr) * rs + (int64_t)(c0 + k) * cs] = a[i][0];
-          Vs[(int64_t)(c0 + k) * m + c0 + rr] = make_double2(rr == k ? 1.0 : 0.0, 0.0);
-        }
+        if (rr < M) Vk[(int64_t)k * m + NT * i] = make_double2(rr == k ? 1.0 : 0.0, 0.0);
       }
       // shift the panel: column k+1 becomes a[.][0]
 #pragma unroll
@@ -395,27 +409,32 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         a[i][W - 1] = make_double2(0.0, 0.0);
       }
     } else {
-      const double beta = -copysign(sqrt(ar * ar + ai * ai + xs), ar);
-      const double2 t = make_double2((beta - ar) / beta, -ai / beta);
-      const double dr = ar - beta, di = ai, den = dr * dr + di * di;
-      const double2 sc = make_double2(dr / den, -di / den);
-      if (tid == 0) s_tau[k] = t;
-      // beta on the diagonal, the reflector tail below; vm = v_k on rows
-      // k..M-1 and 0 elsewhere (masks the dots and the update).  Column k
-      // goes out packed to A and as the explicit reflector to Vs.
+      // beta = -sign(Re alpha) ||(alpha, x)||, tau = (beta - alpha) / beta,
+      // v = x / (alpha - beta); alpha - beta != 0 here
+      const double beta = -copysign(sqrt(fma(ar, ar, fma(ai, ai, xs))), ar);
+      const double ib = rcp_nr(beta);
+      const double dr = ar - beta;
+      const double iden = rcp_nr(fma(dr, dr, ai * ai));
+      const double2 t = make_double2((beta - ar) * ib, -ai * ib);
+      const double2 sc = make_double2(dr * iden, -ai * iden);
+      if (tid == 0) {
+        s_tau[k] = t;
+        tau[c0 + k] = t;
+      }
+      // vm = v_k on rows k..M-1 and 0 elsewhere (masks the dots and the update)
       double2 vm[R];
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        const int rr = tid + NT * i;
-        const double2 x = a[i][0];
-        const double2 y = cmul(x, sc);
-        vm[i] = (rr == k) ? make_double2(1.0, 0.0) : ((rr > k && rr < M) ? y : make_double2(0.0, 0.0));
-        if (rr < M) {
-          A[(int64_t)(c0 + rr) * rs + (int64_t)(c0 + k) * cs] =
-              (rr == k) ? make_double2(beta, 0.0) : ((rr > k) ? y : x);
-          Vs[(int64_t)(c0 + k) * m + c0 + rr] = vm[i];
-        }
+        const double2 y = cmul(a[i][0], sc);
+        if (i == 0)
+          vm[0] = (tid == k) ? make_double2(1.0, 0.0) : ((tid > k) ? y : make_double2(0.0, 0.0));
+        else
+          vm[i] = y;
       }
+      if (tid <= k) Ak[(int64_t)k * cs] = (tid == k) ? make_double2(beta, 0.0) : a[0][0];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        if (tid + NT * i < M) Vk[(int64_t)k * m + NT * i] = vm[i];
       // s_j = v_k^H a_j for the W-1 columns to the right (zero past the
       // panel), four columns per butterfly
 #pragma unroll
@@ -444,19 +463,19 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          const int rr = tid + NT * i;
           const double2 x = a[i][1], v = vm[i];
-          const bool below = (rr > k + 1 && rr < M), at = (rr == k + 1);
-          const double2 xb = below ? x : make_double2(0.0, 0.0);
+          const double2 xb = (i > 0 || tid > k + 1) ? x : make_double2(0.0, 0.0);
+          const double2 vb = (i > 0 || tid > k + 1) ? v : make_double2(0.0, 0.0);
           v8[0] = fma(xb.x, xb.x, fma(xb.y, xb.y, v8[0]));
-          v8[1] = fma(v.x, xb.x, fma(v.y, xb.y, v8[1]));
-          v8[2] = fma(v.x, xb.y, fma(-v.y, xb.x, v8[2]));
-          v8[3] += below ? fma(v.x, v.x, v.y * v.y) : 0.0;
-          v8[4] += at ? x.x : 0.0;
-          v8[5] += at ? x.y : 0.0;
-          v8[6] += at ? v.x : 0.0;
-          v8[7] += at ? v.y : 0.0;
+          v8[1] = fma(vb.x, xb.x, fma(vb.y, xb.y, v8[1]));
+          v8[2] = fma(vb.x, xb.y, fma(-vb.y, xb.x, v8[2]));
+          v8[3] = fma(vb.x, vb.x, fma(vb.y, vb.y, v8[3]));
         }
+        const bool at = (tid == k + 1);
+        v8[4] = at ? a[0][1].x : 0.0;
+        v8[5] = at ? a[0][1].y : 0.0;
+        v8[6] = at ? vm[0].x : 0.0;
+        v8[7] = at ? vm[0].y : 0.0;
         red[ph][(wave * GMAX + GW) * 64 + lane] = wave_rs8(v8);
       }
       __syncthreads();
@@ -475,15 +494,16 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
           // column k+1 after the update: alpha' = a(k+1) - c v_k(k+1) (the
           // owner's update, same operations) and ||x'||^2 =
           // ||a||^2 - 2 Re(conj(c) v_k^H a) + |c|^2 ||v_k||^2, c = conj(tau) s;
-          // used unless cancellation could cost more than ~1e-10 relative
+          // used where the cancellation costs at most ~1e3 ulps
+          // (scale >= the sum of the terms' magnitudes)
           const double u1 = bcast(la, 0), u2r = bcast(la, 8), u2i = bcast(la, 16), u3 = bcast(la, 24);
           const double2 ap = make_double2(bcast(la, 32), bcast(la, 40));
           const double2 v1 = make_double2(bcast(la, 48), bcast(la, 56));
           const double2 al = cupd(ap, v1, fr, fi);
           const double c2 = fr * fr + fi * fi;
           const double xn = u1 - 2.0 * (fr * u2r + fi * u2i) + c2 * u3;
-          const double scale = u1 + 2.0 * sqrt(c2 * u1 * u3) + c2 * u3;
-          if (xn >= 1e-6 * scale) {
+          const double scale = 2.0 * (u1 + c2 * u3);
+          if (xn >= 1e-3 * scale) {
             have_next = true;
             nx_xs = xn;
             nx_ar = al.x;
@@ -508,97 +528,145 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     segment(std::integral_constant<int, 8>{}, 0, min(nbp, 4));
     segment(std::integral_constant<int, 4>{}, 4, nbp);
   }
-  __syncthreads();
-  // Gram entries G[j][l] = v_j^H v_l (j < l) from this thread's own rows of
-  // Vs (explicit zeros above each reflector's diagonal, so the sums run over
-  // rows >= l by themselves): 4 pairs per butterfly, 8 butterflies per
-  // stored block, lane t keeping group (t & 7)'s value (t >> 3)
-  double2 b[R][NB];
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    const int rr = tid + NT * i;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const double2 x = Vs[(int64_t)(c0 + min(j, nbp - 1)) * m + c0 + min(rr, M - 1)];
-      b[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
-    }
-  }
-  static_for<0, NPB>([&](auto bc) {
-    constexpr int blk = decltype(bc)::value;
-    double keep = 0.0;
-    static_for<0, 8>([&](auto gc) {
-      constexpr int gg = decltype(gc)::value;
-      constexpr int g = 8 * blk + gg;
-      if constexpr (4 * g < NP) {
-        double v8[8];
-        static_for<0, 4>([&](auto pc) {
-          constexpr int pq = decltype(pc)::value;
-          constexpr int p = 4 * g + pq;
-          double gr = 0.0, gi = 0.0;
-          if constexpr (p < NP) {
-            constexpr int l = pair_l(p), j = pair_j(p);
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-              gr += b[i][j].x * b[i][l].x + b[i][j].y * b[i][l].y;
-              gi += b[i][j].x * b[i][l].y - b[i][j].y * b[i][l].x;
-            }
-          }
-          v8[2 * pq] = gr;
-          v8[2 * pq + 1] = gi;
-        });
-        const double t = wave_rs8(v8);
-        keep = ((lane & 7) == gg) ? t : keep;
-      }
-    });
-    gram[(wave * NPB + blk) * 64 + lane] = keep;
-  });
-  __syncthreads();
-  for (int e = tid; e < NB * NB; e += NT) {
-    const int j = e % NB, l = e / NB;
-    double2 gv = make_double2(0.0, 0.0);
-    if (j < l && l < nbp) {
-      // pair p = l(l-1)/2 + j: group p/4 (block g/8, lane slot g%8), values 2(p%4), 2(p%4)+1
-      const int p = l * (l - 1) / 2 + j, g = p / 4, f = 2 * (p % 4);
-#pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        gv.x += gram[(w * NPB + g / 8) * 64 + 8 * f + (g % 8)];
-        gv.y += gram[(w * NPB + g / 8) * 64 + 8 * (f + 1) + (g % 8)];
-      }
-    }
-    s_G[j][l] = gv;
-  }
-  __syncthreads();
-  // T: T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k]; lane j
-  // keeps row j of T in registers, G columns are broadcast reads
-  if (tid < 64) {
-    double2 trow[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) trow[c] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-      if (k < nbp) {
-        const double2 tk = s_tau[k];
-        double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int l = 0; l < k; ++l) {
-          const double2 p = cmul(trow[l], s_G[l][k]);   // trow[l] = 0 for l < lane
-          acc.x += p.x;
-          acc.y += p.y;
-        }
-        const double2 v = cmul(tk, acc);
-        trow[k] = (lane < k) ? make_double2(-v.x, -v.y) : ((lane == k) ? tk : make_double2(0.0, 0.0));
-      }
-    }
-    if (lane < NB) {
-#pragma unroll
-      for (int c = 0; c < NB; ++c) s_T[lane][c] = (lane < nbp && c < nbp) ? trow[c] : make_double2(0.0, 0.0);
-    }
-  }
+  // rows above the panel of its reflectors are zero (the GEMMs of the Q
+  // formation run over all m rows)
   for (int r = tid; r < c0; r += NT)
     for (int j = 0; j < nbp; ++j) Vs[(int64_t)(c0 + j) * m + r] = make_double2(0.0, 0.0);
-  if (tid < nbp) tau[c0 + tid] = s_tau[tid];
+  if constexpr (PT) {
+    constexpr int NP = NB * (NB - 1) / 2;                 // Gram pairs
+    constexpr int NPB = (NP + 31) / 32;                   // blocks of 8 groups of 4 pairs
+    __shared__ double gram[NW * NPB * 64];
+    __shared__ double2 s_G[NB][NB];
+    __shared__ double2 s_T[NB][NB + 1];
+    __syncthreads();
+    // Gram entries G[j][l] = v_j^H v_l (j < l) from this thread's own rows of
+    // Vs (explicit zeros above each reflector's diagonal, so the sums run over
+    // rows >= l by themselves): 4 pairs per butterfly, 8 butterflies per
+    // stored block, lane t keeping group (t & 7)'s value (t >> 3)
+    double2 b[R][NB];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int rr = tid + NT * i;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const double2 x = Vs[(int64_t)(c0 + min(j, nbp - 1)) * m + c0 + min(rr, M - 1)];
+        b[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
+      }
+    }
+    static_for<0, NPB>([&](auto bc) {
+      constexpr int blk = decltype(bc)::value;
+      double keep = 0.0;
+      static_for<0, 8>([&](auto gc) {
+        constexpr int gg = decltype(gc)::value;
+        constexpr int g = 8 * blk + gg;
+        if constexpr (4 * g < NP) {
+          double v8[8];
+          static_for<0, 4>([&](auto pc) {
+            constexpr int pq = decltype(pc)::value;
+            constexpr int p = 4 * g + pq;
+            double gr = 0.0, gi = 0.0;
+            if constexpr (p < NP) {
+              constexpr int l = pair_l(p), j = pair_j(p);
+#pragma unroll
+              for (int i = 0; i < R; ++i) {
+                gr += b[i][j].x * b[i][l].x + b[i][j].y * b[i][l].y;
+                gi += b[i][j].x * b[i][l].y - b[i][j].y * b[i][l].x;
+              }
+            }
+            v8[2 * pq] = gr;
+            v8[2 * pq + 1] = gi;
+          });
+          const double t = wave_rs8(v8);
+          keep = ((lane & 7) == gg) ? t : keep;
+        }
+      });
+      gram[(wave * NPB + blk) * 64 + lane] = keep;
+    });
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += NT) {
+      const int j = e % NB, l = e / NB;
+      double2 gv = make_double2(0.0, 0.0);
+      if (j < l && l < nbp) {
+        // pair p = l(l-1)/2 + j: group p/4 (block g/8, lane slot g%8), values 2(p%4), 2(p%4)+1
+        const int p = l * (l - 1) / 2 + j, g = p / 4, f = 2 * (p % 4);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          gv.x += gram[(w * NPB + g / 8) * 64 + 8 * f + (g % 8)];
+          gv.y += gram[(w * NPB + g / 8) * 64 + 8 * (f + 1) + (g % 8)];
+        }
+      }
+      s_G[j][l] = gv;
+    }
+    __syncthreads();
+    // T: T[k][k] = tau_k, T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k]; lane j
+    // keeps row j of T in registers, G columns are broadcast reads
+    if (tid < 64) {
+      double2 trow[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) trow[c] = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        if (k < nbp) {
+          const double2 tk = s_tau[k];
+          double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int l = 0; l < k; ++l) {
+            const double2 p = cmul(trow[l], s_G[l][k]);   // trow[l] = 0 for l < lane
+            acc.x += p.x;
+            acc.y += p.y;
+          }
+          const double2 v = cmul(tk, acc);
+          trow[k] = (lane < k) ? make_double2(-v.x, -v.y) : ((lane == k) ? tk : make_double2(0.0, 0.0));
+        }
+      }
+      if (lane < NB) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) s_T[lane][c] = (lane < nbp && c < nbp) ? trow[c] : make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += NT) Tp[e] = s_T[e % NB][e / NB];
+  }
+}
+
+// T factors of all panels from the Gram matrix G = V^H V (n x n, column
+// major; its diagonal NB x NB blocks are the panels' Gram entries): one
+// wave per panel, lane j keeps row j of T (recurrence as zlarft, forward
+// columnwise).  Tall[p] is NB x NB column major.
+template <int NB>
+__global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restrict__ G, int n,
+                                                        const double2* __restrict__ tau,
+                                                        double2* __restrict__ Tall) {
+  __shared__ double2 sG[NB][NB + 1];
+  const int p = blockIdx.x, c0 = p * NB, nbp = min(NB, n - c0), lane = threadIdx.x;
+  for (int e = lane; e < NB * NB; e += 64) {
+    const int j = e % NB, l = e / NB;
+    sG[j][l] = (j < nbp && l < nbp) ? G[(int64_t)(c0 + l) * n + c0 + j] : make_double2(0.0, 0.0);
+  }
   __syncthreads();
-  for (int e = tid; e < NB * NB; e += NT) Tp[e] = s_T[e % NB][e / NB];
+  double2 trow[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) trow[c] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (k < nbp) {
+      const double2 tk = tau[c0 + k];
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int l = 0; l < k; ++l) {
+        const double2 q = cmul(trow[l], sG[l][k]);
+        acc.x += q.x;
+        acc.y += q.y;
+      }
+      const double2 v = cmul(tk, acc);
+      trow[k] = (lane < k) ? make_double2(-v.x, -v.y) : ((lane == k) ? tk : make_double2(0.0, 0.0));
+    }
+  }
+  if (lane < NB) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      Tall[(int64_t)p * NB * NB + lane + c * NB] = (lane < nbp && c < nbp) ? trow[c] : make_double2(0.0, 0.0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -759,22 +827,34 @@ __global__ __launch_bounds__(WY_NT) void qr_apply_kernel(double2* __restrict__ A
 }
 
 // geqrf trailing update, one workgroup per trailing column j:
-//   a_j <- (I - V T V^H)^H a_j = a_j - V (T^H (V^H a_j))   on rows c0..m-1.
+//   a_j <- (I - V T V^H)^H a_j = a_j - V u,  u = T^H (V^H a_j)   on rows c0..m-1.
 // Thread t holds rows t, t+NT, ... (RA of them) of a_j and of the panel's NB
 // reflectors in registers for both passes; V^H a_j is one butterfly
 // reduction per wave plus an LDS sum over waves.
-template <int RA, int NB, int NT>
+// OWN_G: T is not read; the same reduction also yields the Gram entries
+// G[l][q] = v_l^H v_q (l < q), and since T^{-1} = diag(1/tau) + striu(G),
+// u solves the lower-triangular system
+//   u_q = conj(tau_q) (w_q - sum_{l<q} conj(G[l][q]) u_l)
+// (lane q of the first wave holds row q; u_l is broadcast as it completes).
+template <int RA, int NB, int NT, bool OWN_G>
 __global__ __launch_bounds__(NT) void qr_apply_col_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
                                                           int c0, int nbp, const double2* __restrict__ Vs,
-                                                          const double2* __restrict__ Tp) {
+                                                          const double2* __restrict__ Tp,
+                                                          const double2* __restrict__ tau) {
   constexpr int NW = NT / 64;
-  constexpr int NG = (2 * NB + 7) / 8;
+  constexpr int NPAIR = OWN_G ? NB * (NB - 1) / 2 : 0;
+  constexpr int NVAL = 2 * NB + 2 * NPAIR;               // w (NB complex), then the Gram pairs
+  constexpr int NG = (NVAL + 7) / 8;
+  static_assert(NVAL <= NT, "one thread per reduced value");
   __shared__ double red[NW * NG * 64];
-  __shared__ double2 s_w[NB], s_u[NB];
+  __shared__ double s_val[NG * 8];
+  __shared__ double2 s_u[NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int j = c0 + nbp + blockIdx.x;
   const int M = m - c0;
   const double2* V = Vs + (int64_t)c0 * m + c0;   // reflector q (panel-local), row c0 + r: V[r + q m]
+  double2 ct = make_double2(0.0, 0.0);
+  if (OWN_G && tid < nbp) ct = tau[c0 + tid];
   double2 x[RA], v[RA][NB];
 #pragma unroll
   for (int i = 0; i < RA; ++i) {
@@ -789,48 +869,82 @@ __global__ __launch_bounds__(NT) void qr_apply_col_kernel(double2* __restrict__ 
       v[i][q] = (ok && q < nbp) ? va : make_double2(0.0, 0.0);
     }
   }
-  // w = V^H x
-#pragma unroll
-  for (int g = 0; g < NG; ++g) {
+  // value e: e < 2 NB: Re/Im of w_{e/2} = v_{e/2}^H x; else Re/Im of pair
+  // p = (e - 2 NB) / 2, G[pair_j(p)][pair_l(p)]
+  static_for<0, NG>([&](auto gc) {
+    constexpr int g = decltype(gc)::value;
     double v8[8];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int q = 4 * g + p;
-      double wr = 0.0, wi = 0.0;
-      if (q < NB) {
+    static_for<0, 4>([&](auto hc) {
+      constexpr int h = decltype(hc)::value;
+      constexpr int e = 8 * g + 2 * h;
+      double sr = 0.0, si = 0.0;
+      if constexpr (e < 2 * NB) {
+        constexpr int q = e / 2;
 #pragma unroll
         for (int i = 0; i < RA; ++i) {
-          wr += v[i][q].x * x[i].x + v[i][q].y * x[i].y;
-          wi += v[i][q].x * x[i].y - v[i][q].y * x[i].x;
+          sr += v[i][q].x * x[i].x + v[i][q].y * x[i].y;
+          si += v[i][q].x * x[i].y - v[i][q].y * x[i].x;
+        }
+      } else if constexpr (e < NVAL) {
+        constexpr int p = (e - 2 * NB) / 2;
+        constexpr int l = pair_l(p), jj = pair_j(p);
+#pragma unroll
+        for (int i = 0; i < RA; ++i) {
+          sr += v[i][jj].x * v[i][l].x + v[i][jj].y * v[i][l].y;
+          si += v[i][jj].x * v[i][l].y - v[i][jj].y * v[i][l].x;
         }
       }
-      v8[2 * p] = wr;
-      v8[2 * p + 1] = wi;
-    }
+      v8[2 * h] = sr;
+      v8[2 * h + 1] = si;
+    });
     red[(wave * NG + g) * 64 + lane] = wave_rs8(v8);
+  });
+  __syncthreads();
+  if (tid < NVAL) {
+    const int g = tid / 8, f = tid % 8;
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[(w * NG + g) * 64 + 8 * f];
+    s_val[tid] = s;
   }
   __syncthreads();
-  if (tid < NB) {
-    const int g = tid / 4, f = 2 * (tid % 4);
-    double wr = 0.0, wi = 0.0;
+  if constexpr (OWN_G) {
+    if (tid < 64) {
+      // lane q: acc = w_q, then the substitution over l = 0 .. NB-1
+      const int q = min(lane, NB - 1);
+      double2 acc = make_double2(s_val[2 * q], s_val[2 * q + 1]);
+      double2 u = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      wr += red[(w * NG + g) * 64 + 8 * f];
-      wi += red[(w * NG + g) * 64 + 8 * (f + 1)];
+      for (int l = 0; l < NB; ++l) {
+        if (lane == l) u = make_double2(ct.x * acc.x + ct.y * acc.y, ct.x * acc.y - ct.y * acc.x);
+        const int lo_r = __builtin_amdgcn_readlane(__double2loint(u.x), l);
+        const int hi_r = __builtin_amdgcn_readlane(__double2hiint(u.x), l);
+        const int lo_i = __builtin_amdgcn_readlane(__double2loint(u.y), l);
+        const int hi_i = __builtin_amdgcn_readlane(__double2hiint(u.y), l);
+        const double2 ul = make_double2(__hiloint2double(hi_r, lo_r), __hiloint2double(hi_i, lo_i));
+        if (l + 1 < NB && lane > l && lane < NB) {
+          // conj(G[l][q]), pair index p = q (q - 1) / 2 + l
+          const int p = q * (q - 1) / 2 + l;
+          const double gr = s_val[2 * NB + 2 * p], gi = s_val[2 * NB + 2 * p + 1];
+          acc.x -= gr * ul.x + gi * ul.y;
+          acc.y -= gr * ul.y - gi * ul.x;
+        }
+      }
+      if (lane < NB) s_u[lane] = u;
     }
-    s_w[tid] = make_double2(wr, wi);
-  }
-  __syncthreads();
-  // u = T^H w
-  if (tid < NB) {
-    double ur = 0.0, ui = 0.0;
+  } else {
+    // u = T^H w
+    if (tid < NB) {
+      double ur = 0.0, ui = 0.0;
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      const double2 t = Tp[i + tid * NB], w = s_w[i];
-      ur += t.x * w.x + t.y * w.y;
-      ui += t.x * w.y - t.y * w.x;
+      for (int i = 0; i < NB; ++i) {
+        const double2 t = Tp[i + tid * NB];
+        const double wr = s_val[2 * i], wi = s_val[2 * i + 1];
+        ur += t.x * wr + t.y * wi;
+        ui += t.x * wi - t.y * wr;
+      }
+      s_u[tid] = make_double2(ur, ui);
     }
-    s_u[tid] = make_double2(ur, ui);
   }
   __syncthreads();
   double2 u[NB];
@@ -1009,25 +1123,33 @@ struct QrWork {
   int num_cus;
 };
 
+// PT: the panel kernel forms each panel's T (16-column panels, where the
+// apply kernel's own Gram reduction would cost more than the panel's);
+// otherwise the apply kernel reduces the Gram entries with V^H a_j, and the T
+// factors for the Q formation come from the Gram GEMM's diagonal blocks.
 template <int R, int NB, int NT, int MAXR, int RA, int NTA>
 static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double tol, const QrWork& q,
                        double2* R_out, hipStream_t s) {
+  constexpr bool PT = (NB == 16);
   double2 *Vs = q.Vs, *tau = q.tau, *Tall = q.Tall;
   double* colnrm = q.colnrm;
   if (tol > 0.0) hipLaunchKernelGGL(qr_colnorm_kernel, dim3((unsigned)n), dim3(64), 0, s, A, m, rs, cs, colnrm);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
-    hipLaunchKernelGGL((qr_panel_kernel<R, NB, NT>), dim3(1), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
+    hipLaunchKernelGGL((qr_panel_kernel<R, NB, NT, PT>), dim3(1), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
                        tol, colnrm, Vs, tau, Tp);
     const int trail = n - c0 - nbp;
     if (trail > 0)
-      hipLaunchKernelGGL((qr_apply_col_kernel<RA, NB, NTA>), dim3((unsigned)trail), dim3(NTA), 0, s, A, m, rs, cs,
-                         c0, nbp, Vs, Tp);
+      hipLaunchKernelGGL((qr_apply_col_kernel<RA, NB, NTA, !PT>), dim3((unsigned)trail), dim3(NTA), 0, s, A, m,
+                         rs, cs, c0, nbp, Vs, Tp, tau);
   }
+  const QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
+  if (!PT || n <= FW_NMAX) launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+  if (!PT)
+    hipLaunchKernelGGL((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB)), dim3(64), 0, s, q.G, n, tau,
+                       Tall);
   if (n <= FW_NMAX) {
-    const QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
-    launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
     hipLaunchKernelGGL((qr_formw_kernel<NB>), dim3((unsigned)((n + FW_CW - 1) / FW_CW)), dim3(256), 0, s, A, m,
                        n, rs, cs, Vs, Tall, q.G, q.W, R_out);
     launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
