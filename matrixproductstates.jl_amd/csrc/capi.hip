@@ -1034,7 +1034,11 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     // Asynchronous on the context's stream (no host value is produced): the
     // caller's next call on this context, or a scalar-returning call, orders
     // after it; coefficients are kernel arguments, never host memory.
-    if (!c->graphs || c->profiling) {
+    // Captured graphs serve the launch-bound small solves; a plan large enough
+    // for the side stream runs eagerly (its launches are hidden behind the
+    // kernels, and measured, a second live graph with side-stream branches in
+    // the same context ran at half speed).
+    if (!c->graphs || c->profiling || p.conc) {
       run(p);
       if (c->profiling) c->sync();
       return;

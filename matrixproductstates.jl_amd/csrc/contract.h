@@ -18,6 +18,11 @@ struct Ctx {
   // workspace arena (grow only, outside graph capture)
   char* ws = nullptr;
   size_t ws_bytes = 0;
+  // second stream for GEMM chains independent of the main chain (forked
+  // from and joined back into `stream` within one call; default priority)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void ensure_side();
   double* dscal = nullptr;      // 256 doubles of device scalars
   double* hscal = nullptr;      // pinned host mirror
   bool capturing = false;
@@ -98,7 +103,8 @@ struct Op {
   DType tmp = F64;   // storage of T1/T2 intermediates
 };
 
-void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes);
+// on stream s (nullptr: the context's stream)
+void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s = nullptr);
 
 // --------------------------------------------- L1 contractions (mps_base.jl)
 struct Work {
@@ -106,6 +112,7 @@ struct Work {
   void* splitk = nullptr;
   size_t splitk_bytes = 0;
   void* T1 = nullptr;
+  hipStream_t stream = nullptr;   // nullptr: the context's stream
 };
 
 void prod_AOR(Work& w, const Op& op, void* T2, const void* A, i64 Dl, i64 d, i64 Dr, const void* O,

@@ -30,7 +30,19 @@ void Ctx::prof_end(hipEvent_t a, int kind, double flops, double bytes) {
   prof.push_back(ProfRec{a, b, kind, flops, bytes});
 }
 
+void Ctx::ensure_side() {
+  if (side) return;
+  REQUIRE(!capturing, "side stream creation during graph capture");
+  HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+}
+
 Ctx::~Ctx() {
+  if (side) (void)hipStreamSynchronize(side);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
   if (ws) (void)hipFree(ws);
   if (dscal) (void)hipFree(dscal);
@@ -42,6 +54,7 @@ void Ctx::ensure_ws(size_t bytes) {
   if (bytes <= ws_bytes) return;
   REQUIRE(!capturing, "workspace growth during graph capture");
   HIPCHK(hipStreamSynchronize(stream));
+  if (side) HIPCHK(hipStreamSynchronize(side));
   if (ws) HIPCHK(hipFree(ws));
   ws = nullptr;
   size_t nb = bytes + bytes / 4 + (1 << 20);
@@ -94,11 +107,12 @@ i64 MPO::max_w() const {
 
 static double esize(DType t) { return t == F64 ? 16.0 : 8.0; }
 
-void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes) {
+void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s) {
   if (!c->profiling || c->capturing) {
-    launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, c->stream);
+    launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, s ? s : c->stream);
     return;
   }
+  REQUIRE(s == nullptr || s == c->stream, "profiled GEMMs run on the context's stream");
   hipEvent_t a;
   c->prof_begin(a);
   launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, c->stream);
@@ -129,7 +143,7 @@ void prod_AOR(Work& w, const Op& op, void* T2, const void* A, i64 Dl, i64 d, i64
   g1.A = A; g1.am = im1(1); g1.ak = im1(Dl * d); g1.ta = F64;
   g1.B = R; g1.bk = im1(1); g1.bn = im1(Dr); g1.tb = F64;
   g1.C = w.T1; g1.cm = im1(1); g1.cn = im1(Dl * d); g1.tc = op.tmp;
-  run_gemm(w.c, g1, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g1, op.cmp, w.splitk, w.splitk_bytes, w.stream);
   // batch over c; m = a; k = s + d*w'; n = w + wl*s'
   Gemm g2 = mk(Dl, wl * d, d * wr);
   g2.batch = Dr0;
@@ -137,7 +151,7 @@ void prod_AOR(Work& w, const Op& op, void* T2, const void* A, i64 Dl, i64 d, i64
   // O[w,s',w',s] at w + wl*s' + wl*d*w' + wl*d*wr*s
   g2.B = O; g2.bk = im2(d, wl * d * wr, wl * d); g2.bn = im1(1); g2.b_bs = 0; g2.tb = F64;
   g2.C = T2; g2.cm = im1(1); g2.cn = im1(Dl); g2.c_bs = Dl * wl * d; g2.tc = op.tmp;
-  run_gemm(w.c, g2, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g2, op.cmp, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // prod_AOL (mps_base.jl:26-39)
@@ -149,7 +163,7 @@ void prod_AOL(Work& w, const Op& op, void* AOL, const void* A, i64 Dl, i64 d, i6
   g1.A = L; g1.am = im1(Dl); g1.ak = im1(1); g1.conjA = true; g1.ta = F64;
   g1.B = A; g1.bk = im1(1); g1.bn = im1(Dl); g1.tb = F64;
   g1.C = w.T1; g1.cm = im1(1); g1.cn = im1(wl * Dl0); g1.tc = op.tmp;
-  run_gemm(w.c, g1, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g1, op.cmp, w.splitk, w.splitk_bytes, w.stream);
   // batch over a'; m = b; k = w + wl*s; n = s' + d*w'
   Gemm g2 = mk(Dl0, d * wr, wl * d);
   g2.batch = Dr;
@@ -158,7 +172,7 @@ void prod_AOL(Work& w, const Op& op, void* AOL, const void* A, i64 Dl, i64 d, i6
   g2.B = O; g2.bk = im2(wl, 1, wl * d * wr); g2.bn = im1(wl); g2.tb = F64;
   // AOL[b,s',a',w'] at b + Dl0*s' + Dl0*d*a' + Dl0*d*Dr*w'
   g2.C = AOL; g2.cm = im1(1); g2.cn = im2(d, Dl0, Dl0 * d * Dr); g2.c_bs = Dl0 * d; g2.tc = op.tmp;
-  run_gemm(w.c, g2, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g2, op.cmp, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // prod_LR! with 3-index L, 4-index R (mps_base.jl:53-61):
@@ -170,7 +184,7 @@ void prod_LR_34(Work& w, const Op& op, void* out, const void* L, const void* T2,
   g.B = T2; g.bk = im1(1); g.bn = im1(Dl * wl); g.tb = op.tmp;
   g.C = out; g.cm = im1(1); g.cn = im1(Dl0); g.tc = F64;
   if (accumulate) g.beta_re = 1.0;
-  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // prod_LR! with 4-index L, 3-index R (mps_base.jl:73-81):
@@ -182,7 +196,7 @@ void prod_LR_43(Work& w, const Op& op, void* out, const void* AOL, const void* R
   g.B = R; g.bk = im1(1); g.bn = im1(Dr * wr); g.tb = F64;
   g.C = out; g.cm = im1(1); g.cn = im1(Dl0 * d); g.tc = F64;
   if (accumulate) g.beta_re = 1.0;
-  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // update_re! (mps_base.jl:151-160): R'[a,w,b] = sum_{s',c} T2[a,w,s',c] conj(A0[b,s',c])
@@ -192,7 +206,7 @@ void update_re(Work& w, const Op& op, void* out, const void* T2, const void* A0,
   g.A = T2; g.am = im1(1); g.ak = im1(Dl * wl); g.ta = op.tmp;
   g.B = A0; g.bk = im1(Dl0); g.bn = im1(1); g.conjB = true; g.tb = F64;
   g.C = out; g.cm = im1(1); g.cn = im1(Dl * wl); g.tc = F64;
-  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // update_le! (mps_base.jl:232-241): L'[a',w',c] = sum_{b,s'} conj(AOL[b,s',a',w']) A0[b,s',c]
@@ -202,7 +216,7 @@ void update_le(Work& w, const Op& op, void* out, const void* AOL, const void* A0
   g.A = AOL; g.am = im1(Dl0 * d); g.ak = im1(1); g.conjA = true; g.ta = op.tmp;
   g.B = A0; g.bk = im1(1); g.bn = im1(Dl0 * d); g.tb = F64;
   g.C = out; g.cm = im1(1); g.cn = im1(Dr * wr); g.tc = F64;
-  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, op.cmp, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // prod_LR with 2-index L (mps_base.jl:85-118): out[l1,r2,r3] = L * R, or L^T * R (prod_LR_co)
@@ -220,7 +234,7 @@ void prod_LR_23(Work& w, void* out, const void* L, const void* R, i64 l1, i64 r1
   g.B = R; g.bk = im1(1); g.bn = im1(r1);
   g.C = out; g.cm = im1(1); g.cn = im1(l1);
   if (accumulate) g.beta_re = 1.0;
-  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // prod_LR with 3-index L, 2-index R (mps_base.jl:122-137): out[l1,l2,r2] = L[l1*l2, l3] * R[l3, r2]
@@ -239,7 +253,7 @@ void prod_LR_32(Work& w, void* out, const void* L, const void* R, i64 l1, i64 l2
   }
   g.C = out; g.cm = im1(1); g.cn = im1(l1 * l2);
   if (accumulate) g.beta_re = 1.0;
-  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // update_renv! (mps_base.jl:193-199): out[a, c_l] = sum_{s,c} AR[a,s,c] conj(A0[c_l,s,c])
@@ -248,7 +262,7 @@ void update_renv(Work& w, void* out, const void* A0, const void* AR, i64 a1, i64
   g.A = AR; g.am = im1(1); g.ak = im1(ar1);
   g.B = A0; g.bk = im1(a1); g.bn = im1(1); g.conjB = true;
   g.C = out; g.cm = im1(1); g.cn = im1(ar1);
-  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes, w.stream);
 }
 
 // update_lenv! family (mps_base.jl:267-325), A0[a1,d,a3], AL[a1,d,al3]
@@ -263,14 +277,14 @@ void update_lenv(Work& w, void* out, const void* A0, const void* AL, i64 a1, i64
     g.A = AL; g.am = im1(K); g.ak = im1(1);
     g.B = A0; g.bk = im1(1); g.bn = im1(K); g.conjB = true;
     g.C = out; g.cm = im1(1); g.cn = im1(al3);
-    run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes);
+    run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes, w.stream);
     return;
   }
   Gemm g = mk(a3, al3, K);
   g.A = A0; g.am = im1(K); g.ak = im1(1); g.conjA = (mode == 0);
   g.B = AL; g.bk = im1(1); g.bn = im1(K);
   g.C = out; g.cm = im1(1); g.cn = im1(a3);
-  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes);
+  run_gemm(w.c, g, CMP_F64, w.splitk, w.splitk_bytes, w.stream);
 }
 
 }  // namespace mpship

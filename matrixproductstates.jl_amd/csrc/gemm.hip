@@ -12,10 +12,13 @@
 //               Re = P1 - P2, Im = P3 - P1 - P2), v_mfma_f32_16x16x4_f32,
 //               FP32 accumulation.
 //
-// Workgroup tile 64x64 (complex), K step 16, 256 threads = 4 waves of 64,
-// each wave a 32x32 block = 2x2 MFMA 16x16 tiles.  Deep-K products are split
-// over blockIdx.y (split-K) into a partial buffer and reduced by a second
-// kernel in a fixed order, so results are bitwise reproducible run to run.
+// Workgroup tile TM x TM (complex; TM = 64 or 32), K step 16, 256 threads =
+// 4 waves of 64, each wave a TM/2 x TM/2 block of MFMA 16x16 tiles (2x2 or
+// 1x1).  The 32 tile serves small outputs (a few dozen 64 tiles would leave
+// most CUs idle, and splitting a short K costs more in partials than it
+// gains).  Deep-K products are split over blockIdx.y (split-K) into a
+// partial buffer and reduced by a second kernel in a fixed order, so results
+// are bitwise reproducible run to run.
 #include "common.h"
 
 namespace mpship {
@@ -23,7 +26,7 @@ namespace mpship {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+constexpr int BK = 16, NT = 256;
 
 struct DevMap {
   int32_t n0;
@@ -71,19 +74,20 @@ __device__ __forceinline__ void stc(float2* p, double re, double im) {
 }
 
 // ------------------------------------------------------------ tile loaders
-// Each thread owns BM*BK/NT elements of the A tile (BM x BK) and as many of
-// the B tile (BK x BN).  fetch_* issues the tile's global loads into
+// Each thread owns TM*BK/NT elements of the A tile (TM x BK) and as many of
+// the B tile (BK x TM).  fetch_* issues the tile's global loads into
 // registers (clamped addresses, no branches around the loads), put_* masks
 // and stores them to LDS as split planes S[plane][k][m|n]; the main loop
 // fetches tile k+1 while the MFMAs consume tile k.
-constexpr int FRAG = (BM * BK) / NT;
-static_assert((BN * BK) / NT == FRAG, "A and B tiles hold the same number of elements per thread");
+template <int TM>
 struct Frag {
-  double re[FRAG], im[FRAG];
+  static constexpr int N = (TM * BK) / NT;
+  double re[N], im[N];
 };
 
-template <bool TWO, typename T>
-__device__ __forceinline__ void fetch_a(const GemmParams& p, const T* A, int m0, int k0, int kend, Frag& f) {
+template <int TM, bool TWO, typename T>
+__device__ __forceinline__ void fetch_a(const GemmParams& p, const T* A, int m0, int k0, int kend, Frag<TM>& f) {
+  constexpr int BM = TM;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < (BM * BK) / NT; ++j) {
@@ -94,9 +98,10 @@ __device__ __forceinline__ void fetch_a(const GemmParams& p, const T* A, int m0,
   }
 }
 
-template <typename S, int NPL>
-__device__ __forceinline__ void put_a(const GemmParams& p, int m0, int k0, int kend, const Frag& f,
-                                      S (*As)[BK][BM + 1]) {
+template <int TM, typename S, int NPL>
+__device__ __forceinline__ void put_a(const GemmParams& p, int m0, int k0, int kend, const Frag<TM>& f,
+                                      S (*As)[BK][TM + 1]) {
+  constexpr int BM = TM;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < (BM * BK) / NT; ++j) {
@@ -111,8 +116,9 @@ __device__ __forceinline__ void put_a(const GemmParams& p, int m0, int k0, int k
   }
 }
 
-template <bool TWO, typename T>
-__device__ __forceinline__ void fetch_b(const GemmParams& p, const T* B, int n0, int k0, int kend, Frag& f) {
+template <int TM, bool TWO, typename T>
+__device__ __forceinline__ void fetch_b(const GemmParams& p, const T* B, int n0, int k0, int kend, Frag<TM>& f) {
+  constexpr int BN = TM;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < (BN * BK) / NT; ++j) {
@@ -123,9 +129,10 @@ __device__ __forceinline__ void fetch_b(const GemmParams& p, const T* B, int n0,
   }
 }
 
-template <typename S, int NPL>
-__device__ __forceinline__ void put_b(const GemmParams& p, int n0, int k0, int kend, const Frag& f,
-                                      S (*Bs)[BK][BN + 1]) {
+template <int TM, typename S, int NPL>
+__device__ __forceinline__ void put_b(const GemmParams& p, int n0, int k0, int kend, const Frag<TM>& f,
+                                      S (*Bs)[BK][TM + 1]) {
+  constexpr int BN = TM;
   const int tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < (BN * BK) / NT; ++j) {
@@ -180,48 +187,49 @@ __device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split
 // planes are formed once per tile in LDS.  Rounding stays normwise
 // (about 3 eps |a||b| per product against 2 eps for four MFMAs), and the
 // MFMA work drops by a quarter: FP64 MFMA throughput bounds these tiles.
-template <typename TA, typename TB, typename TC, bool TWO>
+template <int TM, typename TA, typename TB, typename TC, bool TWO>
 __global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
-  __shared__ double As[3][BK][BM + 1];
-  __shared__ double Bs[3][BK][BN + 1];
+  constexpr int WT = TM / 32;   // 16x16 MFMA tiles per wave and dimension
+  __shared__ double As[3][BK][TM + 1];
+  __shared__ double Bs[3][BK][TM + 1];
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * TM, n0 = tn * TM;
   const int kbeg = split * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
   const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
   const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TM / 2);
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4d p1[2][2], p2[2][2], p3[2][2];
+  v4d p1[WT][WT], p2[WT][WT], p3[WT][WT];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WT; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WT; ++j) {
       p1[i][j] = v4d{0, 0, 0, 0};
       p2[i][j] = v4d{0, 0, 0, 0};
       p3[i][j] = v4d{0, 0, 0, 0};
     }
 
-  Frag fa, fb;
-  fetch_a<TWO>(p, A, m0, kbeg, kend, fa);
-  fetch_b<TWO>(p, B, n0, kbeg, kend, fb);
+  Frag<TM> fa, fb;
+  fetch_a<TM, TWO>(p, A, m0, kbeg, kend, fa);
+  fetch_b<TM, TWO>(p, B, n0, kbeg, kend, fb);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    put_a<double, 3>(p, m0, k0, kend, fa, As);
-    put_b<double, 3>(p, n0, k0, kend, fb, Bs);
+    put_a<TM, double, 3>(p, m0, k0, kend, fa, As);
+    put_b<TM, double, 3>(p, n0, k0, kend, fb, Bs);
     __syncthreads();
     if (k0 + BK < kend) {
-      fetch_a<TWO>(p, A, m0, k0 + BK, kend, fa);
-      fetch_b<TWO>(p, B, n0, k0 + BK, kend, fb);
+      fetch_a<TM, TWO>(p, A, m0, k0 + BK, kend, fa);
+      fetch_b<TM, TWO>(p, B, n0, k0 + BK, kend, fb);
     }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
       const int k = kq * 4 + lk;
-      double a_r[2], a_i[2], a_s[2], b_r[2], b_i[2], b_s[2];
+      double a_r[WT], a_i[WT], a_s[WT], b_r[WT], b_i[WT], b_s[WT];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < WT; ++i) {
         a_r[i] = As[0][k][wm + i * 16 + lr];
         a_i[i] = As[1][k][wm + i * 16 + lr];
         a_s[i] = As[2][k][wm + i * 16 + lr];
@@ -230,9 +238,9 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
         b_s[i] = Bs[2][k][wn + i * 16 + lr];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < WT; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < WT; ++j) {
           p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], p1[i][j], 0, 0, 0);
           p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_i[j], p2[i][j], 0, 0, 0);
           p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_s[i], b_s[j], p3[i][j], 0, 0, 0);
@@ -241,9 +249,9 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WT; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WT; ++j) {
       double cr[4], ci[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -256,48 +264,49 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
 }
 
 // --------------------------------------------------------- FP32 3M kernel
-template <typename TA, typename TB, typename TC, bool TWO>
+template <int TM, typename TA, typename TB, typename TC, bool TWO>
 __global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
-  __shared__ float As[3][BK][BM + 1];
-  __shared__ float Bs[3][BK][BN + 1];
+  constexpr int WT = TM / 32;   // 16x16 MFMA tiles per wave and dimension
+  __shared__ float As[3][BK][TM + 1];
+  __shared__ float Bs[3][BK][TM + 1];
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * TM, n0 = tn * TM;
   const int kbeg = split * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
   const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
   const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TM / 2);
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4f p1[2][2], p2[2][2], p3[2][2];
+  v4f p1[WT][WT], p2[WT][WT], p3[WT][WT];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WT; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WT; ++j) {
       p1[i][j] = v4f{0, 0, 0, 0};
       p2[i][j] = v4f{0, 0, 0, 0};
       p3[i][j] = v4f{0, 0, 0, 0};
     }
 
-  Frag fa, fb;
-  fetch_a<TWO>(p, A, m0, kbeg, kend, fa);
-  fetch_b<TWO>(p, B, n0, kbeg, kend, fb);
+  Frag<TM> fa, fb;
+  fetch_a<TM, TWO>(p, A, m0, kbeg, kend, fa);
+  fetch_b<TM, TWO>(p, B, n0, kbeg, kend, fb);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    put_a<float, 3>(p, m0, k0, kend, fa, As);
-    put_b<float, 3>(p, n0, k0, kend, fb, Bs);
+    put_a<TM, float, 3>(p, m0, k0, kend, fa, As);
+    put_b<TM, float, 3>(p, n0, k0, kend, fb, Bs);
     __syncthreads();
     if (k0 + BK < kend) {
-      fetch_a<TWO>(p, A, m0, k0 + BK, kend, fa);
-      fetch_b<TWO>(p, B, n0, k0 + BK, kend, fb);
+      fetch_a<TM, TWO>(p, A, m0, k0 + BK, kend, fa);
+      fetch_b<TM, TWO>(p, B, n0, k0 + BK, kend, fb);
     }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
       const int k = kq * 4 + lk;
-      float a_r[2], a_i[2], a_s[2], b_r[2], b_i[2], b_s[2];
+      float a_r[WT], a_i[WT], a_s[WT], b_r[WT], b_i[WT], b_s[WT];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < WT; ++i) {
         a_r[i] = As[0][k][wm + i * 16 + lr];
         a_i[i] = As[1][k][wm + i * 16 + lr];
         a_s[i] = As[2][k][wm + i * 16 + lr];
@@ -306,9 +315,9 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
         b_s[i] = Bs[2][k][wn + i * 16 + lr];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < WT; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < WT; ++j) {
           p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[i], b_r[j], p1[i][j], 0, 0, 0);
           p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[i], b_i[j], p2[i][j], 0, 0, 0);
           p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_s[i], b_s[j], p3[i][j], 0, 0, 0);
@@ -317,9 +326,9 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WT; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WT; ++j) {
       double cr[4], ci[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -341,10 +350,23 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmParams p) {
     const int b = (int)(idx / MN);
     const int64_t e = idx - (int64_t)b * MN;
     const int col = (int)(e / p.M), row = (int)(e - (int64_t)col * p.M);
+    // partials summed in split order; eight loads in flight per batch
     double sr = 0.0, si = 0.0;
-    const double2* P = (const double2*)p.P;
-    for (int s = 0; s < p.splits; ++s) {
-      const double2 v = P[((int64_t)s * p.batch + b) * MN + e];
+    const double2* P = (const double2*)p.P + (int64_t)b * MN + e;
+    const int64_t ss = (int64_t)p.batch * MN;
+    int s = 0;
+    for (; s + 8 <= p.splits; s += 8) {
+      double2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = P[(s + q) * ss];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        sr += v[q].x;
+        si += v[q].y;
+      }
+    }
+    for (; s < p.splits; ++s) {
+      const double2 v = P[s * ss];
       sr += v.x;
       si += v.y;
     }
@@ -368,22 +390,30 @@ static DevMap dm(const IdxMap& m) { return DevMap{m.n0, m.s0, m.s1}; }
 static bool single_unit(const IdxMap& m) { return m.n0 == 0 && m.s0 == 1; }
 
 struct GemmPlan {
-  int tiles_m, tiles_n, splits, kchunk;
+  int tm, tiles_m, tiles_n, splits, kchunk;
 };
 
-static GemmPlan make_plan(const Gemm& g, int num_cus) {
+static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
+  (void)cmp;   // one plan serves the FP64 and FP32 kernels
   GemmPlan pl;
-  pl.tiles_m = (int)((g.M + BM - 1) / BM);
-  pl.tiles_n = (int)((g.N + BN - 1) / BN);
-  // Split K while the tiles leave CUs idle, up to the 2 x num_cus workgroup
-  // slots (two workgroups fit per CU); a grid with a tile for every CU is
-  // not split (the partials' extra pass costs more than the second slot gains).
+  // 64 tiles unless they would leave more than half of the CUs idle
+  const int64_t t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64) * g.batch;
+  pl.tm = (2 * t64 < num_cus) ? 32 : 64;
+  pl.tiles_m = (int)((g.M + pl.tm - 1) / pl.tm);
+  pl.tiles_n = (int)((g.N + pl.tm - 1) / pl.tm);
+  // Split K while the tiles leave CUs idle, up to about 4 x num_cus
+  // workgroups (two fit per CU at a time; measured on the skinny long-K
+  // products, 1152x128x1792 and 1792x128x1152, the deeper split still wins
+  // over its partials pass); a grid with a tile for every CU is not split.
+  // Chunks are at least 32 deep for 64 tiles and 128 deep for 32 tiles (a
+  // quarter of the MFMA work per K step).
   const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch;
-  const int64_t target = 2LL * num_cus;
+  const int64_t target = 4LL * num_cus;
+  const int64_t min_chunk = (pl.tm == 64) ? 2 * BK : 8 * BK;
   int splits = 1;
-  if (blocks < num_cus && g.K > 4 * BK) {
+  if (blocks < num_cus && g.K >= 2 * min_chunk) {
     splits = (int)((target + blocks - 1) / blocks);
-    const int64_t max_splits = (g.K + 2 * BK - 1) / (2 * BK);   // >= 32-deep chunks
+    const int64_t max_splits = (g.K + min_chunk - 1) / min_chunk;
     if (splits > max_splits) splits = (int)max_splits;
     if (splits > 32) splits = 32;
     if (splits < 1) splits = 1;
@@ -397,41 +427,55 @@ static GemmPlan make_plan(const Gemm& g, int num_cus) {
   return pl;
 }
 
-size_t gemm_workspace_bytes(const Gemm& g, Compute, int num_cus) {
+size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus) {
   if (g.M == 0 || g.N == 0) return 0;
-  GemmPlan pl = make_plan(g, num_cus);
+  GemmPlan pl = make_plan(g, num_cus, cmp);
   if (pl.splits <= 1) return 0;
   return (size_t)pl.splits * g.batch * g.M * g.N * sizeof(double2);
 }
 
 template <typename TA, typename TB, typename TC>
-static void launch_typed(const GemmParams& p, Compute cmp, dim3 grid, hipStream_t s) {
+static void launch_typed(const GemmParams& p, Compute cmp, int tm, dim3 grid, hipStream_t s) {
   const bool two = p.am.n0 || p.ak.n0 || p.bk.n0 || p.bn.n0;
   if (cmp == CMP_F64) {
-    if (two)
-      hipLaunchKernelGGL((zgemm_f64_kernel<TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
-    else
-      hipLaunchKernelGGL((zgemm_f64_kernel<TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+    if (tm == 32) {
+      if (two)
+        hipLaunchKernelGGL((zgemm_f64_kernel<32, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
+      else
+        hipLaunchKernelGGL((zgemm_f64_kernel<32, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+    } else {
+      if (two)
+        hipLaunchKernelGGL((zgemm_f64_kernel<64, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
+      else
+        hipLaunchKernelGGL((zgemm_f64_kernel<64, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+    }
   } else {
-    if (two)
-      hipLaunchKernelGGL((zgemm_f32_3m_kernel<TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
-    else
-      hipLaunchKernelGGL((zgemm_f32_3m_kernel<TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+    if (tm == 32) {
+      if (two)
+        hipLaunchKernelGGL((zgemm_f32_3m_kernel<32, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
+      else
+        hipLaunchKernelGGL((zgemm_f32_3m_kernel<32, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+    } else {
+      if (two)
+        hipLaunchKernelGGL((zgemm_f32_3m_kernel<64, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
+      else
+        hipLaunchKernelGGL((zgemm_f32_3m_kernel<64, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
+    }
   }
 }
 
 template <typename TA, typename TB>
-static void launch_tc(const GemmParams& p, DType tc, Compute cmp, dim3 grid, hipStream_t s) {
+static void launch_tc(const GemmParams& p, DType tc, Compute cmp, int tm, dim3 grid, hipStream_t s) {
   if (tc == F64)
-    launch_typed<TA, TB, double2>(p, cmp, grid, s);
+    launch_typed<TA, TB, double2>(p, cmp, tm, grid, s);
   else
-    launch_typed<TA, TB, float2>(p, cmp, grid, s);
+    launch_typed<TA, TB, float2>(p, cmp, tm, grid, s);
 }
 
 void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_cus, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
   REQUIRE(g.M < (1LL << 31) && g.N < (1LL << 31) && g.K < (1LL << 31), "GEMM too large");
-  GemmPlan pl = make_plan(g, num_cus);
+  GemmPlan pl = make_plan(g, num_cus, cmp);
   GemmParams p;
   p.M = (int)g.M;
   p.N = (int)g.N;
@@ -466,13 +510,13 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
   }
   dim3 grid((unsigned)(pl.tiles_m * pl.tiles_n), (unsigned)pl.splits, (unsigned)g.batch);
   if (g.ta == F64 && g.tb == F64)
-    launch_tc<double2, double2>(p, g.tc, cmp, grid, s);
+    launch_tc<double2, double2>(p, g.tc, cmp, pl.tm, grid, s);
   else if (g.ta == F64 && g.tb == F32)
-    launch_tc<double2, float2>(p, g.tc, cmp, grid, s);
+    launch_tc<double2, float2>(p, g.tc, cmp, pl.tm, grid, s);
   else if (g.ta == F32 && g.tb == F64)
-    launch_tc<float2, double2>(p, g.tc, cmp, grid, s);
+    launch_tc<float2, double2>(p, g.tc, cmp, pl.tm, grid, s);
   else
-    launch_tc<float2, float2>(p, g.tc, cmp, grid, s);
+    launch_tc<float2, float2>(p, g.tc, cmp, pl.tm, grid, s);
   HIPCHK(hipGetLastError());
   if (pl.splits > 1) {
     dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536));

@@ -19,6 +19,18 @@ struct Plan {
   void* extra = nullptr;   // caller-sized buffer (extra_bytes), for per-call arrays
   Work work{};
   Op op{};
+  // Overlap-term chains (their environments and products) run on the
+  // context's side stream with their own split-K partials, concurrently with
+  // the operator chain on the main stream: fork() lets the side stream
+  // continue from the main stream's current point, join() makes the main
+  // stream wait for everything queued on the side stream.  Without overlap
+  // terms, for small sites, or while kernels are being timed, `side` is
+  // `work` and both are no-ops.
+  Work side{};
+  bool conc = false;
+  void* splitk2 = nullptr;
+  void fork();
+  void join();
   Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
        int n_ovl, size_t extra_bytes = 0);
   Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl, 0); }

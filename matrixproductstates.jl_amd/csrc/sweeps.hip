@@ -45,16 +45,30 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
   scr.add(&R, (size_t)Dm * Dm * dm * cz);
   scr.add(&envA, big * cz);
   scr.add(&envB, big * cz);
-  // split-K partials: splits * blocks <= 2*CUs + blocks, blocks < 2*CUs when split
-  const size_t sk = (size_t)(4 * c->num_cus + 64) * 64 * 64 * cz;
+  // split-K partials: splits * blocks < 4*CUs + blocks, blocks < CUs when split
+  const size_t sk = (size_t)(5 * c->num_cus + 64) * 64 * 64 * cz;
   scr.add(&splitk, sk);
   splitk_bytes = sk;
+  // the side stream pays off once a site's products fill a good part of the
+  // GPU (measured: D = 128, d = 9 gains; bond-16/32 chains served many to a
+  // GPU lose, their streams then outnumber the hardware queues)
+  conc = n_ovl > 0 && !c->profiling && site >= 32768;
+  if (conc) {
+    c->ensure_side();
+    scr.add(&splitk2, sk);
+  }
   if (extra_bytes) scr.add(&extra, extra_bytes);
   scr.commit();
   work.c = c;
   work.splitk = splitk;
   work.splitk_bytes = splitk_bytes;
   work.T1 = T1;
+  side = work;
+  if (conc) {
+    side.splitk = splitk2;
+    side.T1 = nullptr;   // the overlap terms need no T1
+    side.stream = c->side;
+  }
   op.cmp = (c->prec == 1) ? CMP_F32_3M : CMP_F64;
   op.tmp = (c->prec == 1) ? F32 : F64;
 }
@@ -117,8 +131,23 @@ double read_scalar(Ctx* c, int slot) {
   return c->hscal[0];
 }
 
+void Plan::fork() {
+  if (!conc) return;
+  HIPCHK(hipEventRecord(ctx->ev_fork, ctx->stream));
+  HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+}
+
+void Plan::join() {
+  if (!conc) return;
+  HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
+  HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+}
+
 // ---------------------------------------------------------------- helpers
 static void set1(Ctx* c, double2* p, cd v) { launch_set_scalar(p, v.real(), v.imag(), c->stream); }
+static void set1(const Work& w, double2* p, cd v) {
+  launch_set_scalar(p, v.real(), v.imag(), w.stream ? w.stream : w.c->stream);
+}
 
 static void check_same_dims(const State* a, const State* b, const char* what) {
   REQUIRE(a->N == b->N, std::string(what) + ": chain lengths differ");
@@ -179,22 +208,25 @@ void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State
   }
 }
 
-// compress_sum_var_apply_H_left! (compression.jl:889-944)
+// compress_sum_var_apply_H_left! (compression.jl:889-944).  The overlap
+// terms (environment builds, per-site products, environment updates) run on
+// the side stream; the operator term joins them before accumulating into A0.
 void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
                                    const cd* coef, State** A, MPO* H) {
-  Ctx* c = p.ctx;
   const int N = Ain->N;
+  Work& ow = p.side;
+  p.fork();
   for (int ll = 0; ll + 1 < numv; ++ll) {
     Env* e = env[ll];
-    set1(c, e->slot(0), 1.0);
+    set1(ow, e->slot(0), 1.0);
     for (int j = 0; j + 1 < N; ++j) {
       // update_lenv!(env[ll][jj+1], Ain[jj], A[ll][jj], env[ll][jj])
-      prod_LR_23(p.work, p.AL, e->slot(j), A[ll]->site(j), Ain->Dl(j), A[ll]->Dl(j), A[ll]->d[j],
+      prod_LR_23(ow, p.AL, e->slot(j), A[ll]->site(j), Ain->Dl(j), A[ll]->Dl(j), A[ll]->d[j],
                  A[ll]->Dr(j), false, false);
-      update_lenv(p.work, e->slot(j + 1), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
+      update_lenv(ow, e->slot(j + 1), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
                   A[ll]->Dr(j), 0);
     }
-    set1(c, e->slot(N), coef[ll]);
+    set1(ow, e->slot(N), coef[ll]);
   }
   State* Ah = A[numv - 1];
   init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
@@ -202,40 +234,45 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
     // overlap terms: AR = prod_LR(A[ll][jj], env[ll][jj+1]); A0 (+)= prod_LR(env[ll][jj], AR)
     for (int ll = 0; ll + 1 < numv; ++ll) {
-      prod_LR_32(p.work, p.ovl[ll], A[ll]->site(j), env[ll]->slot(j + 1), A[ll]->Dl(j), d, A[ll]->Dr(j),
+      prod_LR_32(ow, p.ovl[ll], A[ll]->site(j), env[ll]->slot(j + 1), A[ll]->Dl(j), d, A[ll]->Dr(j),
                  Dr0, false);
-      prod_LR_23(p.work, A0->site(j), env[ll]->slot(j), p.ovl[ll], Dl0, A[ll]->Dl(j), d, Dr0, false,
+      prod_LR_23(ow, A0->site(j), env[ll]->slot(j), p.ovl[ll], Dl0, A[ll]->Dl(j), d, Dr0, false,
                  ll > 0);
     }
     // operator term: AOR = prod_AOR(A[numv][jj], H[jj], envop[jj+1]); A0 += prod_LR(envop[jj], AOR)
     prod_AOR(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
              envop->slot(j + 1), Dr0);
+    p.join();
     prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, Ah->Dl(j), H->wl(j), Dl0, d, Dr0,
                numv > 1);
     if (j == 0) break;
     qr_right(p, A0, j, nullptr);
+    p.fork();
     for (int ll = 0; ll + 1 < numv; ++ll)
-      update_renv(p.work, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
+      update_renv(ow, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
     update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
   }
+  p.join();
 }
 
-// compress_sum_var_apply_H_right! (compression.jl:1007-1063).  A0 may alias A[0].
+// compress_sum_var_apply_H_right! (compression.jl:1007-1063).  A0 may alias
+// A[0] (both touched only by the side stream's overlap chain, in order).
 void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
                                     const cd* coef, State** A, MPO* H) {
-  Ctx* c = p.ctx;
   const int N = Ain->N;
+  Work& ow = p.side;
+  p.fork();
   for (int ll = 0; ll + 1 < numv; ++ll) {
     Env* e = env[ll];
-    set1(c, e->slot(N), 1.0);
+    set1(ow, e->slot(N), 1.0);
     for (int j = N - 1; j >= 1; --j) {
       // update_renv!(env[ll][jj], Ain[jj], A[ll][jj], env[ll][jj+1])
-      prod_LR_32(p.work, p.AL, A[ll]->site(j), e->slot(j + 1), A[ll]->Dl(j), A[ll]->d[j], A[ll]->Dr(j),
+      prod_LR_32(ow, p.AL, A[ll]->site(j), e->slot(j + 1), A[ll]->Dl(j), A[ll]->d[j], A[ll]->Dr(j),
                  Ain->Dr(j), false);
-      update_renv(p.work, e->slot(j), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
+      update_renv(ow, e->slot(j), Ain->site(j), p.AL, Ain->Dl(j), Ain->d[j], Ain->Dr(j),
                   A[ll]->Dl(j));
     }
-    set1(c, e->slot(0), coef[ll]);
+    set1(ow, e->slot(0), coef[ll]);
   }
   State* Ah = A[numv - 1];
   init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
@@ -243,21 +280,24 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
     // overlap terms first: AL[ll] reads A[ll][jj] before A0[jj] is written
     for (int ll = 0; ll + 1 < numv; ++ll)
-      prod_LR_23(p.work, p.ovl[ll], env[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
+      prod_LR_23(ow, p.ovl[ll], env[ll]->slot(j), A[ll]->site(j), Dl0, A[ll]->Dl(j), d, A[ll]->Dr(j),
                  false, false);
     for (int ll = 0; ll + 1 < numv; ++ll)
-      prod_LR_32(p.work, A0->site(j), p.ovl[ll], env[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0, ll > 0);
+      prod_LR_32(ow, A0->site(j), p.ovl[ll], env[ll]->slot(j + 1), Dl0, d, A[ll]->Dr(j), Dr0, ll > 0);
     // operator term: AOL = prod_AOL(A[numv][jj], H[jj], envop[jj]); A0 += prod_LR(AOL, envop[jj+1])
     prod_AOL(p.work, p.op, p.T2, Ah->site(j), Ah->Dl(j), d, Ah->Dr(j), H->site(j), H->wl(j), H->wr(j),
              envop->slot(j), Dl0);
+    p.join();
     prod_LR_43(p.work, p.op, A0->site(j), p.T2, envop->slot(j + 1), Dl0, d, Ah->Dr(j), H->wr(j), Dr0,
                numv > 1);
     if (j == N - 1) break;
     qr_left(p, A0, j, nullptr);
+    p.fork();
     for (int ll = 0; ll + 1 < numv; ++ll)
-      update_lenv(p.work, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
+      update_lenv(ow, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
     update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
   }
+  p.join();
 }
 
 // compress_var_apply_H! (compression.jl:217-258)
