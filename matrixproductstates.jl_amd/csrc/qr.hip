@@ -184,8 +184,10 @@ __global__ __launch_bounds__(QR_NT) void qr_householder_kernel(double2* __restri
 //   qr_colnorm  one wave per column: ||a_c||^2 (rank cutoff only)
 //   qr_panel    one workgroup factors an NB-column panel held in registers,
 //               rows distributed over threads
-//   qr_apply    trailing columns <- (I - V T V^H)^H columns, 4 per workgroup
-//   qr_formq    Q = (I - V_0 T_0 V_0^H) ... (I - V_p T_p V_p^H) I(:, 0:n),
+//   qr_apply    trailing columns <- (I - V T V^H)^H columns, one per workgroup
+//   Q formation, n <= 256: G = V^H V (GEMM), W from T^{-1} W = V[0:n,:]^H
+//               (qr_wsolve), Q = I(:, 0:n) - V W (GEMM), over the panel
+//   qr_formq    n > 256: Q = (I - V_0 T_0 V_0^H) ... (I - V_p T_p V_p^H) I(:, 0:n),
 //               4 columns per workgroup (reflectors read from a separate
 //               buffer, so Q can overwrite the panel in place)
 
@@ -998,98 +1000,126 @@ __global__ __launch_bounds__(WY_NT) void qr_formq_kernel(double2* __restrict__ A
 }
 
 // ---------------------------------------------------------------------------
-// Q from the panel factors with two MFMA GEMMs.  With B_p = I - V_p T_p V_p^H
-// the panel block reflectors, B_p B_{p+1} ... = I - sum_q V_q Z_q^H where
-//   Z_p^H = T_p (V_p^H - sum_{q>p} G_pq Z_q^H),  G_pq = V_p^H V_q,
-// so Q = B_0 ... B_{np-1} I(:, 0:n) = I(:, 0:n) - V W with the block rows
-//   W_p = T_p (V_p[0:n, :]^H - sum_{q>p} G_pq W_q)          (NB x n).
-// qr_formw_kernel does this block back-substitution (8 columns of W per
-// workgroup, G's block row staged in LDS) and copies R out; the GEMM
-// G = V^H V comes before it, and Q = I(:, 0:n) - V W after it (written over
-// the panel, the identity added in the GEMM's epilogue).
-constexpr int FW_CW = 8;           // W columns per workgroup
-constexpr int FW_NMAX = 256;       // n limit of the LDS layout
+// Q from the reflectors with two MFMA GEMMs and one substitution.  The
+// product of all n reflectors is H(0)...H(n-1) = I - V T V^H with
+//   T^{-1} = diag(1/tau) + striu(V^H V)
+// (T upper triangular; the UT form of the compact-WY factor), so
+//   Q = H(0)...H(n-1) I(:, 0:n) = I(:, 0:n) - V W,   W = T V[0:n, :]^H.
+// The GEMM G = V^H V comes first; qr_wsolve_kernel solves T^{-1} W = V[0:n,:]^H
+// by backward substitution, w_i = tau_i (b_i - sum_{k>i} G[i][k] w_k)
+// (tau_i = 0 gives w_i = 0), and copies R out; then one GEMM forms
+// Q = I(:, 0:n) - V W over the panel, the identity added in its epilogue.
+constexpr int FW_NMAX = 256;       // n limit of the substitution kernel
 
-// thread (r, c, ks): row r of the block, column c of the workgroup, slice ks
-// of the k-sum (KS = 256 / (NB * FW_CW) slices, combined through LDS)
-template <int NB>
-__global__ __launch_bounds__(256) void qr_formw_kernel(const double2* __restrict__ A, int m, int n, int64_t rs,
-                                                       int64_t cs, const double2* __restrict__ Vs,
-                                                       const double2* __restrict__ Tall,
-                                                       const double2* __restrict__ G, double2* __restrict__ W,
-                                                       double2* __restrict__ Rout) {
-  constexpr int KS = 256 / (NB * FW_CW);
-  __shared__ double2 sW[FW_NMAX][FW_CW];
-  __shared__ double2 sG[NB][FW_NMAX + 1];
-  __shared__ double2 sp[KS][FW_CW][NB + 1];
-  const int tid = threadIdx.x, r = tid % NB, cc = (tid / NB) % FW_CW, ks = tid / (NB * FW_CW);
-  const int cbase = blockIdx.x * FW_CW;
-  const int c = cbase + cc;
+// One wave per column c of W (4 per workgroup).  Lane l keeps rows
+// l, l+64, ... of that column's running right-hand side; at step i the lane
+// holding row i finishes w_i (and keeps it in place: rows above i are never
+// touched again), a lane read broadcasts it, and every lane subtracts
+// G[k][i] w_i from its rows k < i (column i of G's strict upper triangle;
+// the staged zeros leave the finished rows k >= i alone).  G is staged in LDS
+// CH columns at a time, the next chunk's loads in flight during the current
+// chunk's steps; each chunk's G entries and tau values move to registers,
+// and the steps are unrolled per row slot, so a step is a short VALU chain.
+template <int NS>
+__global__ __launch_bounds__(256) void qr_wsolve_kernel(const double2* __restrict__ A, int m, int n, int64_t rs,
+                                                        int64_t cs, const double2* __restrict__ Vs,
+                                                        const double2* __restrict__ G,
+                                                        const double2* __restrict__ tau, double2* __restrict__ W,
+                                                        double2* __restrict__ Rout) {
+  constexpr int NR = 64 * NS;                  // rows covered (n <= NR)
+  constexpr int CH = (NS <= 2) ? 16 : 8;       // G columns per chunk (divides 64)
+  constexpr int LPT = (CH * NR) / 256;         // staged elements per thread per chunk
+  __shared__ double2 sG[CH][NR];
+  __shared__ double2 sTau[NR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = 4 * (int)blockIdx.x + wave;
   const int cl = min(c, n - 1);
-  const int np = (n + NB - 1) / NB;
   // R (upper triangle of the packed panel) out before Q overwrites it
   if (Rout != nullptr) {
-    for (int e = tid; e < FW_CW * n; e += 256) {
-      const int rr = e % n, q = cbase + e / n;
-      if (q < n)
-        Rout[(int64_t)q * n + rr] = (rr <= q) ? A[(int64_t)rr * rs + (int64_t)q * cs] : make_double2(0.0, 0.0);
+    for (int64_t e = (int64_t)blockIdx.x * 256 + tid; e < (int64_t)n * n; e += (int64_t)gridDim.x * 256) {
+      const int r = (int)(e % n), q = (int)(e / n);
+      Rout[e] = (r <= q) ? A[(int64_t)r * rs + (int64_t)q * cs] : make_double2(0.0, 0.0);
     }
   }
-  for (int p = np - 1; p >= 0; --p) {
-    const int k0 = (p + 1) * NB;
-    // stage G[pNB : pNB+NB, k0 : n]
-    for (int e = tid; e < NB * (n - k0); e += 256) {
-      const int rr = e % NB, k = k0 + e / NB;
-      sG[rr][k] = G[(int64_t)k * n + min(p * NB + rr, n - 1)];
-    }
-    __syncthreads();
-    const int row = p * NB + r;
-    // slice ks of -sum_k G[row, k] W[k, c]; two accumulators
-    double2 y0 = make_double2(0.0, 0.0), y1 = make_double2(0.0, 0.0);
-    if (row < n) {
-      int k = k0 + ks;
-      for (; k + KS < n; k += 2 * KS) {
-        const double2 g0 = sG[r][k], w0 = sW[k][cc], g1 = sG[r][k + KS], w1 = sW[k + KS][cc];
-        y0.x -= g0.x * w0.x - g0.y * w0.y;
-        y0.y -= g0.x * w0.y + g0.y * w0.x;
-        y1.x -= g1.x * w1.x - g1.y * w1.y;
-        y1.y -= g1.x * w1.y + g1.y * w1.x;
-      }
-      if (k < n) {
-        const double2 g0 = sG[r][k], w0 = sW[k][cc];
-        y0.x -= g0.x * w0.x - g0.y * w0.y;
-        y0.y -= g0.x * w0.y + g0.y * w0.x;
-      }
-    }
-    sp[ks][cc][r] = make_double2(y0.x + y1.x, y0.y + y1.y);
-    __syncthreads();
-    if (ks == 0 && row < n) {
-      // y = V_p[0:n, :]^H e_c + sum of the slices; W[row, c] = (T_p y)[r]
-      double2 y[NB];
+  for (int r = tid; r < NR; r += 256) sTau[r] = (r < n) ? tau[r] : make_double2(0.0, 0.0);
+  // b = V[c, 0:n]^H (row c of V, conjugated): b_k = conj(v_k[c])
+  double2 acc[NS];
 #pragma unroll
-      for (int s = 0; s < NB; ++s) {
-        const int rs_ = p * NB + s;
-        const double2 v = Vs[(int64_t)min(rs_, n - 1) * m + cl];
-        double2 acc = (rs_ < n) ? make_double2(v.x, -v.y) : make_double2(0.0, 0.0);
+  for (int s = 0; s < NS; ++s) {
+    const int r = lane + 64 * s;
+    const double2 x = Vs[(int64_t)min(r, n - 1) * m + cl];
+    acc[s] = (r < n) ? make_double2(x.x, -x.y) : make_double2(0.0, 0.0);
+  }
+  // chunk q holds G[0:NR, q*CH + cc] (cc < CH), zero on and below the diagonal
+  double2 reg[LPT];
+  auto fetch = [&](int q) {
 #pragma unroll
-        for (int q = 0; q < KS; ++q) {
-          acc.x += sp[q][cc][s].x;
-          acc.y += sp[q][cc][s].y;
+    for (int t = 0; t < LPT; ++t) {
+      const int e = tid + 256 * t, col = q * CH + e / NR, r = e % NR;
+      const double2 x = G[(int64_t)min(col, n - 1) * n + min(r, n - 1)];
+      reg[t] = (col < n && r < col) ? x : make_double2(0.0, 0.0);
+    }
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (int t = 0; t < LPT; ++t) {
+      const int e = tid + 256 * t;
+      sG[e / NR][e % NR] = reg[t];
+    }
+  };
+  const int nch = (n + CH - 1) / CH;
+  fetch(nch - 1);
+  put();
+  __syncthreads();
+  for (int q = nch - 1; q >= 0; --q) {
+    if (q > 0) fetch(q - 1);
+    double2 gv[CH][NS], tv[CH];
+#pragma unroll
+    for (int cc = 0; cc < CH; ++cc) {
+      tv[cc] = sTau[q * CH + cc];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) gv[cc][s] = sG[cc][lane + 64 * s];
+    }
+    const int sl = (q * CH) >> 6, ob = (q * CH) & 63;   // row slot and first owner lane of the chunk
+    static_for<0, NS>([&](auto sc) {
+      constexpr int S = decltype(sc)::value;
+      if (sl == S) {
+#pragma unroll
+        for (int cc = CH - 1; cc >= 0; --cc) {
+          if (q * CH + cc < n) {
+            const int owner = ob + cc;
+            const double2 wl = cmul(tv[cc], acc[S]);
+            const int lo_r = __builtin_amdgcn_readlane(__double2loint(wl.x), owner);
+            const int hi_r = __builtin_amdgcn_readlane(__double2hiint(wl.x), owner);
+            const int lo_i = __builtin_amdgcn_readlane(__double2loint(wl.y), owner);
+            const int hi_i = __builtin_amdgcn_readlane(__double2hiint(wl.y), owner);
+            const double2 wi = make_double2(__hiloint2double(hi_r, lo_r), __hiloint2double(hi_i, lo_i));
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+              const double2 g = gv[cc][s];
+              const double2 u = make_double2(fma(g.y, wi.y, fma(-g.x, wi.x, acc[s].x)),
+                                             fma(-g.y, wi.x, fma(-g.x, wi.y, acc[s].y)));
+              if (s == S)
+                acc[s] = (lane == owner) ? wl : u;
+              else
+                acc[s] = u;
+            }
+          }
         }
-        y[s] = acc;
       }
-      const double2* Tp = Tall + (int64_t)p * NB * NB;
-      double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-      for (int s = 0; s < NB; ++s) {
-        const double2 t = Tp[r + s * NB];
-        acc.x += t.x * y[s].x - t.y * y[s].y;
-        acc.y += t.x * y[s].y + t.y * y[s].x;
-      }
-      sW[row][cc] = acc;
-      if (c < n) W[(int64_t)c * n + row] = acc;
+    });
+    if (q > 0) {
+      __syncthreads();
+      put();
+      __syncthreads();
     }
-    __syncthreads();
+  }
+  if (c < n) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int r = lane + 64 * s;
+      if (r < n) W[(int64_t)c * n + r] = acc[s];
+    }
   }
 }
 
@@ -1145,15 +1175,22 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
                          rs, cs, c0, nbp, Vs, Tp, tau);
   }
   const QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
-  if (!PT || n <= FW_NMAX) launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
-  if (!PT)
-    hipLaunchKernelGGL((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB)), dim3(64), 0, s, q.G, n, tau,
-                       Tall);
   if (n <= FW_NMAX) {
-    hipLaunchKernelGGL((qr_formw_kernel<NB>), dim3((unsigned)((n + FW_CW - 1) / FW_CW)), dim3(256), 0, s, A, m,
-                       n, rs, cs, Vs, Tall, q.G, q.W, R_out);
+    launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+    const dim3 grid((unsigned)((n + 3) / 4));
+    if (n <= 64)
+      hipLaunchKernelGGL((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out);
+    else if (n <= 128)
+      hipLaunchKernelGGL((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out);
+    else
+      hipLaunchKernelGGL((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out);
     launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
   } else {
+    if (!PT) {
+      launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+      hipLaunchKernelGGL((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB)), dim3(64), 0, s, q.G, n,
+                         tau, Tall);
+    }
     hipLaunchKernelGGL((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4)), dim3(WY_NT), 0, s, A, m, rs,
                        cs, Vs, Tall, R_out, n);
   }
