@@ -289,6 +289,15 @@ __device__ __forceinline__ double rcp_nr(double x) {
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
 }
+// 1/sqrt(x) from the hardware estimate and two Newton steps (x > 0 normal):
+// r <- r (3 - x r^2) / 2, the second step in the residual form h = 1/2 - (x r) r / 2
+__device__ __forceinline__ double rsq_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  r = r * fma(-hx * r, r, 1.5);
+  const double h = fma(-hx * r, r, 0.5);
+  return fma(r, h, r);
+}
 
 // One workgroup factors panel columns c0 .. c0+nbp-1 (nbp <= NB), rows
 // c0..m-1.  Thread t holds rows t, t+NT, ..., t+(R-1)NT of the panel in
@@ -412,9 +421,13 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       }
     } else {
       // beta = -sign(Re alpha) ||(alpha, x)||, tau = (beta - alpha) / beta,
-      // v = x / (alpha - beta); alpha - beta != 0 here
-      const double beta = -copysign(sqrt(fma(ar, ar, fma(ai, ai, xs))), ar);
-      const double ib = rcp_nr(beta);
+      // v = x / (alpha - beta); alpha - beta != 0 here.  One reciprocal
+      // square root gives both beta and 1/beta (the norm is far from the
+      // subnormal range whenever the column is not an identity reflector).
+      const double ns = fma(ar, ar, fma(ai, ai, xs));
+      const double rn = rsq_nr(ns);
+      const double beta = -copysign(ns * rn, ar);
+      const double ib = -copysign(rn, ar);
       const double dr = ar - beta;
       const double iden = rcp_nr(fma(dr, dr, ai * ai));
       const double2 t = make_double2((beta - ar) * ib, -ai * ib);
