@@ -181,69 +181,104 @@ __device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split
   }
 }
 
-// ------------------------------------------------------------ FP64 kernel
+// ------------------------------------------------------------ MFMA kernels
 // Complex products with three real MFMAs (3M): P1 = Re a Re b, P2 = Im a Im b,
-// P3 = (Re a + Im a)(Re b + Im b); Re = P1 - P2, Im = P3 - P1 - P2.  The sum
-// planes are formed once per tile in LDS.  Rounding stays normwise
-// (about 3 eps |a||b| per product against 2 eps for four MFMAs), and the
-// MFMA work drops by a quarter: FP64 MFMA throughput bounds these tiles.
-template <int TM, typename TA, typename TB, typename TC, bool TWO>
-__global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
+// P3 = (Re a + Im a)(Re b + Im b); Re = P1 - P2, Im = P3 - P1 - P2.  Rounding
+// stays normwise (about 3 eps |a||b| per product against 2 eps for four
+// MFMAs), and the MFMA work drops by a quarter: MFMA throughput bounds these
+// tiles.  LDS holds the Re and Im planes, double buffered (one barrier per K
+// step); the sum planes are formed in registers after the LDS reads.  Step
+// it: store the tile of step it+1 (fetched one step earlier) into the other
+// buffer, issue the global loads of step it+2, run the MFMAs of step it.
+template <bool F64>
+struct MfmaT;
+template <>
+struct MfmaT<true> {
+  using S = double;
+  using V = v4d;
+  static __device__ __forceinline__ V mfma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaT<false> {
+  using S = float;
+  using V = v4f;
+  static __device__ __forceinline__ V mfma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <bool F64, int TM, typename TA, typename TB, typename TC, bool TWO>
+__device__ __forceinline__ void zgemm_body(const GemmParams& p) {
+  using S = typename MfmaT<F64>::S;
+  using V = typename MfmaT<F64>::V;
   constexpr int WT = TM / 32;   // 16x16 MFMA tiles per wave and dimension
-  __shared__ double As[3][BK][TM + 1];
-  __shared__ double Bs[3][BK][TM + 1];
+  __shared__ S As[2][2][BK][TM + 1];
+  __shared__ S Bs[2][2][BK][TM + 1];
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
   const int m0 = tm * TM, n0 = tn * TM;
   const int kbeg = split * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
+  const int nit = (kend - kbeg + BK - 1) / BK;
   const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
   const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TM / 2);
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4d p1[WT][WT], p2[WT][WT], p3[WT][WT];
+  V p1[WT][WT], p2[WT][WT], p3[WT][WT];
 #pragma unroll
   for (int i = 0; i < WT; ++i)
 #pragma unroll
     for (int j = 0; j < WT; ++j) {
-      p1[i][j] = v4d{0, 0, 0, 0};
-      p2[i][j] = v4d{0, 0, 0, 0};
-      p3[i][j] = v4d{0, 0, 0, 0};
+      p1[i][j] = V{0, 0, 0, 0};
+      p2[i][j] = V{0, 0, 0, 0};
+      p3[i][j] = V{0, 0, 0, 0};
     }
 
   Frag<TM> fa, fb;
   fetch_a<TM, TWO>(p, A, m0, kbeg, kend, fa);
   fetch_b<TM, TWO>(p, B, n0, kbeg, kend, fb);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    put_a<TM, double, 3>(p, m0, k0, kend, fa, As);
-    put_b<TM, double, 3>(p, n0, k0, kend, fb, Bs);
-    __syncthreads();
-    if (k0 + BK < kend) {
-      fetch_a<TM, TWO>(p, A, m0, k0 + BK, kend, fa);
-      fetch_b<TM, TWO>(p, B, n0, k0 + BK, kend, fb);
+  put_a<TM, S, 2>(p, m0, kbeg, kend, fa, As[0]);
+  put_b<TM, S, 2>(p, n0, kbeg, kend, fb, Bs[0]);
+  if (nit > 1) {
+    fetch_a<TM, TWO>(p, A, m0, kbeg + BK, kend, fa);
+    fetch_b<TM, TWO>(p, B, n0, kbeg + BK, kend, fb);
+  }
+  __syncthreads();
+  for (int it = 0; it < nit; ++it) {
+    const int cur = it & 1;
+    if (it + 1 < nit) {
+      const int k1 = kbeg + (it + 1) * BK;
+      put_a<TM, S, 2>(p, m0, k1, kend, fa, As[cur ^ 1]);
+      put_b<TM, S, 2>(p, n0, k1, kend, fb, Bs[cur ^ 1]);
+      if (it + 2 < nit) {
+        fetch_a<TM, TWO>(p, A, m0, k1 + BK, kend, fa);
+        fetch_b<TM, TWO>(p, B, n0, k1 + BK, kend, fb);
+      }
     }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
       const int k = kq * 4 + lk;
-      double a_r[WT], a_i[WT], a_s[WT], b_r[WT], b_i[WT], b_s[WT];
+      S a_r[WT], a_i[WT], a_s[WT], b_r[WT], b_i[WT], b_s[WT];
 #pragma unroll
       for (int i = 0; i < WT; ++i) {
-        a_r[i] = As[0][k][wm + i * 16 + lr];
-        a_i[i] = As[1][k][wm + i * 16 + lr];
-        a_s[i] = As[2][k][wm + i * 16 + lr];
-        b_r[i] = Bs[0][k][wn + i * 16 + lr];
-        b_i[i] = Bs[1][k][wn + i * 16 + lr];
-        b_s[i] = Bs[2][k][wn + i * 16 + lr];
+        a_r[i] = As[cur][0][k][wm + i * 16 + lr];
+        a_i[i] = As[cur][1][k][wm + i * 16 + lr];
+        b_r[i] = Bs[cur][0][k][wn + i * 16 + lr];
+        b_i[i] = Bs[cur][1][k][wn + i * 16 + lr];
+        a_s[i] = a_r[i] + a_i[i];
+        b_s[i] = b_r[i] + b_i[i];
       }
 #pragma unroll
       for (int i = 0; i < WT; ++i)
 #pragma unroll
         for (int j = 0; j < WT; ++j) {
-          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], p1[i][j], 0, 0, 0);
-          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_i[j], p2[i][j], 0, 0, 0);
-          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_s[i], b_s[j], p3[i][j], 0, 0, 0);
+          p1[i][j] = MfmaT<F64>::mfma(a_r[i], b_r[j], p1[i][j]);
+          p2[i][j] = MfmaT<F64>::mfma(a_i[i], b_i[j], p2[i][j]);
+          p3[i][j] = MfmaT<F64>::mfma(a_s[i], b_s[j], p3[i][j]);
         }
     }
     __syncthreads();
@@ -255,89 +290,24 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
       double cr[4], ci[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
-        cr[r] = x1 - x2;
-        ci[r] = x3 - x1 - x2;
-      }
-      store_tile<TC, true>(p, b, split, m0 + wm + i * 16, n0 + wn + j * 16, cr, ci);
-    }
-}
-
-// --------------------------------------------------------- FP32 3M kernel
-template <int TM, typename TA, typename TB, typename TC, bool TWO>
-__global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
-  constexpr int WT = TM / 32;   // 16x16 MFMA tiles per wave and dimension
-  __shared__ float As[3][BK][TM + 1];
-  __shared__ float Bs[3][BK][TM + 1];
-  const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
-  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
-  const int m0 = tm * TM, n0 = tn * TM;
-  const int kbeg = split * p.kchunk;
-  const int kend = min(p.K, kbeg + p.kchunk);
-  const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
-  const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TM / 2);
-  const int lr = lane & 15, lk = lane >> 4;
-
-  v4f p1[WT][WT], p2[WT][WT], p3[WT][WT];
-#pragma unroll
-  for (int i = 0; i < WT; ++i)
-#pragma unroll
-    for (int j = 0; j < WT; ++j) {
-      p1[i][j] = v4f{0, 0, 0, 0};
-      p2[i][j] = v4f{0, 0, 0, 0};
-      p3[i][j] = v4f{0, 0, 0, 0};
-    }
-
-  Frag<TM> fa, fb;
-  fetch_a<TM, TWO>(p, A, m0, kbeg, kend, fa);
-  fetch_b<TM, TWO>(p, B, n0, kbeg, kend, fb);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    put_a<TM, float, 3>(p, m0, k0, kend, fa, As);
-    put_b<TM, float, 3>(p, n0, k0, kend, fb, Bs);
-    __syncthreads();
-    if (k0 + BK < kend) {
-      fetch_a<TM, TWO>(p, A, m0, k0 + BK, kend, fa);
-      fetch_b<TM, TWO>(p, B, n0, k0 + BK, kend, fb);
-    }
-#pragma unroll
-    for (int kq = 0; kq < BK / 4; ++kq) {
-      const int k = kq * 4 + lk;
-      float a_r[WT], a_i[WT], a_s[WT], b_r[WT], b_i[WT], b_s[WT];
-#pragma unroll
-      for (int i = 0; i < WT; ++i) {
-        a_r[i] = As[0][k][wm + i * 16 + lr];
-        a_i[i] = As[1][k][wm + i * 16 + lr];
-        a_s[i] = As[2][k][wm + i * 16 + lr];
-        b_r[i] = Bs[0][k][wn + i * 16 + lr];
-        b_i[i] = Bs[1][k][wn + i * 16 + lr];
-        b_s[i] = Bs[2][k][wn + i * 16 + lr];
-      }
-#pragma unroll
-      for (int i = 0; i < WT; ++i)
-#pragma unroll
-        for (int j = 0; j < WT; ++j) {
-          p1[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_r[i], b_r[j], p1[i][j], 0, 0, 0);
-          p2[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_i[i], b_i[j], p2[i][j], 0, 0, 0);
-          p3[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_s[i], b_s[j], p3[i][j], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < WT; ++i)
-#pragma unroll
-    for (int j = 0; j < WT; ++j) {
-      double cr[4], ci[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
+        const S x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
         cr[r] = (double)(x1 - x2);
         ci[r] = (double)(x3 - x1 - x2);
       }
-      store_tile<TC, false>(p, b, split, m0 + wm + i * 16, n0 + wn + j * 16, cr, ci);
+      store_tile<TC, F64>(p, b, split, m0 + wm + i * 16, n0 + wn + j * 16, cr, ci);
     }
+}
+
+// complex FP64 (v_mfma_f64_16x16x4_f64, FP64 accumulation)
+template <int TM, typename TA, typename TB, typename TC, bool TWO>
+__global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
+  zgemm_body<true, TM, TA, TB, TC, TWO>(p);
+}
+
+// complex FP32 (v_mfma_f32_16x16x4_f32, FP32 accumulation)
+template <int TM, typename TA, typename TB, typename TC, bool TWO>
+__global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
+  zgemm_body<false, TM, TA, TB, TC, TWO>(p);
 }
 
 // ------------------------------------------------------- split-K reduction
