@@ -74,78 +74,68 @@ __device__ __forceinline__ void stc(float2* p, double re, double im) {
 }
 
 // ------------------------------------------------------------ tile loaders
-// Each thread owns TM*BK/NT elements of the A tile (TM x BK) and as many of
-// the B tile (BK x TM).  fetch_* issues the tile's global loads into
-// registers (clamped addresses, no branches around the loads), put_* masks
-// and stores them to LDS as split planes S[plane][k][m|n]; the main loop
-// fetches tile k+1 while the MFMAs consume tile k.
+// Each thread owns TM*BK/NT elements of an operand's K-step tile (TM rows of
+// M (A) or N (B) by BK of K).  The row part of every element's address, its
+// place in the tile and its row bound are fixed for the whole K loop and set
+// up once (OpLoader::init); per K step, fetch issues the global loads into
+// registers and put masks them (and conjugates) into LDS as Re/Im planes
+// S[plane][k][m|n].  When K is a multiple of BK (KF) every step is full: the
+// k part of each address is a per-step scalar added to a pointer set up once,
+// with no clamp and no k mask; otherwise addresses are clamped and masked.
 template <int TM>
 struct Frag {
   static constexpr int N = (TM * BK) / NT;
   double re[N], im[N];
 };
 
-template <int TM, bool TWO, typename T>
-__device__ __forceinline__ void fetch_a(const GemmParams& p, const T* A, int m0, int k0, int kend, Frag<TM>& f) {
-  constexpr int BM = TM;
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int j = 0; j < (BM * BK) / NT; ++j) {
-    const int mm = p.a_mfast ? (tid % BM) : (tid / BK + j * (NT / BK));
-    const int kk = p.a_mfast ? (tid / BM + j * (NT / BM)) : (tid % BK);
-    const int gm = min(m0 + mm, p.M - 1), gk = min(k0 + kk, kend - 1);
-    ldc(A + loff<TWO>(p.am, gm) + loff<TWO>(p.ak, gk), f.re[j], f.im[j]);
-  }
-}
+template <int TM, bool TWO, bool KF, typename T>
+struct OpLoader {
+  static constexpr int NF = (TM * BK) / NT;
+  const T* row[NF];   // base + row part of the address (row clamped) [+ k part of kk when KF, single map]
+  int kk[NF];         // k within the step
+  int rr[NF];         // row within the tile
+  DevMap km;
 
-template <int TM, typename S, int NPL>
-__device__ __forceinline__ void put_a(const GemmParams& p, int m0, int k0, int kend, const Frag<TM>& f,
-                                      S (*As)[BK][TM + 1]) {
-  constexpr int BM = TM;
-  const int tid = threadIdx.x;
+  __device__ __forceinline__ void init(const T* base, const DevMap& rmap, const DevMap& kmap, int r0, int R,
+                                       int fast) {
+    const int tid = threadIdx.x;
+    km = kmap;
 #pragma unroll
-  for (int j = 0; j < (BM * BK) / NT; ++j) {
-    const int mm = p.a_mfast ? (tid % BM) : (tid / BK + j * (NT / BK));
-    const int kk = p.a_mfast ? (tid / BM + j * (NT / BM)) : (tid % BK);
-    const bool ok = (m0 + mm < p.M) && (k0 + kk < kend);
-    const double re = ok ? f.re[j] : 0.0;
-    const double im = ok ? (p.conjA ? -f.im[j] : f.im[j]) : 0.0;
-    As[0][kk][mm] = (S)re;
-    As[1][kk][mm] = (S)im;
-    if constexpr (NPL == 3) As[2][kk][mm] = (S)re + (S)im;
+    for (int j = 0; j < NF; ++j) {
+      const int r = fast ? (tid % TM) : (tid / BK + j * (NT / BK));
+      kk[j] = fast ? (tid / TM + j * (NT / TM)) : (tid % BK);
+      rr[j] = r;
+      row[j] = base + loff<TWO>(rmap, min(r0 + r, R - 1));
+      if (KF && !TWO) row[j] += (int64_t)kk[j] * km.s0;
+    }
   }
-}
-
-template <int TM, bool TWO, typename T>
-__device__ __forceinline__ void fetch_b(const GemmParams& p, const T* B, int n0, int k0, int kend, Frag<TM>& f) {
-  constexpr int BN = TM;
-  const int tid = threadIdx.x;
+  template <bool FULL>
+  __device__ __forceinline__ void fetch(int k0, int kend, Frag<TM>& f) const {
+    if constexpr (KF && !TWO) {
+      const int64_t ko = (int64_t)k0 * km.s0;
 #pragma unroll
-  for (int j = 0; j < (BN * BK) / NT; ++j) {
-    const int nn = p.b_nfast ? (tid % BN) : (tid / BK + j * (NT / BK));
-    const int kk = p.b_nfast ? (tid / BN + j * (NT / BN)) : (tid % BK);
-    const int gn = min(n0 + nn, p.N - 1), gk = min(k0 + kk, kend - 1);
-    ldc(B + loff<TWO>(p.bk, gk) + loff<TWO>(p.bn, gn), f.re[j], f.im[j]);
-  }
-}
-
-template <int TM, typename S, int NPL>
-__device__ __forceinline__ void put_b(const GemmParams& p, int n0, int k0, int kend, const Frag<TM>& f,
-                                      S (*Bs)[BK][TM + 1]) {
-  constexpr int BN = TM;
-  const int tid = threadIdx.x;
+      for (int j = 0; j < NF; ++j) ldc(row[j] + ko, f.re[j], f.im[j]);
+      return;
+    }
 #pragma unroll
-  for (int j = 0; j < (BN * BK) / NT; ++j) {
-    const int nn = p.b_nfast ? (tid % BN) : (tid / BK + j * (NT / BK));
-    const int kk = p.b_nfast ? (tid / BN + j * (NT / BN)) : (tid % BK);
-    const bool ok = (n0 + nn < p.N) && (k0 + kk < kend);
-    const double re = ok ? f.re[j] : 0.0;
-    const double im = ok ? (p.conjB ? -f.im[j] : f.im[j]) : 0.0;
-    Bs[0][kk][nn] = (S)re;
-    Bs[1][kk][nn] = (S)im;
-    if constexpr (NPL == 3) Bs[2][kk][nn] = (S)re + (S)im;
+    for (int j = 0; j < NF; ++j) {
+      const int gk = FULL ? k0 + kk[j] : min(k0 + kk[j], kend - 1);
+      ldc(row[j] + loff<TWO>(km, gk), f.re[j], f.im[j]);
+    }
   }
-}
+  // Rows past the operand's edge hold a clamped row's values: they feed only
+  // output rows (of A) or columns (of B) past the edge, which are never
+  // stored, so only k past the chunk needs a mask.
+  template <bool FULL, typename S>
+  __device__ __forceinline__ void put(int k0, int kend, int conj, const Frag<TM>& f, S (*Xs)[BK][TM + 1]) const {
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+      const bool ok = FULL || k0 + kk[j] < kend;
+      Xs[0][kk[j]][rr[j]] = (S)(ok ? f.re[j] : 0.0);
+      Xs[1][kk[j]][rr[j]] = (S)(ok ? (conj ? -f.im[j] : f.im[j]) : 0.0);
+    }
+  }
+};
 
 // --------------------------------------------------------------- epilogue
 // MFMA 16x16 output layouts (measured on gfx950 with a lane probe):
@@ -209,7 +199,7 @@ struct MfmaT<false> {
   }
 };
 
-template <bool F64, int TM, typename TA, typename TB, typename TC, bool TWO>
+template <bool F64, int TM, typename TA, typename TB, typename TC, bool TWO, bool KF>
 __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
   using S = typename MfmaT<F64>::S;
   using V = typename MfmaT<F64>::V;
@@ -238,26 +228,29 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
       p3[i][j] = V{0, 0, 0, 0};
     }
 
+  OpLoader<TM, TWO, KF, TA> la;
+  OpLoader<TM, TWO, KF, TB> lb;
+  la.init(A, p.am, p.ak, m0, p.M, p.a_mfast);
+  lb.init(B, p.bn, p.bk, n0, p.N, p.b_nfast);
+  auto stage = [&](int k0, Frag<TM>& fa, Frag<TM>& fb) {
+    la.template fetch<KF>(k0, kend, fa);
+    lb.template fetch<KF>(k0, kend, fb);
+  };
+  auto store = [&](int k0, const Frag<TM>& fa, const Frag<TM>& fb, int buf) {
+    la.template put<KF>(k0, kend, p.conjA, fa, As[buf]);
+    lb.template put<KF>(k0, kend, p.conjB, fb, Bs[buf]);
+  };
   Frag<TM> fa, fb;
-  fetch_a<TM, TWO>(p, A, m0, kbeg, kend, fa);
-  fetch_b<TM, TWO>(p, B, n0, kbeg, kend, fb);
-  put_a<TM, S, 2>(p, m0, kbeg, kend, fa, As[0]);
-  put_b<TM, S, 2>(p, n0, kbeg, kend, fb, Bs[0]);
-  if (nit > 1) {
-    fetch_a<TM, TWO>(p, A, m0, kbeg + BK, kend, fa);
-    fetch_b<TM, TWO>(p, B, n0, kbeg + BK, kend, fb);
-  }
+  stage(kbeg, fa, fb);
+  store(kbeg, fa, fb, 0);
+  if (nit > 1) stage(kbeg + BK, fa, fb);
   __syncthreads();
   for (int it = 0; it < nit; ++it) {
     const int cur = it & 1;
     if (it + 1 < nit) {
       const int k1 = kbeg + (it + 1) * BK;
-      put_a<TM, S, 2>(p, m0, k1, kend, fa, As[cur ^ 1]);
-      put_b<TM, S, 2>(p, n0, k1, kend, fb, Bs[cur ^ 1]);
-      if (it + 2 < nit) {
-        fetch_a<TM, TWO>(p, A, m0, k1 + BK, kend, fa);
-        fetch_b<TM, TWO>(p, B, n0, k1 + BK, kend, fb);
-      }
+      store(k1, fa, fb, cur ^ 1);
+      if (it + 2 < nit) stage(k1 + BK, fa, fb);
     }
 #pragma unroll
     for (int kq = 0; kq < BK / 4; ++kq) {
@@ -299,15 +292,15 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
 }
 
 // complex FP64 (v_mfma_f64_16x16x4_f64, FP64 accumulation)
-template <int TM, typename TA, typename TB, typename TC, bool TWO>
+template <int TM, typename TA, typename TB, typename TC, bool TWO, bool KF>
 __global__ __launch_bounds__(NT, 2) void zgemm_f64_kernel(GemmParams p) {
-  zgemm_body<true, TM, TA, TB, TC, TWO>(p);
+  zgemm_body<true, TM, TA, TB, TC, TWO, KF>(p);
 }
 
 // complex FP32 (v_mfma_f32_16x16x4_f32, FP32 accumulation)
-template <int TM, typename TA, typename TB, typename TC, bool TWO>
+template <int TM, typename TA, typename TB, typename TC, bool TWO, bool KF>
 __global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
-  zgemm_body<false, TM, TA, TB, TC, TWO>(p);
+  zgemm_body<false, TM, TA, TB, TC, TWO, KF>(p);
 }
 
 // ------------------------------------------------------- split-K reduction
@@ -404,33 +397,38 @@ size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus) {
   return (size_t)pl.splits * g.batch * g.M * g.N * sizeof(double2);
 }
 
+template <bool F64, int TM, typename TA, typename TB, typename TC>
+static void launch_kf(const GemmParams& p, bool two, bool kf, dim3 grid, hipStream_t s) {
+  // K a multiple of BK (every step full): the clamp- and mask-free loaders
+#define MPS_GEMM_LAUNCH(TWO_, KF_)                                                                    \
+  do {                                                                                            \
+    if constexpr (F64)                                                                            \
+      hipLaunchKernelGGL((zgemm_f64_kernel<TM, TA, TB, TC, TWO_, KF_>), grid, dim3(NT), 0, s, p);   \
+    else                                                                                          \
+      hipLaunchKernelGGL((zgemm_f32_3m_kernel<TM, TA, TB, TC, TWO_, KF_>), grid, dim3(NT), 0, s, p); \
+  } while (0)
+  if (two) {
+    if (kf) MPS_GEMM_LAUNCH(true, true); else MPS_GEMM_LAUNCH(true, false);
+  } else {
+    if (kf) MPS_GEMM_LAUNCH(false, true); else MPS_GEMM_LAUNCH(false, false);
+  }
+#undef MPS_GEMM_LAUNCH
+}
+
 template <typename TA, typename TB, typename TC>
 static void launch_typed(const GemmParams& p, Compute cmp, int tm, dim3 grid, hipStream_t s) {
   const bool two = p.am.n0 || p.ak.n0 || p.bk.n0 || p.bn.n0;
+  const bool kf = (p.K % BK) == 0;
   if (cmp == CMP_F64) {
-    if (tm == 32) {
-      if (two)
-        hipLaunchKernelGGL((zgemm_f64_kernel<32, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
-      else
-        hipLaunchKernelGGL((zgemm_f64_kernel<32, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
-    } else {
-      if (two)
-        hipLaunchKernelGGL((zgemm_f64_kernel<64, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
-      else
-        hipLaunchKernelGGL((zgemm_f64_kernel<64, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
-    }
+    if (tm == 32)
+      launch_kf<true, 32, TA, TB, TC>(p, two, kf, grid, s);
+    else
+      launch_kf<true, 64, TA, TB, TC>(p, two, kf, grid, s);
   } else {
-    if (tm == 32) {
-      if (two)
-        hipLaunchKernelGGL((zgemm_f32_3m_kernel<32, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
-      else
-        hipLaunchKernelGGL((zgemm_f32_3m_kernel<32, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
-    } else {
-      if (two)
-        hipLaunchKernelGGL((zgemm_f32_3m_kernel<64, TA, TB, TC, true>), grid, dim3(NT), 0, s, p);
-      else
-        hipLaunchKernelGGL((zgemm_f32_3m_kernel<64, TA, TB, TC, false>), grid, dim3(NT), 0, s, p);
-    }
+    if (tm == 32)
+      launch_kf<false, 32, TA, TB, TC>(p, two, kf, grid, s);
+    else
+      launch_kf<false, 64, TA, TB, TC>(p, two, kf, grid, s);
   }
 }
 
