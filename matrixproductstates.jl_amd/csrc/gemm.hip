@@ -142,33 +142,27 @@ struct OpLoader {
 //   v_mfma_f32_16x16x4_f32: lane l, element r -> row 4*(l/16) + r, col l%16
 //   v_mfma_f64_16x16x4_f64: lane l, element r -> row (l/16) + 4*r, col l%16
 // Operands in both: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k.
-template <typename TC, bool F64LAYOUT>
-__device__ __forceinline__ void store_tile(const GemmParams& p, int b, int split, int row0, int col0,
-                                           const double (&cr)[4], const double (&ci)[4]) {
-  const int lane = threadIdx.x & 63;
-  const int col = col0 + (lane & 15);
-  if (col >= p.N) return;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = F64LAYOUT ? row0 + (lane >> 4) + 4 * r : row0 + 4 * (lane >> 4) + r;
-    if (row >= p.M) continue;
-    if (p.splits > 1) {
-      double2* P = (double2*)p.P + ((int64_t)split * p.batch + b) * (int64_t)p.M * p.N;
-      P[(int64_t)col * p.M + row] = make_double2(cr[r], ci[r]);
-    } else {
-      TC* C = (TC*)p.C + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
-      double re = p.ar * cr[r] - p.ai * ci[r];
-      double im = p.ar * ci[r] + p.ai * cr[r];
-      if (p.br != 0.0 || p.bi != 0.0) {
-        double orr, oi;
-        ldc(C, orr, oi);
-        re += p.br * orr - p.bi * oi;
-        im += p.br * oi + p.bi * orr;
-      }
-      re += (row == col) ? p.dg : 0.0;
-      stc(C, re, im);
-    }
+// One output element (row < M, col < N): a split-K partial, or
+// alpha * acc + beta * C (+ dg on the diagonal) through the output maps.
+template <typename TC>
+__device__ __forceinline__ void store_elem(const GemmParams& p, int b, int split, int row, int col, double cr,
+                                           double ci) {
+  if (p.splits > 1) {
+    double2* P = (double2*)p.P + ((int64_t)split * p.batch + b) * (int64_t)p.M * p.N;
+    P[(int64_t)col * p.M + row] = make_double2(cr, ci);
+    return;
   }
+  TC* C = (TC*)p.C + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
+  double re = p.ar * cr - p.ai * ci;
+  double im = p.ar * ci + p.ai * cr;
+  if (p.br != 0.0 || p.bi != 0.0) {
+    double orr, oi;
+    ldc(C, orr, oi);
+    re += p.br * orr - p.bi * oi;
+    im += p.br * oi + p.bi * orr;
+  }
+  re += (row == col) ? p.dg : 0.0;
+  stc(C, re, im);
 }
 
 // ------------------------------------------------------------ MFMA kernels
@@ -204,8 +198,14 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
   using S = typename MfmaT<F64>::S;
   using V = typename MfmaT<F64>::V;
   constexpr int WT = TM / 32;   // 16x16 MFMA tiles per wave and dimension
-  __shared__ S As[2][2][BK][TM + 1];
-  __shared__ S Bs[2][2][BK][TM + 1];
+  constexpr int WB = TM / 2;    // a wave's output block is WB x WB
+  // LDS: the double-buffered operand tiles, then (after the K loop) each
+  // wave's output block, column major with a padded column stride
+  constexpr size_t TILE_BYTES = 2 * sizeof(S[2][2][BK][TM + 1]);
+  constexpr size_t OUT_BYTES = 4 * sizeof(double2) * WB * (WB + 1);
+  __shared__ __align__(16) char smem[TILE_BYTES > OUT_BYTES ? TILE_BYTES : OUT_BYTES];
+  auto As = reinterpret_cast<S(*)[2][BK][TM + 1]>(smem);
+  auto Bs = reinterpret_cast<S(*)[2][BK][TM + 1]>(smem + TILE_BYTES / 2);
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
   const int m0 = tm * TM, n0 = tn * TM;
@@ -276,19 +276,33 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
     }
     __syncthreads();
   }
+  // Epilogue through LDS (the loop's last barrier has retired the operand
+  // tiles): MFMA layouts (measured on gfx950 with a lane probe)
+  //   v_mfma_f64_16x16x4_f64: lane l, element r -> row (l/16) + 4r, col l%16
+  //   v_mfma_f32_16x16x4_f32: lane l, element r -> row 4(l/16) + r, col l%16
+  // go to the wave's block, then each store instruction writes whole column
+  // segments of WB rows (64 / WB columns per instruction).
+  double2* blk = reinterpret_cast<double2*>(smem) + wave * WB * (WB + 1);
 #pragma unroll
   for (int i = 0; i < WT; ++i)
 #pragma unroll
-    for (int j = 0; j < WT; ++j) {
-      double cr[4], ci[4];
+    for (int j = 0; j < WT; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const S x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
-        cr[r] = (double)(x1 - x2);
-        ci[r] = (double)(x3 - x1 - x2);
+        const int row = i * 16 + (F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+        const int col = j * 16 + (lane & 15);
+        blk[col * (WB + 1) + row] = make_double2((double)(x1 - x2), (double)(x3 - x1 - x2));
       }
-      store_tile<TC, F64>(p, b, split, m0 + wm + i * 16, n0 + wn + j * 16, cr, ci);
-    }
+  __syncthreads();
+  constexpr int CPI = 64 / WB;   // columns per store instruction
+#pragma unroll
+  for (int q = 0; q < WB / CPI; ++q) {
+    const int rl = lane % WB, cl = q * CPI + lane / WB;
+    const int row = m0 + wm + rl, col = n0 + wn + cl;
+    const double2 v = blk[cl * (WB + 1) + rl];
+    if (row < p.M && col < p.N) store_elem<TC>(p, b, split, row, col, v.x, v.y);
+  }
 }
 
 // complex FP64 (v_mfma_f64_16x16x4_f64, FP64 accumulation)
