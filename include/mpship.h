@@ -194,9 +194,12 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
 /* ------------------------------------------------ L3 time steppers (a16, a20) */
 /* RK4stp_apply_H_half! (timeevolution.jl:178-196).  psi0 == NULL selects the
  * 10-argument form (result overwrites psi); otherwise the 11-argument form.
- * Returns once the step is queued on the context's stream (after the first
- * two calls with a given set of handles, as one captured graph); later calls
- * on the same context, downloads and scalar-returning calls order after it. */
+ * Returns once the step is queued on the context's stream; later calls on the
+ * same context, downloads and scalar-returning calls order after it.  Small
+ * sites run, from the second call with a given set of handles on, as one
+ * captured graph; sites of 32768 elements or more run eagerly, their overlap
+ * terms on a second stream of the context that is joined back into the
+ * context's stream before the call returns. */
 int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
                             mps_state_t psi0, mps_state_t psi, mps_state_t psi1,
                             mps_state_t psi2, mps_state_t psi3, mps_env_t env1,
