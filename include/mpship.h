@@ -51,6 +51,23 @@ const char* mps_last_error(void);
 int mps_version(void);
 int mps_ctx_create(int device, mps_ctx_t* out);
 int mps_ctx_destroy(mps_ctx_t ctx);
+/* Lockstep batch (no counterpart in the reference; it serves ensembles of
+ * quantum trajectories of one system, spin_model_quantum_trajectory_rk4.jl:
+ * 115-209 run for many jump records at once).  A batched context holds nb
+ * copies of every state, MPO and environment created on it (copy_bytes of
+ * containers per copy, plus a workspace of ws_bytes per copy, reserved at
+ * creation), and every sweep, QR and norm call on it runs all copies with
+ * the same kernels in the same launches.  Uploads write every copy.  One copy
+ * is reached through a view on a companion context (same stream); entry
+ * points that return per-call host values (scalar products, local
+ * expectation values, SVD truncation) and the raw mps_dev_* calls refuse a
+ * batched context. */
+int mps_ctx_create_batch(int device, int nb, int64_t copy_bytes, int64_t ws_bytes, mps_ctx_t* out);
+/* Workspace per copy (ws_bytes above) for RK4 and sweep calls at bond
+ * dimension <= Dmax, local dimension <= d and MPO bond <= wmax. */
+int mps_batch_workspace_bytes(int device, int nb, int64_t Dmax, int64_t d, int64_t wmax, int64_t* out);
+/* A plain context on the batched context's stream, for views. */
+int mps_ctx_create_companion(mps_ctx_t batched, mps_ctx_t* out);
 /* Use an external HIP stream (hipStream_t passed as void*); NULL = own stream. */
 int mps_ctx_set_stream(mps_ctx_t ctx, void* stream);
 int mps_ctx_synchronize(mps_ctx_t ctx);
@@ -192,6 +209,15 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
                         double* err_trim, double* ent_ent);
 
 /* ------------------------------------------------ L3 time steppers (a16, a20) */
+/* Views of copy b of a batched container, on a companion context: they share
+ * the container's memory and must be destroyed before it. */
+int mps_state_view(mps_state_t s, int b, mps_ctx_t companion, mps_state_t* out);
+int mps_env_view(mps_env_t e, int b, mps_ctx_t companion, mps_env_t* out);
+int mps_mpo_view(mps_mpo_t m, int b, mps_ctx_t companion, mps_mpo_t* out);
+/* normalize_lo! (orthonormalisation.jl:97-103) of every copy of a batched
+ * state: norm2[b] (nb doubles) for copy b. */
+int mps_normalize_lo_batch(mps_state_t s, double* norm2);
+
 /* RK4stp_apply_H_half! (timeevolution.jl:178-196).  psi0 == NULL selects the
  * 10-argument form (result overwrites psi); otherwise the 11-argument form.
  * Returns once the step is queued on the context's stream; later calls on the

@@ -70,6 +70,26 @@ static int guard(F&& f) {
 
 static void set_device(Ctx* c) { HIPCHK(hipSetDevice(c->device)); }
 
+// Entry points that produce per-call host values or touch single buffers run
+// on plain contexts only; a batched context's copies go through views.
+static void no_batch(const Ctx* c) {
+  REQUIRE(c->bat.nb == 1, "not available on a batched context (use a view of one copy)");
+}
+
+// zero `bytes` at p in every lockstep copy of the context
+static void zero_copies(Ctx* c, void* p, size_t bytes) {
+  if (c->bat.nb == 1)
+    HIPCHK(hipMemsetAsync(p, 0, bytes, c->stream));
+  else
+    HIPCHK(hipMemset2DAsync(p, (size_t)c->bat.bs, 0, bytes, (size_t)c->bat.nb, c->stream));
+}
+
+// the same host data into every lockstep copy
+static void upload_copies(Ctx* c, void* dst, const void* host, size_t bytes) {
+  for (int b = 0; b < c->bat.nb; ++b)
+    HIPCHK(hipMemcpyAsync(boff((char*)dst, (int64_t)b * c->bat.bs), host, bytes, hipMemcpyHostToDevice, c->stream));
+}
+
 // Any change of device buffers or dims invalidates captured graphs.
 static void invalidate(mps_ctx_s* x) {
   if (x) x->clear_graphs();
@@ -101,6 +121,74 @@ int mps_ctx_create(int device, mps_ctx_t* out) {
       x->c.own_stream = true;
       HIPCHK(hipMalloc((void**)&x->c.dscal, 256 * sizeof(double)));
       HIPCHK(hipHostMalloc((void**)&x->c.hscal, 256 * sizeof(double), hipHostMallocDefault));
+    } catch (...) {
+      delete x;
+      throw;
+    }
+    *out = x;
+  });
+}
+
+int mps_ctx_create_batch(int device, int nb, int64_t copy_bytes, int64_t ws_bytes, mps_ctx_t* out) {
+  return guard([&] {
+    REQUIRE(out != nullptr && nb >= 1 && nb <= 65535, "invalid arguments");
+    REQUIRE(copy_bytes > 0 && ws_bytes >= 0, "invalid arguments");
+    auto* x = new mps_ctx_s();
+    try {
+      Ctx& c = x->c;
+      c.device = device;
+      HIPCHK(hipSetDevice(device));
+      hipDeviceProp_t prop;
+      HIPCHK(hipGetDeviceProperties(&prop, device));
+      c.num_cus = prop.multiProcessorCount;
+      HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+      c.own_stream = true;
+      // copy stride: scalars + workspace + the caller's containers, 256-byte aligned
+      const size_t ws = ((size_t)ws_bytes + 255) & ~(size_t)255;
+      const size_t stride = ((size_t)copy_bytes + ws + 4096 + 255) & ~(size_t)255;
+      c.bat.nb = nb;
+      c.bat.bs = (int64_t)stride;
+      if (hipMalloc((void**)&c.slab, stride * (size_t)nb) != hipSuccess) {
+        c.slab = nullptr;
+        throw Error(MPS_E_ALLOC, "batched context: slab allocation failed");
+      }
+      bool own = false;
+      c.dscal = (double*)c.alloc(256 * sizeof(double), own);
+      c.ws = (char*)c.alloc(ws, own);
+      c.ws_bytes = ws;
+      HIPCHK(hipHostMalloc((void**)&c.hscal, std::max(256, nb) * sizeof(double), hipHostMallocDefault));
+    } catch (...) {
+      delete x;
+      throw;
+    }
+    *out = x;
+  });
+}
+
+int mps_batch_workspace_bytes(int device, int nb, int64_t Dmax, int64_t d, int64_t wmax, int64_t* out) {
+  return guard([&] {
+    REQUIRE(out && nb >= 1 && Dmax >= 1 && d >= 1 && wmax >= 1, "invalid arguments");
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    // the largest plan of the batched calls: the RK4 step's (four overlap products)
+    *out = (int64_t)plan_bytes(Dmax, d, wmax, 4, 0, prop.multiProcessorCount, nb);
+  });
+}
+
+int mps_ctx_create_companion(mps_ctx_t batched, mps_ctx_t* out) {
+  return guard([&] {
+    REQUIRE(batched != nullptr && out != nullptr, "invalid arguments");
+    auto* x = new mps_ctx_s();
+    try {
+      Ctx& c = x->c;
+      c.device = batched->c.device;
+      HIPCHK(hipSetDevice(c.device));
+      c.num_cus = batched->c.num_cus;
+      c.stream = batched->c.stream;   // same stream: calls on views order with the batched calls
+      c.own_stream = false;
+      c.qr_tol = batched->c.qr_tol;
+      HIPCHK(hipMalloc((void**)&c.dscal, 256 * sizeof(double)));
+      HIPCHK(hipHostMalloc((void**)&c.hscal, 256 * sizeof(double), hipHostMallocDefault));
     } catch (...) {
       delete x;
       throw;
@@ -217,12 +305,14 @@ int mps_state_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* dims
       tot += (size_t)s.dims[j] * s.d[j] * s.dims[j + 1];
     }
     s.total = tot;
-    if (hipMalloc((void**)&s.buf, tot * sizeof(double2)) != hipSuccess) {
+    try {
+      s.buf = (double2*)ctx->c.alloc(tot * sizeof(double2), s.own);
+    } catch (...) {
       s.buf = nullptr;
       delete x;
       throw Error(MPS_E_ALLOC, "state allocation failed");
     }
-    HIPCHK(hipMemsetAsync(s.buf, 0, tot * sizeof(double2), ctx->c.stream));
+    zero_copies(&ctx->c, s.buf, tot * sizeof(double2));
     ctx->c.sync();
     *out = x;
   });
@@ -247,8 +337,7 @@ int mps_state_upload_site(mps_state_t s, int site, const double* host, int64_t d
     REQUIRE(site >= 0 && site < st.N, "site out of range");
     REQUIRE(dl == st.Dl(site) && d == st.d[site] && dr == st.Dr(site), "site shape mismatch");
     set_device(st.ctx);
-    HIPCHK(hipMemcpyAsync(st.site(site), host, dl * d * dr * sizeof(double2), hipMemcpyHostToDevice,
-                          st.ctx->stream));
+    upload_copies(st.ctx, st.site(site), host, dl * d * dr * sizeof(double2));
     st.ctx->sync();
   });
 }
@@ -274,11 +363,13 @@ int mps_state_copy(mps_state_t dst, mps_state_t src) {
       REQUIRE(b.dims[j] <= a.cap_l[j] && b.dims[j + 1] <= a.cap_r[j], "destination capacity too small");
     }
     set_device(a.ctx);
+    REQUIRE(a.ctx->bat.nb == b.ctx->bat.nb && a.ctx->bat.bs == b.ctx->bat.bs,
+            "state copy between a batched and a plain container");
     // the source may still be written by work queued on another context's stream
     if (b.ctx != a.ctx) b.ctx->sync();
     bool dims_changed = a.dims != b.dims;
     a.dims = b.dims;
-    for (int j = 0; j < a.N; ++j) launch_copy_c(a.site(j), b.site(j), b.size(j), a.ctx->stream);
+    for (int j = 0; j < a.N; ++j) launch_copy_c(a.site(j), b.site(j), b.size(j), a.ctx->stream, a.ctx->bat);
     if (dims_changed) invalidate(dst->owner);
   });
 }
@@ -297,8 +388,7 @@ int mps_state_product(mps_state_t s, const double* states) {
         for (int k = 0; k < j; ++k) off += st.d[k];
         h[(size_t)(a * Dl)] = make_double2(states[2 * (off + a)], states[2 * (off + a) + 1]);
       }
-      HIPCHK(hipMemcpyAsync(st.site(j), h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice,
-                            st.ctx->stream));
+      upload_copies(st.ctx, st.site(j), h.data(), h.size() * sizeof(double2));
       st.ctx->sync();
     }
     size_t before = st.ctx->ws_bytes;
@@ -328,12 +418,14 @@ int mps_mpo_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* wdims,
       tot += (size_t)o.w[j] * o.d[j] * o.w[j + 1] * o.d[j];
     }
     o.total = tot;
-    if (hipMalloc((void**)&o.buf, tot * sizeof(double2)) != hipSuccess) {
+    try {
+      o.buf = (double2*)ctx->c.alloc(tot * sizeof(double2), o.own);
+    } catch (...) {
       o.buf = nullptr;
       delete x;
       throw Error(MPS_E_ALLOC, "MPO allocation failed");
     }
-    HIPCHK(hipMemsetAsync(o.buf, 0, tot * sizeof(double2), ctx->c.stream));
+    zero_copies(&ctx->c, o.buf, tot * sizeof(double2));
     ctx->c.sync();
     *out = x;
   });
@@ -354,8 +446,7 @@ int mps_mpo_upload_site(mps_mpo_t o, int site, const double* host, int64_t wl, i
     REQUIRE(site >= 0 && site < m.N, "site out of range");
     REQUIRE(wl == m.wl(site) && d == m.d[site] && wr == m.wr(site), "MPO site shape mismatch");
     set_device(m.ctx);
-    HIPCHK(hipMemcpyAsync(m.site(site), host, wl * d * wr * d * sizeof(double2), hipMemcpyHostToDevice,
-                          m.ctx->stream));
+    upload_copies(m.ctx, m.site(site), host, wl * d * wr * d * sizeof(double2));
     m.ctx->sync();
   });
 }
@@ -368,10 +459,12 @@ int mps_mpo_set_block(mps_mpo_t o, int site, int64_t wl_idx, int64_t wr_idx, con
     REQUIRE(wl_idx >= 0 && wl_idx < wl && wr_idx >= 0 && wr_idx < wr, "MPO block index out of range");
     set_device(m.ctx);
     // O[wl_idx, s, wr_idx, t] at wl_idx + wl*s + wl*d*wr_idx + wl*d*wr*t
-    for (int64_t t = 0; t < d; ++t)
-      HIPCHK(hipMemcpy2DAsync(m.site(site) + wl_idx + wl * d * wr_idx + wl * d * wr * t,
-                              wl * sizeof(double2), op + 2 * d * t, sizeof(double2), sizeof(double2), d,
-                              hipMemcpyHostToDevice, m.ctx->stream));
+    for (int b = 0; b < m.ctx->bat.nb; ++b)
+      for (int64_t t = 0; t < d; ++t)
+        HIPCHK(hipMemcpy2DAsync(boff(m.site(site), (int64_t)b * m.ctx->bat.bs) + wl_idx + wl * d * wr_idx +
+                                    wl * d * wr * t,
+                                wl * sizeof(double2), op + 2 * d * t, sizeof(double2), sizeof(double2), d,
+                                hipMemcpyHostToDevice, m.ctx->stream));
     m.ctx->sync();
   });
 }
@@ -404,15 +497,17 @@ static int env_create(mps_ctx_t ctx, int N, const int64_t* dims1, const int64_t*
       tot += (size_t)big * e.wd[j] * big;
     }
     e.total = tot;
-    if (hipMalloc((void**)&e.buf, tot * sizeof(double2)) != hipSuccess) {
+    try {
+      e.buf = (double2*)ctx->c.alloc(tot * sizeof(double2), e.own);
+    } catch (...) {
       e.buf = nullptr;
       delete x;
       throw Error(MPS_E_ALLOC, "environment allocation failed");
     }
-    HIPCHK(hipMemsetAsync(e.buf, 0, tot * sizeof(double2), ctx->c.stream));
+    zero_copies(&ctx->c, e.buf, tot * sizeof(double2));
     // build_env: ones at both boundaries
-    launch_set_scalar(e.slot(0), 1.0, 0.0, ctx->c.stream);
-    launch_set_scalar(e.slot(N), 1.0, 0.0, ctx->c.stream);
+    launch_set_scalar(e.slot(0), 1.0, 0.0, ctx->c.stream, ctx->c.bat);
+    launch_set_scalar(e.slot(N), 1.0, 0.0, ctx->c.stream, ctx->c.bat);
     ctx->c.sync();
     *out = x;
   });
@@ -455,6 +550,7 @@ int mps_env_download_slot(mps_env_t e, int slot, double* host, int64_t n) {
 // ------------------------------------------------- raw L1 contractions
 #define RAW_PLAN(ctx, Dm, dm, wm)            \
   set_device(&ctx->c);                       \
+  no_batch(&ctx->c);                         \
   size_t before_ = ctx->c.ws_bytes;          \
   Plan p(&ctx->c, (Dm), (dm), (wm), 0);      \
   track_ws(ctx, before_);
@@ -640,7 +736,7 @@ int mps_leftorthonormalize_qr(mps_state_t s, double* norm2) {
     State& st = s->s;
     STATE_PLAN(st, s->owner, 0);
     left_orthogonalize(p, &st, 0);
-    launch_scale_rsqrt(st.site(st.N - 1), st.size(st.N - 1), st.ctx->dscal, st.ctx->stream);
+    launch_scale_rsqrt(st.site(st.N - 1), st.size(st.N - 1), st.ctx->dscal, st.ctx->stream, st.ctx->bat);
     const double no = read_scalar(st.ctx, 0);
     if (norm2) *norm2 = no;
   });
@@ -661,7 +757,7 @@ int mps_rightorthonormalize_qr(mps_state_t s, double* norm2) {
     State& st = s->s;
     STATE_PLAN(st, s->owner, 0);
     right_orthogonalize(p, &st, 0);
-    launch_scale_rsqrt(st.site(0), st.size(0), st.ctx->dscal, st.ctx->stream);
+    launch_scale_rsqrt(st.site(0), st.size(0), st.ctx->dscal, st.ctx->stream, st.ctx->bat);
     const double no = read_scalar(st.ctx, 0);
     if (norm2) *norm2 = no;
   });
@@ -672,8 +768,8 @@ int mps_normalize_lo(mps_state_t s, double* norm2) {
     State& st = s->s;
     set_device(st.ctx);
     const int n = st.N - 1;
-    launch_sum_abs2(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream);
-    launch_scale_rsqrt(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream);
+    launch_sum_abs2(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream, st.ctx->bat);
+    launch_scale_rsqrt(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream, st.ctx->bat);
     const double no = read_scalar(st.ctx, 0);
     if (norm2) *norm2 = no;
   });
@@ -684,9 +780,101 @@ int mps_norm_lo(mps_state_t s, double* norm2) {
     State& st = s->s;
     set_device(st.ctx);
     const int n = st.N - 1;
-    launch_sum_abs2(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream);
+    launch_sum_abs2(st.site(n), st.size(n), st.ctx->dscal, st.ctx->stream, st.ctx->bat);
     const double no = read_scalar(st.ctx, 0);
     if (norm2) *norm2 = no;
+  });
+}
+
+// ------------------------------------------------------------- lockstep views
+// A view of copy b of a batched container, owned by a companion context on
+// the same stream: every plain entry point works on it.  It shares the
+// container's memory and must not outlive it.
+static void check_view(const Ctx* batched, const Ctx* comp, int b) {
+  REQUIRE(batched->bat.nb > 1 && comp->bat.nb == 1, "views are of batched containers, on a plain context");
+  REQUIRE(comp->stream == batched->stream, "the view's context must share the batched context's stream");
+  REQUIRE(b >= 0 && b < batched->bat.nb, "copy index out of range");
+}
+
+int mps_state_view(mps_state_t s, int b, mps_ctx_t comp, mps_state_t* out) {
+  return guard([&] {
+    REQUIRE(s && comp && out, "invalid arguments");
+    check_view(s->s.ctx, &comp->c, b);
+    auto* x = new mps_state_s();
+    x->owner = comp;
+    x->s = State();
+    State& v = x->s;
+    const State& o = s->s;
+    v.ctx = &comp->c;
+    v.N = o.N;
+    v.d = o.d;
+    v.dims = o.dims;
+    v.cap_l = o.cap_l;
+    v.cap_r = o.cap_r;
+    v.off = o.off;
+    v.total = o.total;
+    v.buf = boff(o.buf, (int64_t)b * o.ctx->bat.bs);
+    v.own = false;
+    *out = x;
+  });
+}
+
+int mps_env_view(mps_env_t e, int b, mps_ctx_t comp, mps_env_t* out) {
+  return guard([&] {
+    REQUIRE(e && comp && out, "invalid arguments");
+    check_view(e->e.ctx, &comp->c, b);
+    auto* x = new mps_env_s();
+    x->owner = comp;
+    Env& v = x->e;
+    const Env& o = e->e;
+    v.ctx = &comp->c;
+    v.N = o.N;
+    v.op = o.op;
+    v.d1 = o.d1;
+    v.wd = o.wd;
+    v.d2 = o.d2;
+    v.off = o.off;
+    v.total = o.total;
+    v.buf = boff(o.buf, (int64_t)b * o.ctx->bat.bs);
+    v.own = false;
+    *out = x;
+  });
+}
+
+int mps_mpo_view(mps_mpo_t m, int b, mps_ctx_t comp, mps_mpo_t* out) {
+  return guard([&] {
+    REQUIRE(m && comp && out, "invalid arguments");
+    check_view(m->o.ctx, &comp->c, b);
+    auto* x = new mps_mpo_s();
+    x->owner = comp;
+    MPO& v = x->o;
+    const MPO& o = m->o;
+    v.ctx = &comp->c;
+    v.N = o.N;
+    v.d = o.d;
+    v.w = o.w;
+    v.off = o.off;
+    v.total = o.total;
+    v.buf = boff(o.buf, (int64_t)b * o.ctx->bat.bs);
+    v.own = false;
+    *out = x;
+  });
+}
+
+// normalize_lo! of every copy of a batched state: norm2[b] for copy b
+int mps_normalize_lo_batch(mps_state_t s, double* norm2) {
+  return guard([&] {
+    REQUIRE(s && norm2, "invalid arguments");
+    State& st = s->s;
+    Ctx* c = st.ctx;
+    set_device(c);
+    const int n = st.N - 1;
+    launch_sum_abs2(st.site(n), st.size(n), c->dscal, c->stream, c->bat);
+    launch_scale_rsqrt(st.site(n), st.size(n), c->dscal, c->stream, c->bat);
+    HIPCHK(hipMemcpy2DAsync(c->hscal, sizeof(double), c->dscal, (size_t)c->bat.bs, sizeof(double),
+                            (size_t)c->bat.nb, hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    for (int b = 0; b < c->bat.nb; ++b) norm2[b] = c->hscal[b];
   });
 }
 
@@ -799,6 +987,7 @@ int mps_compress_var_apply_H(mps_state_t A0, mps_env_t envop, mps_state_t Ain, m
 int mps_compress_var(mps_state_t A0, mps_state_t A, mps_env_t env, int maxsweeps) {
   return guard([&] {
     check_states({&A0->s, &A->s});
+    no_batch(A0->s.ctx);
     check_ovl_env(env->e, A0->s);
     REQUIRE(maxsweeps >= 0, "maxsweeps >= 0");
     Ctx* c = A0->s.ctx;
@@ -835,6 +1024,7 @@ int mps_compress_sum_var(mps_state_t A0out, mps_state_t Ain, int numv, const dou
                          const mps_state_t* A, int maxsweeps) {
   return guard([&] {
     REQUIRE(numv >= 1 && A && coef, "numv >= 1 states and coefficients required");
+    no_batch(Ain->s.ctx);
     check_states({&A0out->s, &Ain->s});
     std::vector<State*> As(numv);
     std::vector<cd> cf(numv);
@@ -865,7 +1055,7 @@ int mps_compress_sum_var(mps_state_t A0out, mps_state_t Ain, int numv, const dou
       compress_sum_var(p, &out->s, &Ain->s, numv, cf.data(), As.data(), maxsweeps, FL);
       if (alias) {
         A0out->s.dims = out->s.dims;
-        for (int j = 0; j < out->s.N; ++j) launch_copy_c(A0out->s.site(j), out->s.site(j), out->s.size(j), c->stream);
+        for (int j = 0; j < out->s.N; ++j) launch_copy_c(A0out->s.site(j), out->s.site(j), out->s.size(j), c->stream, c->bat);
       }
       c->sync();
     } catch (...) {
@@ -911,6 +1101,7 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
                         double* ent_ent) {
   return guard([&] {
     State& st = s->s;
+    no_batch(st.ctx);
     REQUIRE(dmax >= 1, "compressionSVD!: dmax >= 1 required");
     REQUIRE(st.dims[0] == 1 && st.dims[st.N] == 1, "compressionSVD!: open boundary bonds must be 1");
     // small device buffers: sigma | scale | perm | flag, in the plan's extra buffer
@@ -929,7 +1120,7 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
     int* perm_d = (int*)(scl_d + nmax);
     int* flag_d = perm_d + nmax;
     int* flag_h = (int*)c->hscal;
-    launch_set_scalar(p.AL, 1.0, 0.0, c->stream);   // r = 1 (1 x 1)
+    launch_set_scalar(p.AL, 1.0, 0.0, c->stream, c->bat);   // r = 1 (1 x 1)
     i64 dnew = 1;
     std::vector<double> sig;
     for (int i = st.N - 1; i >= 0; --i) {
@@ -1076,6 +1267,7 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
 // ------------------------------------------------------------ scalar products
 static void scal_common(mps_state_t A, mps_state_t B, int mode, double* out) {
   check_states({&A->s, &B->s});
+  no_batch(A->s.ctx);
   Ctx* c = A->s.ctx;
   set_device(c);
   size_t before = c->ws_bytes;
@@ -1099,6 +1291,7 @@ int mps_scal_prod_no_conj(mps_state_t A, mps_state_t B, double* out) {
 int mps_scal_op_prod(mps_state_t A, mps_mpo_t O, mps_state_t B, double* out) {
   return guard([&] {
     check_states({&A->s, &B->s});
+    no_batch(A->s.ctx);
     check_mpo(O->o, A->s);
     Ctx* c = A->s.ctx;
     set_device(c);
@@ -1117,6 +1310,7 @@ int mps_scal_op_prod(mps_state_t A, mps_mpo_t O, mps_state_t B, double* out) {
 int mps_measure_local(mps_state_t A, const double* op, double* out) {
   return guard([&] {
     State& s = A->s;
+    no_batch(s.ctx);
     Ctx* c = s.ctx;
     set_device(c);
     size_t before = c->ws_bytes;
@@ -1128,7 +1322,7 @@ int mps_measure_local(mps_state_t A, const double* op, double* out) {
     double2* Ms = (double2*)p.extra;
     double2* cur = (double2*)p.envA;
     double2* nxt = (double2*)p.envB;
-    launch_set_scalar(cur, 1.0, 0.0, c->stream);
+    launch_set_scalar(cur, 1.0, 0.0, c->stream, c->bat);
     for (int n = s.N - 1; n >= 0; --n) {
       const i64 Dl = s.Dl(n), d = s.d[n], Dr = s.Dr(n);
       // Aenv = prod_LR(A[n], env)
@@ -1157,6 +1351,7 @@ int mps_measure_local(mps_state_t A, const double* op, double* out) {
 int mps_apply_site_operator(mps_state_t A, int site, const double* op) {
   return guard([&] {
     State& s = A->s;
+    no_batch(s.ctx);
     REQUIRE(site >= 0 && site < s.N, "site out of range");
     Ctx* c = s.ctx;
     set_device(c);
@@ -1175,7 +1370,7 @@ int mps_apply_site_operator(mps_state_t A, int site, const double* op) {
     g.B = p.extra; g.bk = im1(d); g.bn = im1(1);
     g.C = p.tmp; g.cm = im1(1); g.cn = im1(Dl); g.c_bs = Dl * d;
     run_gemm(c, g, CMP_F64, p.splitk, p.splitk_bytes);
-    launch_copy_c(s.site(site), p.tmp, Dl * d * Dr, c->stream);
+    launch_copy_c(s.site(site), p.tmp, Dl * d * Dr, c->stream, c->bat);
     c->sync();
   });
 }
@@ -1247,6 +1442,7 @@ int mps_arnoldi_expm(mps_mpo_t A, int numv, const mps_state_t* rhov, mps_env_t /
                      double* coef_out) {
   return guard([&] {
     REQUIRE(numv >= 1 && rhov, "numv >= 1 Krylov states required");
+    no_batch(A->o.ctx);
     for (int l = 0; l < numv; ++l) check_states({&rhov[0]->s, &rhov[l]->s});
     check_mpo(A->o, rhov[0]->s);
     check_op_env(envop->e, rhov[0]->s);

@@ -41,6 +41,24 @@ inline IdxMap im2(int64_t n0, int64_t s0, int64_t s1) { return IdxMap{(int32_t)n
 
 enum DType { F64 = 0, F32 = 1 };   // storage of complex elements: double2 / float2
 
+// A lockstep batch: nb copies of every buffer of a context, copy b at byte
+// offset b * bs from copy 0 (all of a batched context's buffers live in one
+// slab with that stride).  Launches through a batched context run every
+// kernel for all copies at once (an extra grid dimension), with each copy's
+// pointers offset by its stride.  The default is one copy.
+struct Bat {
+  int nb = 1;
+  int64_t bs = 0;
+};
+template <typename T>
+__host__ __device__ inline T* boff(T* p, int64_t bytes) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(p) + bytes);
+}
+template <typename T>
+__host__ __device__ inline const T* boff(const T* p, int64_t bytes) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + bytes);
+}
+
 // C[m,n] = alpha * sum_k opA(A)[m,k] opB(B)[k,n] + beta * C[m,n], batched.
 struct Gemm {
   int64_t M = 0, N = 0, K = 0, batch = 1;
@@ -49,6 +67,7 @@ struct Gemm {
   void* C = nullptr; IdxMap cm{}, cn{}; int64_t c_bs = 0; DType tc = F64;
   double alpha_re = 1, alpha_im = 0, beta_re = 0, beta_im = 0;
   double diag_re = 0;   // added to C[i][i] after alpha/beta (Q = I - V W)
+  Bat bat;              // lockstep copies (A, B, C and the split-K partials all offset)
 };
 
 enum Compute { CMP_F64 = 0, CMP_F32_3M = 1 };
@@ -66,7 +85,7 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
 // num_cus).  R_out (optional): n x n column-major upper triangle.
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus);
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
-               void* R_out, hipStream_t s);
+               void* R_out, hipStream_t s, Bat bat = Bat());
 
 // ------------------------------------------------------- device helpers
 // DPP move of a double (two 32-bit halves), bound_ctrl: out-of-row sources read 0
@@ -89,11 +108,11 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 }
 
 // small kernels (misc.hip)
-void launch_set_scalar(void* p, double re, double im, hipStream_t s);
-void launch_fill_zero(void* p, int64_t n_complex, hipStream_t s);
-void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s);
-void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s);
-void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s);
+void launch_set_scalar(void* p, double re, double im, hipStream_t s, Bat bat = Bat());
+void launch_fill_zero(void* p, int64_t n_complex, hipStream_t s, Bat bat = Bat());
+void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s, Bat bat = Bat());
+void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s, Bat bat = Bat());
+void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s, Bat bat = Bat());
 void launch_convert(void* dst, DType td, const void* src, DType ts, int64_t n, hipStream_t s);
 void launch_trace_small(const void* M, int64_t d, void* out, hipStream_t s);
 // one-sided Jacobi SVD (svd.hip)

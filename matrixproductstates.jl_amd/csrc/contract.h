@@ -18,6 +18,15 @@ struct Ctx {
   // workspace arena (grow only, outside graph capture)
   char* ws = nullptr;
   size_t ws_bytes = 0;
+  // Lockstep batch: a batched context holds bat.nb copies of every buffer it
+  // allocates (states, MPOs, environments, workspace, device scalars) in one
+  // slab, copy b at byte offset b * bat.bs, and runs every launch for all
+  // copies.  Copy 0 is carved from [slab, slab + bat.bs) by a bump allocator.
+  Bat bat;
+  char* slab = nullptr;
+  size_t slab_used = 0;
+  // device memory for a container: from the slab (owned = false) or hipMalloc
+  void* alloc(size_t bytes, bool& owned);
   // second stream for GEMM chains independent of the main chain (forked
   // from and joined back into `stream` within one call; default priority)
   hipStream_t side = nullptr;
@@ -53,6 +62,7 @@ struct State {
   std::vector<i64> d, dims, cap_l, cap_r, off;
   double2* buf = nullptr;
   size_t total = 0;
+  bool own = true;   // buf from hipMalloc (false: a batched context's slab, or a view)
   ~State();
   double2* site(int j) const { return buf + off[j]; }
   i64 Dl(int j) const { return dims[j]; }
@@ -68,6 +78,7 @@ struct MPO {
   std::vector<i64> d, w, off;   // w: N+1 bond dims
   double2* buf = nullptr;
   size_t total = 0;
+  bool own = true;
   ~MPO();
   double2* site(int j) const { return buf + off[j]; }
   i64 wl(int j) const { return w[j]; }
@@ -82,6 +93,7 @@ struct Env {
   std::vector<i64> d1, wd, d2, off;   // N+1 slots
   double2* buf = nullptr;
   size_t total = 0;
+  bool own = true;
   ~Env();
   double2* slot(int j) const { return buf + off[j]; }
 };

@@ -38,20 +38,41 @@ void Ctx::ensure_side() {
   HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 }
 
+void* Ctx::alloc(size_t bytes, bool& owned) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (slab != nullptr) {
+    REQUIRE(slab_used + bytes <= (size_t)bat.bs, "batched context: slab too small for this allocation");
+    void* p = slab + slab_used;
+    slab_used += bytes;
+    owned = false;
+    return p;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) throw Error(-3, "device allocation failed");
+  owned = true;
+  return p;
+}
+
 Ctx::~Ctx() {
   if (side) (void)hipStreamSynchronize(side);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (side) (void)hipStreamDestroy(side);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
-  if (ws) (void)hipFree(ws);
-  if (dscal) (void)hipFree(dscal);
+  if (slab) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(slab);   // holds the workspace and the device scalars too
+  } else {
+    if (ws) (void)hipFree(ws);
+    if (dscal) (void)hipFree(dscal);
+  }
   if (hscal) (void)hipHostFree(hscal);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
 void Ctx::ensure_ws(size_t bytes) {
   if (bytes <= ws_bytes) return;
+  REQUIRE(slab == nullptr, "batched context: workspace reserved at creation is too small for this call");
   REQUIRE(!capturing, "workspace growth during graph capture");
   HIPCHK(hipStreamSynchronize(stream));
   if (side) HIPCHK(hipStreamSynchronize(side));
@@ -79,13 +100,13 @@ void Scratch::commit() {
 }
 
 State::~State() {
-  if (buf) (void)hipFree(buf);
+  if (buf && own) (void)hipFree(buf);
 }
 MPO::~MPO() {
-  if (buf) (void)hipFree(buf);
+  if (buf && own) (void)hipFree(buf);
 }
 Env::~Env() {
-  if (buf) (void)hipFree(buf);
+  if (buf && own) (void)hipFree(buf);
 }
 
 i64 State::max_site() const {
@@ -108,6 +129,7 @@ i64 MPO::max_w() const {
 static double esize(DType t) { return t == F64 ? 16.0 : 8.0; }
 
 void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s) {
+  g.bat = c->bat;
   if (!c->profiling || c->capturing) {
     launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, s ? s : c->stream);
     return;

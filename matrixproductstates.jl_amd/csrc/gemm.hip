@@ -43,6 +43,7 @@ struct GemmParams {
   double dg;      // added to the real part of every diagonal element C[i][i]
   void* P;        // split-K partials: [(split*batch + b)][n][m]
   int batch;
+  int64_t bst;    // byte stride of the lockstep copies (blockIdx.z = copy * batch + b)
 };
 
 __device__ __forceinline__ int64_t moff(const DevMap& mp, int i) {
@@ -145,14 +146,14 @@ struct OpLoader {
 // One output element (row < M, col < N): a split-K partial, or
 // alpha * acc + beta * C (+ dg on the diagonal) through the output maps.
 template <typename TC>
-__device__ __forceinline__ void store_elem(const GemmParams& p, int b, int split, int row, int col, double cr,
-                                           double ci) {
+__device__ __forceinline__ void store_elem(const GemmParams& p, int b, int64_t bo, int split, int row, int col,
+                                           double cr, double ci) {
   if (p.splits > 1) {
-    double2* P = (double2*)p.P + ((int64_t)split * p.batch + b) * (int64_t)p.M * p.N;
+    double2* P = boff((double2*)p.P, bo) + ((int64_t)split * p.batch + b) * (int64_t)p.M * p.N;
     P[(int64_t)col * p.M + row] = make_double2(cr, ci);
     return;
   }
-  TC* C = (TC*)p.C + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
+  TC* C = boff((TC*)p.C, bo) + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
   double re = p.ar * cr - p.ai * ci;
   double im = p.ar * ci + p.ai * cr;
   if (p.br != 0.0 || p.bi != 0.0) {
@@ -206,14 +207,15 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
   __shared__ __align__(16) char smem[TILE_BYTES > OUT_BYTES ? TILE_BYTES : OUT_BYTES];
   auto As = reinterpret_cast<S(*)[2][BK][TM + 1]>(smem);
   auto Bs = reinterpret_cast<S(*)[2][BK][TM + 1]>(smem + TILE_BYTES / 2);
-  const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z;
+  const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z % p.batch;
+  const int64_t bo = (int64_t)(blockIdx.z / p.batch) * p.bst;   // lockstep copy
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
   const int m0 = tm * TM, n0 = tn * TM;
   const int kbeg = split * p.kchunk;
   const int kend = min(p.K, kbeg + p.kchunk);
   const int nit = (kend - kbeg + BK - 1) / BK;
-  const TA* A = (const TA*)p.A + (int64_t)b * p.a_bs;
-  const TB* B = (const TB*)p.B + (int64_t)b * p.b_bs;
+  const TA* A = boff((const TA*)p.A, bo) + (int64_t)b * p.a_bs;
+  const TB* B = boff((const TB*)p.B, bo) + (int64_t)b * p.b_bs;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * (TM / 2), wn = (wave >> 1) * (TM / 2);
   const int lr = lane & 15, lk = lane >> 4;
@@ -301,7 +303,7 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
     const int rl = lane % WB, cl = q * CPI + lane / WB;
     const int row = m0 + wm + rl, col = n0 + wn + cl;
     const double2 v = blk[cl * (WB + 1) + rl];
-    if (row < p.M && col < p.N) store_elem<TC>(p, b, split, row, col, v.x, v.y);
+    if (row < p.M && col < p.N) store_elem<TC>(p, b, bo, split, row, col, v.x, v.y);
   }
 }
 
@@ -320,6 +322,7 @@ __global__ __launch_bounds__(NT, 2) void zgemm_f32_3m_kernel(GemmParams p) {
 // ------------------------------------------------------- split-K reduction
 template <typename TC>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmParams p) {
+  const int64_t bo = (int64_t)blockIdx.y * p.bst;   // lockstep copy
   const int64_t MN = (int64_t)p.M * p.N;
   const int64_t total = MN * p.batch;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmParams p) {
     const int col = (int)(e / p.M), row = (int)(e - (int64_t)col * p.M);
     // partials summed in split order; eight loads in flight per batch
     double sr = 0.0, si = 0.0;
-    const double2* P = (const double2*)p.P + (int64_t)b * MN + e;
+    const double2* P = boff((const double2*)p.P, bo) + (int64_t)b * MN + e;
     const int64_t ss = (int64_t)p.batch * MN;
     int s = 0;
     for (; s + 8 <= p.splits; s += 8) {
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmParams p) {
       sr += v.x;
       si += v.y;
     }
-    TC* C = (TC*)p.C + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
+    TC* C = boff((TC*)p.C, bo) + (int64_t)b * p.c_bs + moff(p.cm, row) + moff(p.cn, col);
     double re = p.ar * sr - p.ai * si;
     double im = p.ar * si + p.ai * sr;
     if (p.br != 0.0 || p.bi != 0.0) {
@@ -373,8 +376,9 @@ struct GemmPlan {
 static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   (void)cmp;   // one plan serves the FP64 and FP32 kernels
   GemmPlan pl;
-  // 64 tiles unless they would leave more than half of the CUs idle
-  const int64_t t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64) * g.batch;
+  // 64 tiles unless they would leave more than half of the CUs idle (every
+  // lockstep copy is a block of the grid too)
+  const int64_t t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64) * g.batch * g.bat.nb;
   pl.tm = (2 * t64 < num_cus) ? 32 : 64;
   pl.tiles_m = (int)((g.M + pl.tm - 1) / pl.tm);
   pl.tiles_n = (int)((g.N + pl.tm - 1) / pl.tm);
@@ -384,7 +388,7 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   // over its partials pass); a grid with a tile for every CU is not split.
   // Chunks are at least 32 deep for 64 tiles and 128 deep for 32 tiles (a
   // quarter of the MFMA work per K step).
-  const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch;
+  const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch * g.bat.nb;
   const int64_t target = 4LL * num_cus;
   const int64_t min_chunk = (pl.tm == 64) ? 2 * BK : 8 * BK;
   int splits = 1;
@@ -404,6 +408,7 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   return pl;
 }
 
+// split-K partials of one lockstep copy
 size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus) {
   if (g.M == 0 || g.N == 0) return 0;
   GemmPlan pl = make_plan(g, num_cus, cmp);
@@ -474,11 +479,13 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
   p.ar = g.alpha_re; p.ai = g.alpha_im; p.br = g.beta_re; p.bi = g.beta_im;
   p.dg = g.diag_re;
   p.batch = (int)g.batch;
+  p.bst = g.bat.bs;
   p.P = nullptr;
+  REQUIRE(g.bat.nb >= 1 && g.batch * g.bat.nb <= 65535, "GEMM batch too large");
   if (g.K == 0) {
     // C = beta * C (alpha term vanishes): run the reducer with zero splits.
     p.splits = 0;
-    dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536));
+    dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536), (unsigned)g.bat.nb);
     if (g.tc == F64)
       hipLaunchKernelGGL(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
     else
@@ -490,7 +497,7 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
     REQUIRE(ws != nullptr && ws_bytes >= need, "GEMM workspace too small");
     p.P = ws;
   }
-  dim3 grid((unsigned)(pl.tiles_m * pl.tiles_n), (unsigned)pl.splits, (unsigned)g.batch);
+  dim3 grid((unsigned)(pl.tiles_m * pl.tiles_n), (unsigned)pl.splits, (unsigned)(g.batch * g.bat.nb));
   if (g.ta == F64 && g.tb == F64)
     launch_tc<double2, double2>(p, g.tc, cmp, pl.tm, grid, s);
   else if (g.ta == F64 && g.tb == F32)
@@ -501,7 +508,7 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
     launch_tc<float2, float2>(p, g.tc, cmp, pl.tm, grid, s);
   HIPCHK(hipGetLastError());
   if (pl.splits > 1) {
-    dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536));
+    dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536), (unsigned)g.bat.nb);
     if (g.tc == F64)
       hipLaunchKernelGGL(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
     else

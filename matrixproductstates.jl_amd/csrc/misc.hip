@@ -3,30 +3,34 @@
 
 namespace mpship {
 
-__global__ void set_scalar_kernel(double2* p, double re, double im) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = make_double2(re, im);
+// Batched copies: blockIdx.y selects the copy (byte offset blockIdx.y * bs).
+__global__ void set_scalar_kernel(double2* p, double re, double im, int64_t bs) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *boff(p, blockIdx.y * bs) = make_double2(re, im);
 }
 
-void launch_set_scalar(void* p, double re, double im, hipStream_t s) {
-  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, s, (double2*)p, re, im);
+void launch_set_scalar(void* p, double re, double im, hipStream_t s, Bat bat) {
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1, bat.nb), dim3(64), 0, s, (double2*)p, re, im, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
-__global__ void fill_zero_kernel(double2* p, int64_t n) {
+__global__ void fill_zero_kernel(double2* p, int64_t n, int64_t bs) {
+  p = boff(p, blockIdx.y * bs);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = make_double2(0.0, 0.0);
 }
 
-void launch_fill_zero(void* p, int64_t n, hipStream_t s) {
+void launch_fill_zero(void* p, int64_t n, hipStream_t s, Bat bat) {
   if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(fill_zero_kernel, dim3(g), dim3(256), 0, s, (double2*)p, n);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)p, n, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
-// sum |p_i|^2 with a fixed reduction order (one workgroup) -> out[0]
-__global__ __launch_bounds__(1024) void sum_abs2_kernel(const double2* p, int64_t n, double* out) {
+// sum |p_i|^2 with a fixed reduction order (one workgroup per copy) -> out[0]
+__global__ __launch_bounds__(1024) void sum_abs2_kernel(const double2* p, int64_t n, double* out, int64_t bs) {
+  p = boff(p, blockIdx.y * bs);
+  out = boff(out, blockIdx.y * bs);
   __shared__ double sh[16];
   double s = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += 1024) {
@@ -43,13 +47,15 @@ __global__ __launch_bounds__(1024) void sum_abs2_kernel(const double2* p, int64_
   }
 }
 
-void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(sum_abs2_kernel, dim3(1), dim3(1024), 0, s, (const double2*)p, n, out);
+void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s, Bat bat) {
+  hipLaunchKernelGGL(sum_abs2_kernel, dim3(1, bat.nb), dim3(1024), 0, s, (const double2*)p, n, out, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
 // p <- p / sqrt(norm2[0])   (normalize_lo!, orthonormalisation.jl:97-103)
-__global__ void scale_rsqrt_kernel(double2* p, int64_t n, const double* norm2) {
+__global__ void scale_rsqrt_kernel(double2* p, int64_t n, const double* norm2, int64_t bs) {
+  p = boff(p, blockIdx.y * bs);
+  norm2 = boff(norm2, blockIdx.y * bs);
   const double f = 1.0 / sqrt(norm2[0]);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -58,22 +64,25 @@ __global__ void scale_rsqrt_kernel(double2* p, int64_t n, const double* norm2) {
   }
 }
 
-void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s) {
+void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s, Bat bat) {
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(scale_rsqrt_kernel, dim3(g), dim3(256), 0, s, (double2*)p, n, norm2);
+  hipLaunchKernelGGL(scale_rsqrt_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)p, n, norm2, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
-__global__ void copy_c_kernel(double2* d, const double2* s, int64_t n) {
+__global__ void copy_c_kernel(double2* d, const double2* s, int64_t n, int64_t bs) {
+  d = boff(d, blockIdx.y * bs);
+  s = boff(s, blockIdx.y * bs);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     d[i] = s[i];
 }
 
-void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s) {
+void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s, Bat bat) {
   if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(copy_c_kernel, dim3(g), dim3(256), 0, s, (double2*)dst, (const double2*)src, n);
+  hipLaunchKernelGGL(copy_c_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)dst, (const double2*)src, n,
+                     bat.bs);
   HIPCHK(hipGetLastError());
 }
 

@@ -46,7 +46,10 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 __global__ __launch_bounds__(QR_NT) void qr_householder_kernel(double2* __restrict__ A, int m, int n,
                                                                int64_t rs, int64_t cs, double tol,
                                                                double2* __restrict__ tau,
-                                                               double2* __restrict__ Rout) {
+                                                               double2* __restrict__ Rout, int64_t bs) {
+  A = boff(A, blockIdx.y * bs);
+  tau = boff(tau, blockIdx.y * bs);
+  if (Rout != nullptr) Rout = boff(Rout, blockIdx.y * bs);
   __shared__ double sh[QR_NW];
   __shared__ double2 s_tau, s_scale;
   __shared__ double s_beta;
@@ -205,7 +208,9 @@ __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {   // conj(a)
 }
 
 __global__ __launch_bounds__(64) void qr_colnorm_kernel(const double2* __restrict__ A, int m, int64_t rs,
-                                                        int64_t cs, double* __restrict__ colnrm) {
+                                                        int64_t cs, double* __restrict__ colnrm, int64_t bs) {
+  A = boff(A, blockIdx.y * bs);
+  colnrm = boff(colnrm, blockIdx.y * bs);
   const int c = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
   for (int r = lane; r < m; r += 64) {
@@ -320,8 +325,16 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
                                                       int64_t cs, int c0, int nbp, double tol,
                                                       const double* __restrict__ colnrm,
                                                       double2* __restrict__ Vs, double2* __restrict__ tau,
-                                                      double2* __restrict__ Tp) {
+                                                      double2* __restrict__ Tp, int64_t bs) {
   static_assert(NT > NB, "the first row slot must cover the panel's diagonal");
+  {
+    const int64_t bo = blockIdx.y * bs;   // lockstep copy
+    A = boff(A, bo);
+    colnrm = boff(colnrm, bo);
+    Vs = boff(Vs, bo);
+    tau = boff(tau, bo);
+    Tp = boff(Tp, bo);
+  }
   constexpr int NW = NT / 64;
   constexpr int GMAX = (2 * (NB - 1) + 7) / 8 + 1;        // 8-value groups per column step (+1 look-ahead)
   static_assert(GMAX <= 8, "panel too wide");
@@ -651,7 +664,10 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
 template <int NB>
 __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restrict__ G, int n,
                                                         const double2* __restrict__ tau,
-                                                        double2* __restrict__ Tall) {
+                                                        double2* __restrict__ Tall, int64_t bs) {
+  G = boff(G, blockIdx.y * bs);
+  tau = boff(tau, blockIdx.y * bs);
+  Tall = boff(Tall, blockIdx.y * bs);
   __shared__ double2 sG[NB][NB + 1];
   const int p = blockIdx.x, c0 = p * NB, nbp = min(NB, n - c0), lane = threadIdx.x;
   for (int e = lane; e < NB * NB; e += 64) {
@@ -814,33 +830,6 @@ __device__ __forceinline__ void wy_apply(double2 (&x)[MAXR], const double2* __re
 }
 
 // geqrf trailing update: 4 trailing columns per workgroup, rows c0..m-1
-template <int MAXR, int NB>
-__global__ __launch_bounds__(WY_NT) void qr_apply_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
-                                                         int c0, int nbp, int ntrail,
-                                                         const double2* __restrict__ Vs,
-                                                         const double2* __restrict__ Tp) {
-  __shared__ WyStage<NB> S;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int jj = 4 * blockIdx.x + wave;
-  const int j = c0 + nbp + min(jj, ntrail - 1);
-  const int M = m - c0;
-  double2 x[MAXR];
-#pragma unroll
-  for (int i = 0; i < MAXR; ++i) {
-    const int rr = lane + 64 * i;
-    const double2 v = A[(int64_t)(c0 + min(rr, M - 1)) * rs + (int64_t)j * cs];
-    x[i] = (rr < M) ? v : make_double2(0.0, 0.0);
-  }
-  wy_apply<MAXR, NB, true>(x, Vs + (int64_t)c0 * m + c0, m, M, nbp, Tp, S);
-  if (jj < ntrail) {
-#pragma unroll
-    for (int i = 0; i < MAXR; ++i) {
-      const int rr = lane + 64 * i;
-      if (rr < M) A[(int64_t)(c0 + rr) * rs + (int64_t)j * cs] = x[i];
-    }
-  }
-}
-
 // geqrf trailing update, one workgroup per trailing column j:
 //   a_j <- (I - V T V^H)^H a_j = a_j - V u,  u = T^H (V^H a_j)   on rows c0..m-1.
 // Thread t holds rows t, t+NT, ... (RA of them) of a_j and of the panel's NB
@@ -855,7 +844,14 @@ template <int RA, int NB, int NT, bool OWN_G>
 __global__ __launch_bounds__(NT) void qr_apply_col_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
                                                           int c0, int nbp, const double2* __restrict__ Vs,
                                                           const double2* __restrict__ Tp,
-                                                          const double2* __restrict__ tau) {
+                                                          const double2* __restrict__ tau, int64_t bs) {
+  {
+    const int64_t bo = blockIdx.y * bs;   // lockstep copy
+    A = boff(A, bo);
+    Vs = boff(Vs, bo);
+    Tp = boff(Tp, bo);
+    tau = boff(tau, bo);
+  }
   constexpr int NW = NT / 64;
   constexpr int NPAIR = OWN_G ? NB * (NB - 1) / 2 : 0;
   constexpr int NVAL = 2 * NB + 2 * NPAIR;               // w (NB complex), then the Gram pairs
@@ -985,7 +981,11 @@ template <int MAXR, int NB>
 __global__ __launch_bounds__(WY_NT) void qr_formq_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
                                                          const double2* __restrict__ Vs,
                                                          const double2* __restrict__ Tall,
-                                                         double2* __restrict__ Rout, int n) {
+                                                         double2* __restrict__ Rout, int n, int64_t bs) {
+  A = boff(A, blockIdx.y * bs);
+  Vs = boff(Vs, blockIdx.y * bs);
+  Tall = boff(Tall, blockIdx.y * bs);
+  if (Rout != nullptr) Rout = boff(Rout, blockIdx.y * bs);
   __shared__ WyStage<NB> S;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = min(4 * (int)blockIdx.x + wave, n - 1);
@@ -1038,7 +1038,16 @@ __global__ __launch_bounds__(256) void qr_wsolve_kernel(const double2* __restric
                                                         int64_t cs, const double2* __restrict__ Vs,
                                                         const double2* __restrict__ G,
                                                         const double2* __restrict__ tau, double2* __restrict__ W,
-                                                        double2* __restrict__ Rout) {
+                                                        double2* __restrict__ Rout, int64_t bs) {
+  {
+    const int64_t bo = blockIdx.y * bs;   // lockstep copy
+    A = boff(A, bo);
+    Vs = boff(Vs, bo);
+    G = boff(G, bo);
+    tau = boff(tau, bo);
+    W = boff(W, bo);
+    if (Rout != nullptr) Rout = boff(Rout, bo);
+  }
   constexpr int NR = 64 * NS;                  // rows covered (n <= NR)
   constexpr int CH = (NS <= 2) ? 16 : 8;       // G columns per chunk (divides 64)
   constexpr int LPT = (CH * NR) / 256;         // staged elements per thread per chunk
@@ -1172,40 +1181,48 @@ struct QrWork {
 // factors for the Q formation come from the Gram GEMM's diagonal blocks.
 template <int R, int NB, int NT, int MAXR, int RA, int NTA>
 static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double tol, const QrWork& q,
-                       double2* R_out, hipStream_t s) {
+                       double2* R_out, hipStream_t s, Bat bat) {
   constexpr bool PT = (NB == 16);
   double2 *Vs = q.Vs, *tau = q.tau, *Tall = q.Tall;
   double* colnrm = q.colnrm;
-  if (tol > 0.0) hipLaunchKernelGGL(qr_colnorm_kernel, dim3((unsigned)n), dim3(64), 0, s, A, m, rs, cs, colnrm);
+  const unsigned nb = (unsigned)bat.nb;
+  const int64_t bs = bat.bs;
+  if (tol > 0.0)
+    hipLaunchKernelGGL(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(64), 0, s, A, m, rs, cs, colnrm, bs);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
-    hipLaunchKernelGGL((qr_panel_kernel<R, NB, NT, PT>), dim3(1), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
-                       tol, colnrm, Vs, tau, Tp);
+    hipLaunchKernelGGL((qr_panel_kernel<R, NB, NT, PT>), dim3(1, nb), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
+                       tol, colnrm, Vs, tau, Tp, bs);
     const int trail = n - c0 - nbp;
     if (trail > 0)
-      hipLaunchKernelGGL((qr_apply_col_kernel<RA, NB, NTA, !PT>), dim3((unsigned)trail), dim3(NTA), 0, s, A, m,
-                         rs, cs, c0, nbp, Vs, Tp, tau);
+      hipLaunchKernelGGL((qr_apply_col_kernel<RA, NB, NTA, !PT>), dim3((unsigned)trail, nb), dim3(NTA), 0, s, A,
+                         m, rs, cs, c0, nbp, Vs, Tp, tau, bs);
   }
-  const QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
+  QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
+  g.gram.bat = bat;
+  g.q.bat = bat;
   if (n <= FW_NMAX) {
     launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
-    const dim3 grid((unsigned)((n + 3) / 4));
+    const dim3 grid((unsigned)((n + 3) / 4), nb);
     if (n <= 64)
-      hipLaunchKernelGGL((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out);
+      hipLaunchKernelGGL((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
+                         bs);
     else if (n <= 128)
-      hipLaunchKernelGGL((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out);
+      hipLaunchKernelGGL((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
+                         bs);
     else
-      hipLaunchKernelGGL((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out);
+      hipLaunchKernelGGL((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
+                         bs);
     launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
   } else {
     if (!PT) {
       launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
-      hipLaunchKernelGGL((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB)), dim3(64), 0, s, q.G, n,
-                         tau, Tall);
+      hipLaunchKernelGGL((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB), nb), dim3(64), 0, s, q.G,
+                         n, tau, Tall, bs);
     }
-    hipLaunchKernelGGL((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4)), dim3(WY_NT), 0, s, A, m, rs,
-                       cs, Vs, Tall, R_out, n);
+    hipLaunchKernelGGL((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4), nb), dim3(WY_NT), 0, s, A, m,
+                       rs, cs, Vs, Tall, R_out, n, bs);
   }
 }
 
@@ -1240,8 +1257,9 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus) { return qr_layout(m, n, num_cus).total; }
 
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
-               void* R_out, hipStream_t s) {
+               void* R_out, hipStream_t s, Bat bat) {
   REQUIRE(m >= n && n >= 1, "qrq!: requires m >= n >= 1");
+  REQUIRE(bat.nb >= 1 && bat.nb <= 65535, "qrq!: too many lockstep copies");
   REQUIRE(m < (1LL << 31) && m * n < (1LL << 40), "qrq!: panel too large");
   char* w = (char*)ws;
   const QrLayout L = qr_layout(m, n, num_cus);
@@ -1266,18 +1284,18 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
   // m = 1152 panel 3.6 us/column vs 5.7 with 16 columns on 6 waves; at m = 256
   // the 16-column panel wins, at m = 2304 two waves per SIMD beat AGPR storage)
   if (m <= 256)
-    qr_blocked<1, 16, 256, 4, 1, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<1, 16, 256, 4, 1, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 512)
-    qr_blocked<2, 8, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<2, 8, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 768)
-    qr_blocked<3, 8, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<3, 8, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 1280)
-    qr_blocked<5, 8, 256, 20, 5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<5, 8, 256, 20, 5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 2560)
-    qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s);
+    qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else
-    hipLaunchKernelGGL(qr_householder_kernel, dim3(1), dim3(QR_NT), 0, s, A, (int)m, (int)n, rs, cs, tol, q.tau,
-                       R);
+    hipLaunchKernelGGL(qr_householder_kernel, dim3(1, (unsigned)bat.nb), dim3(QR_NT), 0, s, A, (int)m, (int)n, rs,
+                       cs, tol, q.tau, R, bat.bs);
   HIPCHK(hipGetLastError());
 }
 

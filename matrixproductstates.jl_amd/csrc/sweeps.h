@@ -37,6 +37,9 @@ struct Plan {
   void init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes);
 };
 
+// workspace bytes a plan of these sizes commits (per lockstep copy)
+size_t plan_bytes(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes, int num_cus, int nb);
+
 void qr_left(Plan& p, State* s, int j, void* Rout);
 void qr_right(Plan& p, State* s, int j, void* Rout);
 void left_orthogonalize(Plan& p, State* s, int slot);

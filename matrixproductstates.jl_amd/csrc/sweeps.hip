@@ -29,35 +29,53 @@ Plan::Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_
   init(Dm, dm, wm, n_ovl, extra_bytes);
 }
 
-void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
-  Ctx* c = ctx;
+// split-K partials per lockstep copy: splits * blocks < 4*CUs + blocks, and a
+// GEMM splits only while blocks (all copies counted) < CUs
+static size_t splitk_bound(int num_cus, int nb) {
+  return (size_t)((5 * num_cus + nb - 1) / nb + 64) * 64 * 64 * sizeof(double2);
+}
+
+// the regions of a plan, in order: add(slot, bytes)
+template <typename F>
+static void plan_layout(Plan* p, i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes, int num_cus, int nb,
+                        bool conc, F&& add) {
   const size_t cz = sizeof(double2);
   const size_t site = (size_t)Dm * dm * Dm;
   const size_t big = site * wm;
-  scr.add(&T1, big * cz);
-  scr.add(&T2, big * cz);
-  scr.add(&T3, big * cz);
-  scr.add(&AL, site * cz);
-  scr.add(&tmp, site * cz);
+  add(p ? &p->T1 : nullptr, big * cz);
+  add(p ? &p->T2 : nullptr, big * cz);
+  add(p ? &p->T3 : nullptr, big * cz);
+  add(p ? &p->AL : nullptr, site * cz);
+  add(p ? &p->tmp : nullptr, site * cz);
+  for (int l = 0; l < std::max(n_ovl, 0); ++l) add(p ? &p->ovl[l] : nullptr, site * cz);
+  add(p ? &p->tau : nullptr, qr_workspace_bytes(Dm * dm, Dm, num_cus));
+  add(p ? &p->R : nullptr, (size_t)Dm * Dm * dm * cz);
+  add(p ? &p->envA : nullptr, big * cz);
+  add(p ? &p->envB : nullptr, big * cz);
+  add(p ? &p->splitk : nullptr, splitk_bound(num_cus, nb));
+  if (conc) add(p ? &p->splitk2 : nullptr, splitk_bound(num_cus, nb));
+  if (extra_bytes) add(p ? &p->extra : nullptr, extra_bytes);
+}
+
+size_t plan_bytes(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes, int num_cus, int nb) {
+  size_t tot = 0;
+  plan_layout(nullptr, Dm, dm, wm, n_ovl, extra_bytes, num_cus, nb, false,
+              [&](void**, size_t b) { tot += (b + 255) & ~size_t(255); });
+  return tot;
+}
+
+void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
+  Ctx* c = ctx;
+  const size_t site = (size_t)Dm * dm * Dm;
   ovl.assign(std::max(n_ovl, 0), nullptr);
-  for (auto& p : ovl) scr.add(&p, site * cz);
-  scr.add(&tau, qr_workspace_bytes(Dm * dm, Dm, c->num_cus));
-  scr.add(&R, (size_t)Dm * Dm * dm * cz);
-  scr.add(&envA, big * cz);
-  scr.add(&envB, big * cz);
-  // split-K partials: splits * blocks < 4*CUs + blocks, blocks < CUs when split
-  const size_t sk = (size_t)(5 * c->num_cus + 64) * 64 * 64 * cz;
-  scr.add(&splitk, sk);
-  splitk_bytes = sk;
   // the side stream pays off once a site's products fill a good part of the
   // GPU (measured: D = 128, d = 9 gains; bond-16/32 chains served many to a
   // GPU lose, their streams then outnumber the hardware queues)
-  conc = n_ovl > 0 && !c->profiling && site >= 32768;
-  if (conc) {
-    c->ensure_side();
-    scr.add(&splitk2, sk);
-  }
-  if (extra_bytes) scr.add(&extra, extra_bytes);
+  conc = n_ovl > 0 && !c->profiling && site >= 32768 && c->bat.nb == 1;
+  if (conc) c->ensure_side();
+  plan_layout(this, Dm, dm, wm, n_ovl, extra_bytes, c->num_cus, c->bat.nb, conc,
+              [&](void** slot, size_t b) { scr.add(slot, b); });
+  splitk_bytes = splitk_bound(c->num_cus, c->bat.nb);
   scr.commit();
   work.c = c;
   work.splitk = splitk;
@@ -80,7 +98,7 @@ static void run_qr(Plan& p, void* base, i64 m, i64 n, i64 rs, i64 cs, void* Rout
   const bool prof = c->profiling && !c->capturing;
   hipEvent_t a = nullptr;
   if (prof) c->prof_begin(a);
-  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, c->num_cus, Rout, c->stream);
+  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, c->num_cus, Rout, c->stream, c->bat);
   // zgeqrf + zungqr: 2 x 8 (m n^2 - n^3 / 3) flops; panel read and written once
   if (prof) c->prof_end(a, 1, 16.0 * ((double)m * n * n - (double)n * n * n / 3.0), 32.0 * (double)m * n);
 }
@@ -107,10 +125,10 @@ void left_orthogonalize(Plan& p, State* s, int slot) {
     // mps[i+1] = prod_LR(r, mps[i+1]) : r is Dr x Dr
     const i64 Dr = s->Dr(i), d1 = s->d[i + 1], Dr1 = s->Dr(i + 1);
     prod_LR_23(p.work, p.tmp, p.R, s->site(i + 1), Dr, Dr, d1, Dr1, false, false);
-    launch_copy_c(s->site(i + 1), p.tmp, Dr * d1 * Dr1, p.ctx->stream);
+    launch_copy_c(s->site(i + 1), p.tmp, Dr * d1 * Dr1, p.ctx->stream, p.ctx->bat);
   }
   const int n = s->N - 1;
-  launch_sum_abs2(s->site(n), s->size(n), p.ctx->dscal + slot, p.ctx->stream);
+  launch_sum_abs2(s->site(n), s->size(n), p.ctx->dscal + slot, p.ctx->stream, p.ctx->bat);
 }
 
 // rightorthogonalizeQR! (orthonormalisation.jl:77-90)
@@ -120,9 +138,9 @@ void right_orthogonalize(Plan& p, State* s, int slot) {
     // mps[i-1] = prod_LR(mps[i-1], C.') with C = R (Dl x Dl) -> multiply by R^T
     const i64 Dl = s->Dl(i), Dlm = s->Dl(i - 1), dm = s->d[i - 1];
     prod_LR_32(p.work, p.tmp, s->site(i - 1), p.R, Dlm, dm, Dl, Dl, false, true);
-    launch_copy_c(s->site(i - 1), p.tmp, Dlm * dm * Dl, p.ctx->stream);
+    launch_copy_c(s->site(i - 1), p.tmp, Dlm * dm * Dl, p.ctx->stream, p.ctx->bat);
   }
-  launch_sum_abs2(s->site(0), s->size(0), p.ctx->dscal + slot, p.ctx->stream);
+  launch_sum_abs2(s->site(0), s->size(0), p.ctx->dscal + slot, p.ctx->stream, p.ctx->bat);
 }
 
 double read_scalar(Ctx* c, int slot) {
@@ -144,9 +162,9 @@ void Plan::join() {
 }
 
 // ---------------------------------------------------------------- helpers
-static void set1(Ctx* c, double2* p, cd v) { launch_set_scalar(p, v.real(), v.imag(), c->stream); }
+static void set1(Ctx* c, double2* p, cd v) { launch_set_scalar(p, v.real(), v.imag(), c->stream, c->bat); }
 static void set1(const Work& w, double2* p, cd v) {
-  launch_set_scalar(p, v.real(), v.imag(), w.stream ? w.stream : w.c->stream);
+  launch_set_scalar(p, v.real(), v.imag(), w.stream ? w.stream : w.c->stream, w.c->bat);
 }
 
 static void check_same_dims(const State* a, const State* b, const char* what) {
@@ -384,7 +402,7 @@ void compress_sum_var(Plan& p, State* out, State* Ain, int numv, const cd* coef,
   }
   // Ain (if distinct) is only read during initialisation; copy it into out.
   if (out != Ain)
-    for (int j = 0; j < N; ++j) launch_copy_c(out->site(j), Ain->site(j), Ain->size(j), c->stream);
+    for (int j = 0; j < N; ++j) launch_copy_c(out->site(j), Ain->site(j), Ain->size(j), c->stream, c->bat);
   for (int sw = 0; sw < maxsweeps; ++sw) {
     for (int j = N - 1; j >= 1; --j) {
       const i64 Dl0 = out->Dl(j), Dr0 = out->Dr(j), d = out->d[j];
@@ -525,7 +543,7 @@ void scal_prod(Plan& p, State* A, State* B, int mode, int slot) {
     update_lenv(p.work, nxt, A->site(j), p.AL, A->Dl(j), A->d[j], A->Dr(j), B->Dr(j), mode);
     std::swap(cur, nxt);
   }
-  launch_copy_c(c->dscal + slot, cur, 1, c->stream);
+  launch_copy_c(c->dscal + slot, cur, 1, c->stream, c->bat);
 }
 
 // scal_op_prod (MatrixProductStates.jl:327-340): conj(<A|O|B>) -> dscal[slot..]
@@ -541,7 +559,7 @@ void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot) {
     update_le(p.work, p.op, nxt, p.T2, A->site(j), A->Dl(j), A->d[j], B->Dr(j), O->wr(j), A->Dr(j));
     std::swap(cur, nxt);
   }
-  launch_copy_c(c->dscal + slot, cur, 1, c->stream);
+  launch_copy_c(c->dscal + slot, cur, 1, c->stream, c->bat);
 }
 
 }  // namespace mpship

@@ -7,7 +7,7 @@ Julia's mutating ``f!`` is spelled ``f_``.
 from . import models  # noqa: F401  (host-side model builders, numpy only)
 from ._lib import MPSError, DimensionMismatch, load  # noqa: F401
 from .core import (  # noqa: F401
-    Context, MPS, MPO, Env, default_context,
+    Context, MPS, MPO, Env, default_context, batch_workspace_bytes, normalize_lo_batch,
     mpsproductstate, mpsgroundstate, mpsrandom, makempo, build_env, mpsdims,
     leftorth_, rightorth_, leftorthogonalizeQR_, leftorthonormalizationQR_,
     rightorthogonalizeQR_, rightorthonormalizationQR_, normalize_lo_, norm_lo,

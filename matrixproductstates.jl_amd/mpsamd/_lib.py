@@ -28,6 +28,13 @@ PVP = C.POINTER(C.c_void_p)
 SIGNATURES = {
     "mps_version": [],
     "mps_ctx_create": [i32, PVP],
+    "mps_ctx_create_batch": [i32, i32, i64, i64, PVP],
+    "mps_batch_workspace_bytes": [i32, i32, i64, i64, i64, P64],
+    "mps_ctx_create_companion": [vp, PVP],
+    "mps_state_view": [vp, i32, vp, PVP],
+    "mps_env_view": [vp, i32, vp, PVP],
+    "mps_mpo_view": [vp, i32, vp, PVP],
+    "mps_normalize_lo_batch": [vp, PD],
     "mps_ctx_destroy": [vp],
     "mps_ctx_set_stream": [vp, vp],
     "mps_ctx_synchronize": [vp],

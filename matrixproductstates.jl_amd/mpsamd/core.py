@@ -43,6 +43,26 @@ class Context:
         call("mps_ctx_create", int(device), C.byref(h))
         self.h = h
         self.device = device
+        self.nb = 1
+
+    @classmethod
+    def batched(cls, nb: int, copy_bytes: int, ws_bytes: int, device: int = 0) -> "Context":
+        """A lockstep batch: nb copies of every container created on it
+        (copy_bytes of containers and ws_bytes of workspace per copy), every
+        sweep / QR / norm call running all copies in the same launches."""
+        obj = cls.__new__(cls)
+        h = C.c_void_p()
+        call("mps_ctx_create_batch", int(device), int(nb), int(copy_bytes), int(ws_bytes), C.byref(h))
+        obj.h, obj.device, obj.nb = h, device, int(nb)
+        return obj
+
+    def companion(self) -> "Context":
+        """A plain context on this context's stream, for views of single copies."""
+        obj = Context.__new__(Context)
+        h = C.c_void_p()
+        call("mps_ctx_create_companion", self.h, C.byref(h))
+        obj.h, obj.device, obj.nb = h, self.device, 1
+        return obj
 
     def close(self):
         if self.h:
@@ -77,6 +97,22 @@ class Context:
         call("mps_ctx_kernel_stats", self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return {"gemm_ms": out[0], "gemm_flops": out[1], "gemm_bytes": out[2], "gemm_launches": int(out[3]),
                 "qr_ms": out[4], "qr_flops": out[5], "qr_launches": int(out[6])}
+
+
+def batch_workspace_bytes(nb: int, Dmax: int, d: int, wmax: int, device: int = 0) -> int:
+    """Workspace per copy of a batched context for RK4 / sweep calls at these sizes."""
+    out = C.c_int64()
+    call("mps_batch_workspace_bytes", int(device), int(nb), int(Dmax), int(d), int(wmax), C.byref(out))
+    return int(out.value)
+
+
+def _view(cls, name, base, b, ctx):
+    obj = cls.__new__(cls)
+    obj.__dict__.update({k: v for k, v in base.__dict__.items() if k not in ("h", "ctx")})
+    h = C.c_void_p()
+    call(name, base.h, int(b), ctx.h, C.byref(h))
+    obj.h, obj.ctx, obj._base = h, ctx, base   # the view keeps its batched container alive
+    return obj
 
 
 _default_ctx: Context | None = None
@@ -142,6 +178,10 @@ class MPS:
     def to_host(self):
         return [self.site(j) for j in range(self.N)]
 
+    def view(self, b: int, ctx: Context) -> "MPS":
+        """Copy b of a batched state, on the companion context ctx."""
+        return _view(MPS, "mps_state_view", self, b, ctx)
+
     def copy_from(self, other: "MPS"):
         """A .= copy.(B)"""
         call("mps_state_copy", self.h, other.h)
@@ -181,6 +221,9 @@ class MPO:
         keep, p = _cbuf(s)
         call("mps_mpo_upload_site", self.h, j, p, s.shape[0], s.shape[1], s.shape[2])
 
+    def view(self, b: int, ctx: Context) -> "MPO":
+        return _view(MPO, "mps_mpo_view", self, b, ctx)
+
     def set_block(self, j, wl_idx, wr_idx, op):
         """O[j][wl_idx, :, wr_idx, :] = op (update_h! / update_L_Ryd!)."""
         keep, p = _cbuf(np.asarray(op, dtype=CDT))
@@ -214,6 +257,9 @@ class Env:
         self.h = h
         self.dims1, self.dims2 = list(dims1), list(dims2)
         self.mpo_dims = list(mpo_dims) if mpo_dims is not None else None
+
+    def view(self, b: int, ctx: Context) -> "Env":
+        return _view(Env, "mps_env_view", self, b, ctx)
 
     def slot(self, j, shape):
         n = int(np.prod(shape))
@@ -345,6 +391,13 @@ def rightorthonormalizationQR_(mps):
 def normalize_lo_(mps):
     """orthonormalisation.jl:97-103."""
     return _scalar("mps_normalize_lo", mps)
+
+
+def normalize_lo_batch(mps) -> np.ndarray:
+    """normalize_lo! of every copy of a batched state; norm^2 per copy."""
+    out = np.zeros(mps.ctx.nb)
+    call("mps_normalize_lo_batch", mps.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
 
 
 def norm_lo(mps):

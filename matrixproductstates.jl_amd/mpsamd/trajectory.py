@@ -37,10 +37,10 @@ class SpinTrajectory:
     """
 
     def __init__(self, N, D, seed, dt=0.01, Omega=0.5, Gamma=1.0, k=2 * np.pi, L=None, gam_eg=0.0,
-                 init=(1.0, 0.0), ctx: M.Context | None = None):
+                 init=(1.0, 0.0), ctx: M.Context | None = None, pos_seed=None):
         self.N, self.D, self.dt, self.seed, self.gam_eg = N, D, dt, seed, gam_eg
         self.ctx = ctx if ctx is not None else M.default_context()
-        self.z = MD.positions(N, float(N if L is None else L), seed)
+        self.z = MD.positions(N, float(N if L is None else L), seed if pos_seed is None else pos_seed)
         self.H = [M.MPO(h, self.ctx) for h in MD.spin_rk4_mpos(self.z, 0.0, dt, Omega, Gamma, k)]
         jr, jl = MD.spin_jump_mpos(self.z, Gamma, k)
         self.jr, self.jl = M.MPO(jr, self.ctx), M.MPO(jl, self.ctx)
@@ -129,3 +129,119 @@ def pick_jump(acc, r2, N):
     if r2 < acc[N + 1]:
         return "r", -1
     return "l", -1
+
+
+def _state_bytes(dims, d):
+    return 16 * sum(dims[j] * d * dims[j + 1] for j in range(len(dims) - 1))
+
+
+def _env_bytes(dims, w):
+    N = len(dims) - 1
+    tot = 0
+    for j in range(N + 1):
+        edge = j in (0, N)
+        b = 1 if edge else dims[j]
+        tot += b * b * (1 if edge else w)
+    return 16 * tot
+
+
+def _mpo_bytes(sites):
+    return 16 * sum(s.size for s in sites)
+
+
+class SpinEnsemble:
+    """A lockstep ensemble of B trajectories of one driven spin chain.
+
+    All trajectories share the system (positions from ``pos_seed``, hence the
+    MPOs) and differ in their jump records: trajectory b draws r1, r2 from
+    ``numpy.random.default_rng(seeds[b])`` exactly as ``SpinTrajectory`` with
+    ``seed = seeds[b], pos_seed = pos_seed`` does.  The coherent RK4 step and
+    ``normalize_lo!`` of all B run as one batched call (every GEMM, QR panel
+    and reduction launched once for the ensemble, on a batched context); the
+    jump branch of a trajectory (spin_model_quantum_trajectory_rk4.jl:140-177)
+    runs on views of its copy, on the companion context (same stream).
+    """
+
+    def __init__(self, N, D, pos_seed, seeds, dt=0.01, Omega=0.5, Gamma=1.0, k=2 * np.pi, L=None,
+                 gam_eg=0.0, init=(1.0, 0.0), qr_rank_tol=0.0, device=0):
+        self.N, self.D, self.dt, self.gam_eg = N, D, dt, gam_eg
+        self.B = B = len(seeds)
+        self.z = MD.positions(N, float(N if L is None else L), pos_seed)
+        Hs = MD.spin_rk4_mpos(self.z, 0.0, dt, Omega, Gamma, k)
+        jr, jl = MD.spin_jump_mpos(self.z, Gamma, k)
+        dims = M._dims_rule(N, D, 2)
+        w = max(h.shape[0] for hs in Hs for h in hs)
+        slack = 256 * 64
+        copy_bytes = (5 * (_state_bytes(dims, 2) + 256) + 3 * (_env_bytes(dims, 1) + 256)
+                      + _env_bytes(dims, w) + 256 + sum(_mpo_bytes(h) + 256 for h in Hs) + slack)
+        ws = M.batch_workspace_bytes(B, max(dims), 2, w, device)
+        self.ctx = M.Context.batched(B, copy_bytes, ws, device)
+        self.ctx.set_qr_rank_tol(qr_rank_tol)
+        self.comp = self.ctx.companion()
+        self.comp.set_qr_rank_tol(qr_rank_tol)
+        self.H = [M.MPO(h, self.ctx) for h in Hs]
+        self.jr, self.jl = M.MPO(jr, self.comp), M.MPO(jl, self.comp)
+        self.A, self.dims = M.mpsproductstate(N, D, 2, np.asarray(init, dtype=complex), self.ctx)
+        self.A_coh, self.A1, self.A2, self.A3 = (self.A.similar() for _ in range(4))
+        for s in (self.A_coh, self.A1, self.A2, self.A3):
+            s.copy_from(self.A)
+        self.envop = M.build_env(self.dims, self.dims, [1] + [w] * (N - 1) + [1], ctx=self.ctx)
+        self.env = [M.build_env(self.dims, self.dims, ctx=self.ctx) for _ in range(3)]
+        # jump scratch on the companion (one trajectory at a time)
+        self.A_r = M.MPS(N, 2, self.dims, self.comp)
+        self.A_l = M.MPS(N, 2, self.dims, self.comp)
+        self.envop_jump = M.build_env(self.dims, self.dims, [1] + [2] * (N - 1) + [1], ctx=self.comp)
+        self.Av = [self.A.view(b, self.comp) for b in range(B)]
+        self.Acv = [self.A_coh.view(b, self.comp) for b in range(B)]
+        self.rngs = [np.random.default_rng(s) for s in seeds]
+        self.p_coh = [[] for _ in range(B)]
+        self.jumps = [[] for _ in range(B)]
+        self.t = 0.0
+
+    def step(self):
+        """One step of every trajectory; returns the B coherent norms p_coh."""
+        M.RK4stp_apply_H_half_(self.dt, self.H, self.A_coh, self.A, self.A1, self.A2, self.A3, *self.env,
+                               self.envop)
+        p = M.normalize_lo_batch(self.A_coh)
+        for b in range(self.B):
+            self.p_coh[b].append(float(p[b]))
+            r1, r2 = self.rngs[b].random(2)
+            i = len(self.p_coh[b]) - 1
+            if r1 < p[b]:
+                continue
+            A, Ac = self.Av[b], self.Acv[b]
+            M.compress_var_apply_H_(self.A_r, self.envop_jump, A, A, self.jr, 1)
+            p_r = M.normalize_lo_(self.A_r)
+            M.compress_var_apply_H_(self.A_l, self.envop_jump, A, A, self.jl, 1)
+            p_l = M.normalize_lo_(self.A_l)
+            epop = M.measure_excitations_(None, A, HEE)
+            acc = jump_table(epop, p_r, p_l, self.dt, self.gam_eg)
+            kind, j = pick_jump(acc, r2, self.N)
+            # the jump result replaces the candidate coherent state of copy b
+            if kind == "eg":
+                Ac.copy_from(A)
+                M.apply_site_operator_(Ac, j, HGE)
+                M.leftorthonormalizationQR_(Ac)
+            elif kind == "r":
+                Ac.copy_from(self.A_r)
+            else:
+                Ac.copy_from(self.A_l)
+            self.jumps[b].append((i, kind, j))
+        self.A.copy_from(self.A_coh)
+        self.t += self.dt
+        return p
+
+    def sigma_z(self, b):
+        """<sigma_z^j> of trajectory b."""
+        return 2.0 * M.measure_excitations_(None, self.Av[b], HEE) - 1.0
+
+    def close(self):
+        """Release views, containers, then the two contexts."""
+        for name in ("Av", "Acv"):
+            for v in getattr(self, name):
+                v.__del__()
+        for o in (self.A_r, self.A_l, self.envop_jump, self.jr, self.jl, self.A, self.A_coh, self.A1, self.A2,
+                  self.A3, self.envop, *self.env, *self.H):
+            o.__del__()
+        self.comp.close()
+        self.ctx.close()
