@@ -93,6 +93,44 @@ def cpu_baseline(N, D, n_exp, dt, seed, steps_cpu=1):
                       f"(numpy/OpenBLAS zgemm + LAPACK zgeqrf/zungqr), {t:.1f} s"}
 
 
+def ensemble_leg(seed, N=60, D=32, B=256, steps=10, n_seq=4):
+    """Lockstep ensemble of B quantum-jump trajectories of one system (one
+    batched context: every kernel launched once for all B) against the same
+    trajectories solved one after another on a plain context."""
+    import mpsamd as M
+    from mpsamd.trajectory import SpinEnsemble, SpinTrajectory
+    ctx = M.Context(0)
+    trs = [SpinTrajectory(N, D, seed + s, dt=0.01, ctx=ctx, pos_seed=seed) for s in range(n_seq)]
+    for tr in trs:          # first two steps: eager, then graph capture
+        tr.step(), tr.step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for tr in trs:
+        for _ in range(steps):
+            tr.step()
+    ctx.synchronize()
+    seq = (time.perf_counter() - t0) / (n_seq * steps)
+    for tr in trs:
+        for v in list(vars(tr).values()):
+            for o in (v if isinstance(v, (list, tuple)) else [v]):
+                if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                    o.__del__()
+    ctx.close()
+    ens = SpinEnsemble(N, D, seed, [seed + s for s in range(B)], dt=0.01)
+    ens.step(), ens.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step()
+    per = (time.perf_counter() - t0) / (B * steps)
+    jumps = sum(len(j) for j in ens.jumps)
+    ens.close()
+    out = {"trajectories": B, "N": N, "D": D, "steps_timed": steps, "jumps": jumps,
+           "trajectory_steps_per_s": round(1.0 / per, 1), "sequential_trajectory_steps_per_s": round(1.0 / seq, 1),
+           "speedup_vs_sequential": round(seq / per, 2)}
+    print(f"lockstep ensemble: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_and_serve(seed):
     """Batched quantum-jump trajectories (concurrent per-stream solves vs the same
     solves one after another) and a Poisson request stream through the server."""
@@ -121,6 +159,21 @@ def batch_and_serve(seed):
                "throughput_req_s": round(st["throughput_req_s"], 3),
                "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
     return batched, serving
+
+
+def pooled_serving_leg(seed, n=480, rate=120.0, systems=4, N=60, D=32, slots=64):
+    """Trajectory requests over a few systems, continuously batched in lockstep
+    pools (one batched context per system, requests join and leave slots
+    between ticks)."""
+    from mpsamd import serving as S
+    sy = [(N, D, seed + i) for i in range(systems)]
+    reqs = S.poisson_trajectory_requests(n, rate=rate, seed=seed, systems=sy, steps_range=(10, 40))
+    st = S.PooledServer(slots=slots, max_pools=systems, dt=0.01).run(reqs)
+    print(f"pooled serving: {st}", file=sys.stderr, flush=True)
+    return {"requests": st["requests"], "arrival_rate_req_s": rate, "systems": systems, "slots_per_pool": slots,
+            "mix": f"quantum-jump trajectories of {systems} driven chains (d=2, N={N}, D={D}), 10-40 steps each",
+            "throughput_req_s": round(st["throughput_req_s"], 3),
+            "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
 
 
 def main():
@@ -263,7 +316,9 @@ def main():
 
     batched = serving = None
     if not args.no_serve:
-        batched, serving = batch_and_serve(args.seed)
+        uneven, per_solve = batch_and_serve(args.seed)
+        batched = {"lockstep_ensemble": ensemble_leg(args.seed), "concurrent_uneven": uneven}
+        serving = {"lockstep_pools": pooled_serving_leg(args.seed), "per_solve_streams": per_solve}
 
     cpu = None
     if not args.no_cpu:

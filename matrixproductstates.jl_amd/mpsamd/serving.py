@@ -17,6 +17,14 @@ footprint of the active set stays under a budget and a slot is free, and
 reports throughput and p50/p99 latency.  A request's result does not depend
 on what else is running: each solve has its own stream and workspace, and
 every kernel reduces in a fixed order.
+
+Trajectory requests of one system (same chain, same MPOs, same shapes) are
+batched in lockstep instead (``PooledServer``): a pool holds ``slots`` copies
+of the system on one batched context, every tick advances all of them with
+one launch per kernel, and a request joins a free slot as soon as it arrives
+and leaves it as soon as it is done (continuous batching).  Each copy's
+arithmetic reads only its own data, in a fixed order, so a response is the
+same whichever slot it took and whatever else shared the pool.
 """
 from __future__ import annotations
 
@@ -27,7 +35,7 @@ import numpy as np
 
 from . import core as M
 from . import models as MD
-from .trajectory import SpinTrajectory
+from .trajectory import SpinEnsemble, SpinTrajectory
 
 
 class RydbergEvolution:
@@ -236,4 +244,159 @@ def poisson_requests(n, rate, seed, spin_frac=0.7, N_range=(20, 60), D_choices=(
             out.append(Request(i, "rydberg", int(rng.integers(*rydberg_N_range, endpoint=True)),
                                int(rng.choice(rydberg_D_choices)), int(rng.integers(*steps_range, endpoint=True)),
                                int(rng.integers(1 << 30)), t))
+    return out
+
+
+# ------------------------------------------------------ lockstep trajectory pools
+@dataclass
+class TrajRequest:
+    """One quantum-jump trajectory of a system: (N, D, pos_seed) fixes the
+    chain and its MPOs, ``seed`` the jump draws, ``steps`` its length."""
+    rid: int
+    N: int
+    D: int
+    pos_seed: int
+    seed: int
+    steps: int
+    arrival: float = 0.0
+    start: float = 0.0
+    done: float = 0.0
+    result: np.ndarray | None = None
+    left: int = 0
+
+
+class TrajectoryPool:
+    """Continuous lockstep batching of one system's trajectories.
+
+    ``slots`` copies of the system live on one batched context
+    (``SpinEnsemble``).  A request takes a free slot (its copy reset to the
+    initial product state and its draws to the request's seed); every tick
+    advances all slots with one batched RK4 call and one batched norm, then
+    runs the jump branch of the occupied slots; a finished request's slot is
+    free for the next one at once.  A slot's arithmetic is independent of the
+    other slots, so a response does not depend on what else runs.
+    """
+
+    def __init__(self, N, D, pos_seed, slots=64, **system):
+        self.key = (N, D, pos_seed)
+        self.ens = SpinEnsemble(N, D, pos_seed, [0] * slots, qr_rank_tol=QR_RANK_TOL, **system)
+        self.req = [None] * slots
+
+    def free_slots(self):
+        return [b for b, r in enumerate(self.req) if r is None]
+
+    def admit(self, r: TrajRequest, b: int):
+        self.ens.reset(b, r.seed)
+        self.req[b] = r
+        r.left = r.steps
+
+    def busy(self):
+        return any(r is not None for r in self.req)
+
+    def launch(self):
+        """Queue the batched coherent step (no host wait)."""
+        self.ens.launch()
+
+    def finish(self):
+        """Norms, jump branches and retirement; returns the finished requests."""
+        active = [b for b, r in enumerate(self.req) if r is not None]
+        self.ens.finish(active)
+        out = []
+        for b in active:
+            r = self.req[b]
+            r.left -= 1
+            if r.left == 0:
+                r.result = np.concatenate([self.ens.sigma_z(b), [self.ens.p_coh[b][-1]]])
+                self.req[b] = None
+                out.append(r)
+        return out
+
+    def close(self):
+        self.ens.close()
+
+
+def solve_alone(r: TrajRequest, **system):
+    """The same trajectory on its own plain context (the sequential baseline)."""
+    ctx = M.Context(0)
+    ctx.set_qr_rank_tol(QR_RANK_TOL)
+    tr = SpinTrajectory(r.N, r.D, r.seed, ctx=ctx, pos_seed=r.pos_seed, **system)
+    for _ in range(r.steps):
+        tr.step()
+    r.result = np.concatenate([tr.sigma_z(), [tr.p_coh[-1]]])
+    for v in list(vars(tr).values()):
+        for o in (v if isinstance(v, (list, tuple)) else [v]):
+            if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                o.__del__()
+    ctx.close()
+    return r
+
+
+class PooledServer:
+    """Continuous batching of trajectory requests over per-system pools.
+
+    Requests arrive with arrival times; each tick admits the arrived requests
+    into free slots of their system's pool (a pool is created, within
+    ``max_pools``, when a system first arrives), queues one batched step of
+    every busy pool (each pool has its own stream), then collects the norms,
+    runs the jump branches and retires finished requests.  Latency =
+    completion - arrival.  ``system`` (dt, Omega, Gamma, gam_eg, init ...) is
+    shared by every pool.
+    """
+
+    def __init__(self, slots=64, max_pools=8, **system):
+        self.slots, self.max_pools, self.system = slots, max_pools, system
+
+    def run(self, requests):
+        pending = sorted(requests, key=lambda r: r.arrival)
+        pools = {}
+        done = []
+        t0 = time.perf_counter()
+        while pending or any(p.busy() for p in pools.values()):
+            now = time.perf_counter() - t0
+            keep = []
+            for r in pending:
+                if r.arrival > now:
+                    keep.append(r)
+                    continue
+                key = (r.N, r.D, r.pos_seed)
+                pool = pools.get(key)
+                if pool is None and len(pools) < self.max_pools:
+                    pool = pools[key] = TrajectoryPool(r.N, r.D, r.pos_seed, self.slots, **self.system)
+                free = pool.free_slots() if pool is not None else []
+                if not free:
+                    keep.append(r)
+                    continue
+                r.start = now
+                pool.admit(r, free[0])
+            pending = keep
+            busy = [p for p in pools.values() if p.busy()]
+            if not busy:
+                time.sleep(max(0.0, min(pending[0].arrival - now, 0.01)) if pending else 0.0)
+                continue
+            for p in busy:
+                p.launch()
+            for p in busy:
+                for r in p.finish():
+                    r.done = time.perf_counter() - t0
+                    done.append(r)
+        wall = time.perf_counter() - t0
+        for p in pools.values():
+            p.close()
+        lat = np.array([r.done - r.arrival for r in done])
+        return {"requests": len(done), "wall_s": wall, "pools": len(pools),
+                "throughput_req_s": len(done) / wall,
+                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+
+
+def poisson_trajectory_requests(n, rate, seed, systems, steps_range=(10, 40)):
+    """Seeded stream of trajectory requests over ``systems`` [(N, D, pos_seed)]:
+    exponential inter-arrival times at ``rate`` per second."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        N, D, ps = systems[int(rng.integers(len(systems)))]
+        out.append(TrajRequest(i, N, D, ps, int(rng.integers(1 << 30)),
+                               int(rng.integers(steps_range[0], steps_range[1], endpoint=True)), t))
     return out

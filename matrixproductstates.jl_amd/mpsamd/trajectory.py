@@ -194,16 +194,39 @@ class SpinEnsemble:
         self.Av = [self.A.view(b, self.comp) for b in range(B)]
         self.Acv = [self.A_coh.view(b, self.comp) for b in range(B)]
         self.rngs = [np.random.default_rng(s) for s in seeds]
+        self._init_vec = init
         self.p_coh = [[] for _ in range(B)]
         self.jumps = [[] for _ in range(B)]
         self.t = 0.0
 
-    def step(self):
-        """One step of every trajectory; returns the B coherent norms p_coh."""
+    def reset(self, b, seed):
+        """Start a new trajectory in copy b: the initial product state, a fresh
+        jump record and draws from default_rng(seed)."""
+        if not hasattr(self, "_init"):
+            self._init, _ = M.mpsproductstate(self.N, self.D, 2, np.asarray(self._init_vec, dtype=complex),
+                                              self.comp)
+        self.Av[b].copy_from(self._init)
+        self.rngs[b] = np.random.default_rng(seed)
+        self.p_coh[b] = []
+        self.jumps[b] = []
+
+    def launch(self):
+        """Queue the batched coherent RK4 step of all B copies (no host wait)."""
         M.RK4stp_apply_H_half_(self.dt, self.H, self.A_coh, self.A, self.A1, self.A2, self.A3, *self.env,
                                self.envop)
+
+    def step(self, active=None):
+        """One step of every trajectory (the jump branch only for copies in
+        ``active``, default all); returns the B coherent norms p_coh."""
+        self.launch()
+        return self.finish(active)
+
+    def finish(self, active=None):
+        """Batched ``normalize_lo!`` (waits for the stream), then the draws and
+        jump branches of the copies in ``active``; copies outside it evolve
+        coherently."""
         p = M.normalize_lo_batch(self.A_coh)
-        for b in range(self.B):
+        for b in (range(self.B) if active is None else active):
             self.p_coh[b].append(float(p[b]))
             r1, r2 = self.rngs[b].random(2)
             i = len(self.p_coh[b]) - 1
@@ -240,6 +263,8 @@ class SpinEnsemble:
         for name in ("Av", "Acv"):
             for v in getattr(self, name):
                 v.__del__()
+        if hasattr(self, "_init"):
+            self._init.__del__()
         for o in (self.A_r, self.A_l, self.envop_jump, self.jr, self.jl, self.A, self.A_coh, self.A1, self.A2,
                   self.A3, self.envop, *self.env, *self.H):
             o.__del__()

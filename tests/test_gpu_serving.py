@@ -60,3 +60,79 @@ def test_server_poisson_stream():
         a = S.Request(r.rid, r.kind, r.N, r.D, r.steps, r.seed)
         S.run_batch([a])
         assert np.abs(a.result - r.result).max() <= SAME
+
+
+# ------------------------------------------------------ lockstep trajectory pools
+SYS = dict(dt=0.05, gam_eg=0.5, init=(0.0, 1.0))   # decay from |e...e>: jumps within the first steps
+
+
+def _pool_run(pool, schedule, ticks):
+    """schedule: {tick: [(request, slot)]}; runs ``ticks`` ticks."""
+    done = []
+    for t in range(ticks):
+        for r, b in schedule.get(t, []):
+            pool.admit(r, b)
+        if pool.busy():
+            pool.launch()
+            done += pool.finish()
+    return done
+
+
+def test_pool_result_same_alone_and_in_the_mix():
+    N, D, ps = 10, 8, 3
+    alone = S.TrajRequest(0, N, D, ps, seed=77, steps=12)
+    pool = S.TrajectoryPool(N, D, ps, slots=6, **SYS)
+    _pool_run(pool, {0: [(alone, 0)]}, 12)
+    pool.close()
+    # the same request in slot 4, admitted at tick 3 next to others that come and go
+    rs = [S.TrajRequest(i, N, D, ps, seed=100 + i, steps=5 + 2 * i) for i in range(5)]
+    mix = S.TrajRequest(9, N, D, ps, seed=77, steps=12)
+    pool = S.TrajectoryPool(N, D, ps, slots=6, **SYS)
+    done = _pool_run(pool, {0: [(rs[0], 0), (rs[1], 1)], 2: [(rs[2], 2)], 3: [(mix, 4), (rs[3], 3)],
+                            6: [(rs[4], 0)]}, 20)
+    pool.close()
+    assert mix in done and alone.result is not None
+    d = np.abs(mix.result - alone.result).max()
+    print("pool alone vs mix max difference", d, flush=True)
+    assert d <= SAME
+    # and the lockstep copy agrees with the trajectory on a plain context
+    single = S.solve_alone(S.TrajRequest(0, N, D, ps, seed=77, steps=12), **SYS)
+    assert np.abs(single.result - alone.result).max() < 1e-10
+
+
+def test_pool_request_matches_oracle():
+    # full bond dimension (N = 6, D = 8): every compression exact
+    N, D, ps, seed, steps = 6, 8, 5, 31, 15
+    r = S.TrajRequest(0, N, D, ps, seed=seed, steps=steps)
+    pool = S.TrajectoryPool(N, D, ps, slots=3, **SYS)
+    _pool_run(pool, {0: [(S.TrajRequest(1, N, D, ps, seed=1, steps=4), 0)], 1: [(r, 2)]}, steps + 1)
+    pool.close()
+    from oracle import mps_oracle as O
+    O.set_qr_rank_tol(S.QR_RANK_TOL)
+    z = models.positions(N, float(N), ps)
+    H = models.spin_rk4_mpos(z, 0.0, SYS["dt"])
+    jr, jl = models.spin_jump_mpos(z)
+    ref = OT.OracleSpinTrajectory(N, D, seed, H, jr, jl, SYS["dt"], SYS["gam_eg"], init=SYS["init"])
+    for _ in range(steps):
+        ref.step()
+    assert ref.jumps, "the case is meant to take jumps"
+    assert np.abs(r.result[:-1] - ref.sigma_z()).max() < 1e-10
+    assert abs(r.result[-1] - ref.p_coh[-1]) < 1e-10
+
+
+def test_pooled_server_poisson_stream():
+    systems = [(8, 8, 1), (10, 8, 2)]
+    reqs = S.poisson_trajectory_requests(14, rate=200.0, seed=6, systems=systems, steps_range=(3, 9))
+    stats = S.PooledServer(slots=4, max_pools=2, **SYS).run(reqs)
+    print(stats, flush=True)
+    assert stats["requests"] == len(reqs) and stats["pools"] == 2
+    assert 0 < stats["p50_latency_s"] <= stats["p99_latency_s"]
+    # every response equals the same request run alone in slot 0 of a fresh pool
+    for key in systems:
+        mine = [r for r in reqs if (r.N, r.D, r.pos_seed) == key]
+        for r in mine:
+            a = S.TrajRequest(r.rid, r.N, r.D, r.pos_seed, r.seed, r.steps)
+            pool = S.TrajectoryPool(*key, slots=4, **SYS)
+            _pool_run(pool, {0: [(a, 0)]}, r.steps)
+            pool.close()
+            assert np.abs(a.result - r.result).max() <= SAME, r.rid
