@@ -73,7 +73,8 @@ static void set_device(Ctx* c) { HIPCHK(hipSetDevice(c->device)); }
 // Entry points that produce per-call host values or touch single buffers run
 // on plain contexts only; a batched context's copies go through views.
 static void no_batch(const Ctx* c) {
-  REQUIRE(c->bat.nb == 1, "not available on a batched context (use a view of one copy)");
+  if (c->bat.nb != 1)
+    throw Error(MPS_E_STATE, "not available on a batched context (use a view of one copy)");
 }
 
 // zero `bytes` at p in every lockstep copy of the context

@@ -63,8 +63,9 @@ def test_ensemble_member_matches_oracle():
 
 def test_batched_context_refuses_single_copy_calls():
     ens = SpinEnsemble(6, 8, 1, [1, 2], dt=0.05)
-    with pytest.raises(M.MPSError):
+    with pytest.raises(M.MPSError) as e:
         M.scal_prod(ens.A, ens.A)
+    assert e.value.code == -4          # MPS_E_STATE
     # the same call on a view of one copy works
     v = ens.Av[0]
     assert abs(M.scal_prod(v, v) - 1.0) < 1e-12
