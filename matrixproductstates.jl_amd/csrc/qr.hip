@@ -841,7 +841,7 @@ __device__ __forceinline__ void wy_apply(double2 (&x)[MAXR], const double2* __re
 //   u_q = conj(tau_q) (w_q - sum_{l<q} conj(G[l][q]) u_l)
 // (lane q of the first wave holds row q; u_l is broadcast as it completes).
 template <int RA, int NB, int NT, bool OWN_G>
-__global__ __launch_bounds__(NT) void qr_apply_col_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void qr_apply_col_kernel(double2* __restrict__ A, int m, int64_t rs, int64_t cs,
                                                           int c0, int nbp, const double2* __restrict__ Vs,
                                                           const double2* __restrict__ Tp,
                                                           const double2* __restrict__ tau, int64_t bs) {
@@ -881,33 +881,32 @@ __global__ __launch_bounds__(NT) void qr_apply_col_kernel(double2* __restrict__ 
     }
   }
   // value e: e < 2 NB: Re/Im of w_{e/2} = v_{e/2}^H x; else Re/Im of pair
-  // p = (e - 2 NB) / 2, G[pair_j(p)][pair_l(p)]
+  // p = (e - 2 NB) / 2, G[pair_j(p)][pair_l(p)].  Rows outermost: the 8
+  // sums of a group advance together, so consecutive fmas are independent
+  // (a sum finished before the next starts is a chain of dependent FP64
+  // latencies at one wave per SIMD).
   static_for<0, NG>([&](auto gc) {
     constexpr int g = decltype(gc)::value;
-    double v8[8];
-    static_for<0, 4>([&](auto hc) {
-      constexpr int h = decltype(hc)::value;
-      constexpr int e = 8 * g + 2 * h;
-      double sr = 0.0, si = 0.0;
-      if constexpr (e < 2 * NB) {
-        constexpr int q = e / 2;
+    double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int i = 0; i < RA; ++i) {
-          sr += v[i][q].x * x[i].x + v[i][q].y * x[i].y;
-          si += v[i][q].x * x[i].y - v[i][q].y * x[i].x;
+    for (int i = 0; i < RA; ++i) {
+      static_for<0, 4>([&](auto hc) {
+        constexpr int h = decltype(hc)::value;
+        constexpr int e = 8 * g + 2 * h;
+        if constexpr (e < 2 * NB) {
+          constexpr int q = e / 2;
+          const double2 a = v[i][q], b = x[i];
+          v8[2 * h] = fma(a.x, b.x, fma(a.y, b.y, v8[2 * h]));
+          v8[2 * h + 1] = fma(a.x, b.y, fma(-a.y, b.x, v8[2 * h + 1]));
+        } else if constexpr (e < NVAL) {
+          constexpr int p = (e - 2 * NB) / 2;
+          constexpr int l = pair_l(p), jj = pair_j(p);
+          const double2 a = v[i][jj], b = v[i][l];
+          v8[2 * h] = fma(a.x, b.x, fma(a.y, b.y, v8[2 * h]));
+          v8[2 * h + 1] = fma(a.x, b.y, fma(-a.y, b.x, v8[2 * h + 1]));
         }
-      } else if constexpr (e < NVAL) {
-        constexpr int p = (e - 2 * NB) / 2;
-        constexpr int l = pair_l(p), jj = pair_j(p);
-#pragma unroll
-        for (int i = 0; i < RA; ++i) {
-          sr += v[i][jj].x * v[i][l].x + v[i][jj].y * v[i][l].y;
-          si += v[i][jj].x * v[i][l].y - v[i][jj].y * v[i][l].x;
-        }
-      }
-      v8[2 * h] = sr;
-      v8[2 * h + 1] = si;
-    });
+      });
+    }
     red[(wave * NG + g) * 64 + lane] = wave_rs8(v8);
   });
   __syncthreads();

@@ -1,0 +1,91 @@
+"""Half-sweeps and RK4 at bond dimensions whose sites use the context's second
+stream (sites of at least 32768 elements: the overlap terms run concurrently
+with the operator chain).  Same FP64 oracle, same 1e-10 bar as the
+single-stream sizes, and the same result as the single-stream schedule that
+kernel timing forces (the sums are accumulated in the same order)."""
+import numpy as np
+import pytest
+
+import mpsamd as M
+from mpsamd import models
+from oracle import mps_oracle as O
+from helpers import rel, copy_sites, random_mpo, crandn
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.mark.parametrize("side", ["left", "right"])
+@pytest.mark.parametrize("numv", [2, 4])
+def test_compress_sum_var_apply_H_two_streams(side, numv, rng):
+    N, D, d, w = 6, 64, 9, 4          # middle sites 64 x 9 x 64 = 36864 elements
+    print("case", side, numv, flush=True)
+    states = [O.mpsrandom(N, D, d, rng)[0] for _ in range(numv)]
+    Ain, dims = O.mpsrandom(N, D, d, rng)
+    if side == "right":
+        O.rightorthonormalizationQR_(Ain)
+    H = random_mpo(rng, N, d, w)
+    coef = list(crandn(rng, numv))
+    wd = [1] + [w] * (N - 1) + [1]
+    A0 = copy_sites(Ain)
+    env = [O.build_env(dims, dims) for _ in range(numv - 1)]
+    envop = O.build_env_op(dims, dims, wd)
+    fn = O.compress_sum_var_apply_H_left_ if side == "left" else O.compress_sum_var_apply_H_right_
+    fn(A0, env, envop, Ain, coef, states, H)
+
+    gfn = M.compress_sum_var_apply_H_left_ if side == "left" else M.compress_sum_var_apply_H_right_
+    ctx = M.default_context()
+    out = []
+    for timed in (False, True):        # kernel timing runs everything on one stream
+        ctx.set_profiling(timed)
+        g_states = [M.MPS.from_host(s) for s in states]
+        g_A0 = M.MPS.from_host(Ain)
+        g_env = [M.build_env(dims, dims) for _ in range(numv - 1)]
+        g_envop = M.build_env(dims, dims, wd)
+        gfn(g_A0, g_env, g_envop, M.MPS.from_host(Ain), coef, g_states, M.MPO(H))
+        out.append(g_A0.to_host())
+        ctx.set_profiling(False)
+    for j in range(N):
+        assert rel(out[0][j], A0[j]) < TOL, j
+        assert np.array_equal(out[0][j], out[1][j]), j
+
+
+@pytest.mark.parametrize("eleven", [False, True])
+def test_rk4_rydberg_two_streams(eleven):
+    N, D, dt, steps = 6, 64, 0.005, 2
+    print("case eleven", eleven, flush=True)
+    p = models.RydbergParams(n_exp=2)
+    z = models.positions(N, 6.0, 7)
+    L = models.rydberg_rk4_mpos(p, z, dt)
+    rho, dims = O.mpsrandom(N, D, 9, np.random.default_rng(11))
+    wop = [1] + [p.w] * (N - 1) + [1]
+    # oracle
+    ref = copy_sites(rho)
+    envop = O.build_env_op(dims, dims, wop)
+    e = [O.build_env(dims, dims) for _ in range(3)]
+    r1, r2, r3 = copy_sites(rho), copy_sites(rho), copy_sites(rho)
+    for _ in range(steps):
+        if eleven:
+            o = copy_sites(ref)
+            O.RK4stp_apply_H_half_(dt, L, ref, r1, r2, r3, *e, envop, psi0=o)
+            ref = o
+        else:
+            O.RK4stp_apply_H_half_(dt, L, ref, r1, r2, r3, *e, envop)
+    # device, two streams
+    gL = [M.MPO(h) for h in L]
+    g = M.MPS.from_host(rho)
+    g1, g2, g3 = (M.MPS.from_host(rho) for _ in range(3))
+    ge = [M.build_env(dims, dims) for _ in range(3)]
+    geop = M.build_env(dims, dims, wop)
+    go = M.MPS.from_host(rho) if eleven else None
+    for _ in range(steps):
+        if eleven:
+            M.RK4stp_apply_H_half_(dt, gL, go, g, g1, g2, g3, *ge, geop)
+            g.copy_from(go)
+        else:
+            M.RK4stp_apply_H_half_(dt, gL, g, g1, g2, g3, *ge, geop)
+    got = g.to_host()
+    for j in range(N):
+        assert rel(got[j], ref[j]) < TOL, j
+    I = models.identity_vec_mps(N)
+    assert abs(M.scal_prod_no_conj(M.MPS.from_host(I), g) - O.scal_prod_no_conj(I, ref)) < TOL
