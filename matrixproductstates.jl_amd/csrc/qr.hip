@@ -379,15 +379,19 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
   // reduced together with the current column's dot products (uniform)
   bool have_next = false;
   double nx_xs = 0.0, nx_ar = 0.0, nx_ai = 0.0;
-  // steps k in [kb, ke) with the panel's live columns in a[.][0..W-1]: once
-  // at most W-1 columns trail column k, the loop body shrinks to width W
-  auto segment = [&](auto Wc, int kb, int ke) {
+  // One column step: column k sits in slot O of a[.][], the columns trailing
+  // it in slots O+1 .. W-1 (zero past the panel) and are updated in place.
+  // Steps go in pairs (O = 0, then 1) and the panel then shifts left by two
+  // slots, so a register shift is paid once per two columns; once at most
+  // W-1 columns trail column k, the step shrinks to width W.
+  auto step = [&](auto Wc, auto Oc, int k) {
     constexpr int W = decltype(Wc)::value;
-    constexpr int GW = (2 * (W - 1) + 7) / 8;
-#pragma unroll 1
-  for (int k = kb; k < ke; ++k) {
+    constexpr int O = decltype(Oc)::value;
+    constexpr int T = W - 1 - O;                 // trailing slots
+    static_assert(T >= 1, "a step needs a trailing slot");
+    constexpr int GW = (2 * T + 7) / 8;
     if (k >= NB) __builtin_unreachable();
-    // zlarfg on column k (= a[.][0]): ||x||^2 below row k and alpha = a(k, k)
+    // zlarfg on column k (= a[.][O]): ||x||^2 below row k and alpha = a(k, k)
     double xs, ar, ai;
     if (have_next) {
       xs = nx_xs;
@@ -397,12 +401,12 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        const double2 x = a[i][0];
+        const double2 x = a[i][O];
         const double n2 = fma(x.x, x.x, x.y * x.y);
         v8[0] += (i > 0 || tid > k) ? n2 : 0.0;
       }
-      v8[1] = (tid == k) ? a[0][0].x : 0.0;
-      v8[2] = (tid == k) ? a[0][0].y : 0.0;
+      v8[1] = (tid == k) ? a[0][O].x : 0.0;
+      v8[2] = (tid == k) ? a[0][O].y : 0.0;
       red[ph][wave * GMAX * 64 + lane] = wave_rs8(v8);
       __syncthreads();
       const double s0 = wave_sum_slots(red[ph], 0);
@@ -418,19 +422,13 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         s_tau[k] = make_double2(0.0, 0.0);
         tau[c0 + k] = make_double2(0.0, 0.0);
       }
-      // column k out unchanged (R part to A; Vs: 1 at row k, zero elsewhere)
-      if (tid <= k) Ak[(int64_t)k * cs] = a[0][0];
+      // column k out unchanged (R part to A; Vs: 1 at row k, zero elsewhere);
+      // the trailing columns stay as they are
+      if (tid <= k) Ak[(int64_t)k * cs] = a[0][O];
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         const int rr = tid + NT * i;
         if (rr < M) Vk[(int64_t)k * m + NT * i] = make_double2(rr == k ? 1.0 : 0.0, 0.0);
-      }
-      // shift the panel: column k+1 becomes a[.][0]
-#pragma unroll
-      for (int i = 0; i < R; ++i) {
-#pragma unroll
-        for (int j = 0; j + 1 < W; ++j) a[i][j] = a[i][j + 1];
-        a[i][W - 1] = make_double2(0.0, 0.0);
       }
     } else {
       // beta = -sign(Re alpha) ||(alpha, x)||, tau = (beta - alpha) / beta,
@@ -453,24 +451,23 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       double2 vm[R];
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        const double2 y = cmul(a[i][0], sc);
+        const double2 y = cmul(a[i][O], sc);
         if (i == 0)
           vm[0] = (tid == k) ? make_double2(1.0, 0.0) : ((tid > k) ? y : make_double2(0.0, 0.0));
         else
           vm[i] = y;
       }
-      if (tid <= k) Ak[(int64_t)k * cs] = (tid == k) ? make_double2(beta, 0.0) : a[0][0];
+      if (tid <= k) Ak[(int64_t)k * cs] = (tid == k) ? make_double2(beta, 0.0) : a[0][O];
 #pragma unroll
       for (int i = 0; i < R; ++i)
         if (tid + NT * i < M) Vk[(int64_t)k * m + NT * i] = vm[i];
-      // s_j = v_k^H a_j for the W-1 columns to the right (zero past the
-      // panel), four columns per butterfly
+      // s_j = v_k^H a_j for the T trailing slots, four columns per butterfly
 #pragma unroll
       for (int g = 0; g < GW; ++g) {
         double v8[8];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
-          const int j = 1 + 4 * g + p;
+          const int j = O + 1 + 4 * g + p;
           double sr = 0.0, si = 0.0;
           if (j < W) {
 #pragma unroll
@@ -485,13 +482,13 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         }
         red[ph][(wave * GMAX + g) * 64 + lane] = wave_rs8(v8);
       }
-      // look-ahead sums for column k+1 (a[.][1] before its update), rows > k+1:
+      // look-ahead sums for column k+1 (a[.][O+1] before its update), rows > k+1:
       // ||a||^2, v_k^H a, ||v_k||^2, and a(k+1), v_k(k+1) from their owner
       {
         double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-          const double2 x = a[i][1], v = vm[i];
+          const double2 x = a[i][O + 1], v = vm[i];
           const double2 xb = (i > 0 || tid > k + 1) ? x : make_double2(0.0, 0.0);
           const double2 vb = (i > 0 || tid > k + 1) ? v : make_double2(0.0, 0.0);
           v8[0] = fma(xb.x, xb.x, fma(xb.y, xb.y, v8[0]));
@@ -500,8 +497,8 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
           v8[3] = fma(vb.x, vb.x, fma(vb.y, vb.y, v8[3]));
         }
         const bool at = (tid == k + 1);
-        v8[4] = at ? a[0][1].x : 0.0;
-        v8[5] = at ? a[0][1].y : 0.0;
+        v8[4] = at ? a[0][O + 1].x : 0.0;
+        v8[5] = at ? a[0][O + 1].y : 0.0;
         v8[6] = at ? vm[0].x : 0.0;
         v8[7] = at ? vm[0].y : 0.0;
         red[ph][(wave * GMAX + GW) * 64 + lane] = wave_rs8(v8);
@@ -512,13 +509,13 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       for (int g = 0; g < GW; ++g) sg[g] = wave_sum_slots(red[ph], g);
       const double la = wave_sum_slots(red[ph], GW);
       ph ^= 1;
-      // a_j <- a_j - conj(tau) v_k s_j, written one column to the left
+      // a_j <- a_j - conj(tau) v_k s_j, in place
 #pragma unroll
-      for (int j = 1; j < W; ++j) {
-        const int f = 2 * (j - 1), g = f / 8, q = f % 8;
+      for (int j = O + 1; j < W; ++j) {
+        const int f = 2 * (j - O - 1), g = f / 8, q = f % 8;
         const double sr = bcast(sg[g], 8 * q), si = bcast(sg[g], 8 * q + 8);
         const double fr = t.x * sr + t.y * si, fi = t.x * si - t.y * sr;
-        if (j == 1 && k + 1 < nbp) {
+        if (j == O + 1 && k + 1 < nbp) {
           // column k+1 after the update: alpha' = a(k+1) - c v_k(k+1) (the
           // owner's update, same operations) and ||x'||^2 =
           // ||a||^2 - 2 Re(conj(c) v_k^H a) + |c|^2 ||v_k||^2, c = conj(tau) s;
@@ -539,12 +536,25 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
           }
         }
 #pragma unroll
-        for (int i = 0; i < R; ++i) a[i][j - 1] = cupd(a[i][j], vm[i], fr, fi);
+        for (int i = 0; i < R; ++i) a[i][j] = cupd(a[i][j], vm[i], fr, fi);
       }
-#pragma unroll
-      for (int i = 0; i < R; ++i) a[i][W - 1] = make_double2(0.0, 0.0);
     }
-  }
+  };
+  auto segment = [&](auto Wc, int kb, int ke) {
+    constexpr int W = decltype(Wc)::value;
+#pragma unroll 1
+    for (int k = kb; k < ke; k += 2) {
+      step(Wc, std::integral_constant<int, 0>{}, k);
+      if (k + 1 < ke) step(Wc, std::integral_constant<int, 1>{}, k + 1);
+      // shift the panel: column k+2 becomes a[.][0]
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+#pragma unroll
+        for (int j = 0; j + 2 < W; ++j) a[i][j] = a[i][j + 2];
+        a[i][W - 2] = make_double2(0.0, 0.0);
+        a[i][W - 1] = make_double2(0.0, 0.0);
+      }
+    }
   };
   if constexpr (NB == 16) {
     segment(std::integral_constant<int, 16>{}, 0, min(nbp, 4));
