@@ -161,19 +161,29 @@ def batch_and_serve(seed):
     return batched, serving
 
 
-def pooled_serving_leg(seed, n=480, rate=120.0, systems=4, N=60, D=32, slots=64):
-    """Trajectory requests over a few systems, continuously batched in lockstep
-    pools (one batched context per system, requests join and leave slots
-    between ticks)."""
+def pooled_serving_leg(seed, n_burst=480, n=320, rate=30.0, systems=4, N=60, D=32, slots=256):
+    """Trajectory requests over a few chains of one shape, continuously batched
+    in a lockstep pool (one batched context, each slot with its own chain's
+    MPOs, requests join and leave slots between ticks; the pool is built
+    before the clock starts).  A burst (every
+    request at t = 0) measures throughput; a Poisson stream below that
+    capacity measures latency."""
     from mpsamd import serving as S
     sy = [(N, D, seed + i) for i in range(systems)]
-    reqs = S.poisson_trajectory_requests(n, rate=rate, seed=seed, systems=sy, steps_range=(10, 40))
-    st = S.PooledServer(slots=slots, max_pools=systems, dt=0.01).run(reqs)
-    print(f"pooled serving: {st}", file=sys.stderr, flush=True)
-    return {"requests": st["requests"], "arrival_rate_req_s": rate, "systems": systems, "slots_per_pool": slots,
-            "mix": f"quantum-jump trajectories of {systems} driven chains (d=2, N={N}, D={D}), 10-40 steps each",
-            "throughput_req_s": round(st["throughput_req_s"], 3),
-            "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
+    out = {"chains": systems, "slots": slots,
+           "mix": f"quantum-jump trajectories of {systems} driven chains (d=2, N={N}, D={D}), 10-40 steps each"}
+    burst = S.poisson_trajectory_requests(n_burst, rate=1e12, seed=seed, systems=sy, steps_range=(10, 40))
+    st = S.PooledServer(slots=slots, dt=0.01).run(burst, warm=[(N, D)])
+    print(f"pooled serving, burst: {st}", file=sys.stderr, flush=True)
+    out["burst"] = {"requests": st["requests"], "throughput_req_s": round(st["throughput_req_s"], 3),
+                    "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
+    reqs = S.poisson_trajectory_requests(n, rate=rate, seed=seed + 1, systems=sy, steps_range=(10, 40))
+    st = S.PooledServer(slots=slots, dt=0.01).run(reqs, warm=[(N, D)])
+    print(f"pooled serving, Poisson {rate}/s: {st}", file=sys.stderr, flush=True)
+    out["poisson"] = {"requests": st["requests"], "arrival_rate_req_s": rate,
+                      "throughput_req_s": round(st["throughput_req_s"], 3),
+                      "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
+    return out
 
 
 def main():

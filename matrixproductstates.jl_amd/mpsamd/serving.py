@@ -18,13 +18,14 @@ reports throughput and p50/p99 latency.  A request's result does not depend
 on what else is running: each solve has its own stream and workspace, and
 every kernel reduces in a fixed order.
 
-Trajectory requests of one system (same chain, same MPOs, same shapes) are
-batched in lockstep instead (``PooledServer``): a pool holds ``slots`` copies
-of the system on one batched context, every tick advances all of them with
-one launch per kernel, and a request joins a free slot as soon as it arrives
-and leaves it as soon as it is done (continuous batching).  Each copy's
-arithmetic reads only its own data, in a fixed order, so a response is the
-same whichever slot it took and whatever else shared the pool.
+Trajectory requests of one shape (chain length N, bond dimension D; the
+chains themselves may differ) are batched in lockstep instead
+(``PooledServer``): a pool holds ``slots`` copies on one batched context, each
+with its own MPOs, every tick advances all of them with one launch per
+kernel, and a request joins a free slot as soon as it arrives and leaves it as
+soon as it is done (continuous batching).  Each copy's arithmetic reads only
+its own data, in a fixed order, so a response is the same whichever slot it
+took and whatever else shared the pool.
 """
 from __future__ import annotations
 
@@ -266,26 +267,37 @@ class TrajRequest:
 
 
 class TrajectoryPool:
-    """Continuous lockstep batching of one system's trajectories.
+    """Continuous lockstep batching of same-shape trajectories.
 
-    ``slots`` copies of the system live on one batched context
-    (``SpinEnsemble``).  A request takes a free slot (its copy reset to the
-    initial product state and its draws to the request's seed); every tick
-    advances all slots with one batched RK4 call and one batched norm, then
-    runs the jump branch of the occupied slots; a finished request's slot is
-    free for the next one at once.  A slot's arithmetic is independent of the
-    other slots, so a response does not depend on what else runs.
+    ``slots`` copies of an (N, D) chain live on one batched context
+    (``SpinEnsemble``); every copy holds its own MPOs, so the slots may follow
+    different chains of that shape.  A request takes a free slot (its copy
+    set to the request's chain, reset to the initial product state, its draws
+    to the request's seed); every tick advances all slots with one batched
+    RK4 call and one batched norm, then runs the jump branch of the occupied
+    slots; a finished request's slot is free for the next one at once.  A
+    slot's arithmetic is independent of the other slots, so a response does
+    not depend on what else runs.
     """
 
-    def __init__(self, N, D, pos_seed, slots=64, **system):
-        self.key = (N, D, pos_seed)
+    def __init__(self, N, D, slots=64, pos_seed=0, **system):
+        self.key = (N, D)
         self.ens = SpinEnsemble(N, D, pos_seed, [0] * slots, qr_rank_tol=QR_RANK_TOL, **system)
         self.req = [None] * slots
 
     def free_slots(self):
         return [b for b, r in enumerate(self.req) if r is None]
 
+    def pick_slot(self, r: TrajRequest, free):
+        """A free slot already on the request's chain if there is one (no MPO
+        upload), else the first free slot."""
+        for b in free:
+            if self.ens.system[b] == r.pos_seed:
+                return b
+        return free[0]
+
     def admit(self, r: TrajRequest, b: int):
+        self.ens.set_system(b, r.pos_seed)
         self.ens.reset(b, r.seed)
         self.req[b] = r
         r.left = r.steps
@@ -332,53 +344,79 @@ def solve_alone(r: TrajRequest, **system):
 
 
 class PooledServer:
-    """Continuous batching of trajectory requests over per-system pools.
+    """Continuous batching of trajectory requests over per-shape pools.
 
     Requests arrive with arrival times; each tick admits the arrived requests
-    into free slots of their system's pool (a pool is created, within
-    ``max_pools``, when a system first arrives), queues one batched step of
+    into free slots of the pool of their shape (N, D) (a pool is created,
+    within ``max_pools``, when a shape first arrives), queues one batched step of
     every busy pool (each pool has its own stream), then collects the norms,
-    runs the jump branches and retires finished requests.  Latency =
-    completion - arrival.  ``system`` (dt, Omega, Gamma, gam_eg, init ...) is
+    runs the jump branches and retires finished requests; a pool's next step
+    is queued as soon as its last one is collected.  A pool with nothing in
+    flight is released when ``max_pools`` are open and another shape
+    arrives.  Latency = completion - arrival.  ``system`` (dt, Omega, Gamma, gam_eg, init ...) is
     shared by every pool.
     """
 
     def __init__(self, slots=64, max_pools=8, **system):
         self.slots, self.max_pools, self.system = slots, max_pools, system
 
-    def run(self, requests):
+    def run(self, requests, warm=()):
+        """Serve ``requests``; pools of the shapes in ``warm`` [(N, D)] are
+        built and stepped twice (capturing their step graphs) before the clock
+        starts, as a running server keeps them."""
         pending = sorted(requests, key=lambda r: r.arrival)
         pools = {}
+        for key in warm:
+            pool = pools[key] = TrajectoryPool(*key, self.slots, **self.system)
+            for _ in range(2):
+                pool.launch()
+                pool.finish()
         done = []
+        inflight = set()
         t0 = time.perf_counter()
-        while pending or any(p.busy() for p in pools.values()):
-            now = time.perf_counter() - t0
+
+        def admit(now):
             keep = []
             for r in pending:
                 if r.arrival > now:
                     keep.append(r)
                     continue
-                key = (r.N, r.D, r.pos_seed)
+                key = (r.N, r.D)
                 pool = pools.get(key)
+                if pool is None and len(pools) >= self.max_pools:
+                    # make room: release a pool with nothing in flight or queued
+                    idle = [k for k, q in pools.items() if k not in inflight and not q.busy()]
+                    if idle:
+                        pools.pop(idle[0]).close()
                 if pool is None and len(pools) < self.max_pools:
-                    pool = pools[key] = TrajectoryPool(r.N, r.D, r.pos_seed, self.slots, **self.system)
-                free = pool.free_slots() if pool is not None else []
+                    pool = pools[key] = TrajectoryPool(r.N, r.D, self.slots, **self.system)
+                # a pool with a step in flight takes requests only once that
+                # step is collected (the step's norms belong to the old slots)
+                free = pool.free_slots() if pool is not None and key not in inflight else []
                 if not free:
                     keep.append(r)
                     continue
                 r.start = now
-                pool.admit(r, free[0])
-            pending = keep
-            busy = [p for p in pools.values() if p.busy()]
-            if not busy:
-                time.sleep(max(0.0, min(pending[0].arrival - now, 0.01)) if pending else 0.0)
-                continue
-            for p in busy:
-                p.launch()
-            for p in busy:
-                for r in p.finish():
-                    r.done = time.perf_counter() - t0
-                    done.append(r)
+                pool.admit(r, pool.pick_slot(r, free))
+            return keep
+
+        # Pools advance independently: a pool's next step is queued as soon as
+        # its previous one is collected, so the other pools' queued steps keep
+        # the GPU busy while the host handles one pool.
+        while pending or inflight:
+            pending = admit(time.perf_counter() - t0)
+            for key, p in list(pools.items()):
+                if key in inflight:
+                    for r in p.finish():
+                        r.done = time.perf_counter() - t0
+                        done.append(r)
+                    inflight.discard(key)
+                    pending = admit(time.perf_counter() - t0)
+                if p.busy():
+                    p.launch()
+                    inflight.add(key)
+            if not inflight and pending:
+                time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
         wall = time.perf_counter() - t0
         for p in pools.values():
             p.close()

@@ -152,10 +152,12 @@ def _mpo_bytes(sites):
 class SpinEnsemble:
     """A lockstep ensemble of B trajectories of one driven spin chain.
 
-    All trajectories share the system (positions from ``pos_seed``, hence the
-    MPOs) and differ in their jump records: trajectory b draws r1, r2 from
-    ``numpy.random.default_rng(seeds[b])`` exactly as ``SpinTrajectory`` with
-    ``seed = seeds[b], pos_seed = pos_seed`` does.  The coherent RK4 step and
+    All trajectories share the shapes (N, D, the MPO bonds); each copy holds
+    its own MPOs, so copies may follow different chains (``set_system``; at
+    creation every copy has positions from ``pos_seed``).  Trajectory b draws
+    r1, r2 from ``numpy.random.default_rng(seeds[b])`` exactly as
+    ``SpinTrajectory`` with ``seed = seeds[b]`` and the copy's ``pos_seed``
+    does.  The coherent RK4 step and
     ``normalize_lo!`` of all B run as one batched call (every GEMM, QR panel
     and reduction launched once for the ensemble, on a batched context); the
     jump branch of a trajectory (spin_model_quantum_trajectory_rk4.jl:140-177)
@@ -165,6 +167,7 @@ class SpinEnsemble:
     def __init__(self, N, D, pos_seed, seeds, dt=0.01, Omega=0.5, Gamma=1.0, k=2 * np.pi, L=None,
                  gam_eg=0.0, init=(1.0, 0.0), qr_rank_tol=0.0, device=0):
         self.N, self.D, self.dt, self.gam_eg = N, D, dt, gam_eg
+        self._model = (float(N if L is None else L), Omega, Gamma, k)
         self.B = B = len(seeds)
         self.z = MD.positions(N, float(N if L is None else L), pos_seed)
         Hs = MD.spin_rk4_mpos(self.z, 0.0, dt, Omega, Gamma, k)
@@ -181,6 +184,8 @@ class SpinEnsemble:
         self.comp.set_qr_rank_tol(qr_rank_tol)
         self.H = [M.MPO(h, self.ctx) for h in Hs]
         self.jr, self.jl = M.MPO(jr, self.comp), M.MPO(jl, self.comp)
+        self._jumps = {pos_seed: (self.jr, self.jl)}     # jump MPOs per chain (companion)
+        self.system = [pos_seed] * B
         self.A, self.dims = M.mpsproductstate(N, D, 2, np.asarray(init, dtype=complex), self.ctx)
         self.A_coh, self.A1, self.A2, self.A3 = (self.A.similar() for _ in range(4))
         for s in (self.A_coh, self.A1, self.A2, self.A3):
@@ -198,6 +203,23 @@ class SpinEnsemble:
         self.p_coh = [[] for _ in range(B)]
         self.jumps = [[] for _ in range(B)]
         self.t = 0.0
+
+    def set_system(self, b, pos_seed):
+        """Copy b follows the chain with positions ``models.positions(N, L,
+        pos_seed)`` from now on (its RK4 MPOs rewritten in place)."""
+        if self.system[b] == pos_seed:
+            return
+        L, Omega, Gamma, k = self._model
+        z = MD.positions(self.N, L, pos_seed)
+        for h_dev, h_host in zip(self.H, MD.spin_rk4_mpos(z, 0.0, self.dt, Omega, Gamma, k)):
+            v = h_dev.view(b, self.comp)
+            for j, site in enumerate(h_host):
+                v.upload_site(j, site)
+            v.__del__()
+        if pos_seed not in self._jumps:
+            jr, jl = MD.spin_jump_mpos(z, Gamma, k)
+            self._jumps[pos_seed] = (M.MPO(jr, self.comp), M.MPO(jl, self.comp))
+        self.system[b] = pos_seed
 
     def reset(self, b, seed):
         """Start a new trajectory in copy b: the initial product state, a fresh
@@ -233,9 +255,10 @@ class SpinEnsemble:
             if r1 < p[b]:
                 continue
             A, Ac = self.Av[b], self.Acv[b]
-            M.compress_var_apply_H_(self.A_r, self.envop_jump, A, A, self.jr, 1)
+            jr, jl = self._jumps[self.system[b]]
+            M.compress_var_apply_H_(self.A_r, self.envop_jump, A, A, jr, 1)
             p_r = M.normalize_lo_(self.A_r)
-            M.compress_var_apply_H_(self.A_l, self.envop_jump, A, A, self.jl, 1)
+            M.compress_var_apply_H_(self.A_l, self.envop_jump, A, A, jl, 1)
             p_l = M.normalize_lo_(self.A_l)
             epop = M.measure_excitations_(None, A, HEE)
             acc = jump_table(epop, p_r, p_l, self.dt, self.gam_eg)
@@ -265,7 +288,10 @@ class SpinEnsemble:
                 v.__del__()
         if hasattr(self, "_init"):
             self._init.__del__()
-        for o in (self.A_r, self.A_l, self.envop_jump, self.jr, self.jl, self.A, self.A_coh, self.A1, self.A2,
+        for jr, jl in self._jumps.values():
+            jr.__del__()
+            jl.__del__()
+        for o in (self.A_r, self.A_l, self.envop_jump, self.A, self.A_coh, self.A1, self.A2,
                   self.A3, self.envop, *self.env, *self.H):
             o.__del__()
         self.comp.close()

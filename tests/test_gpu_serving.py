@@ -81,13 +81,14 @@ def _pool_run(pool, schedule, ticks):
 def test_pool_result_same_alone_and_in_the_mix():
     N, D, ps = 10, 8, 3
     alone = S.TrajRequest(0, N, D, ps, seed=77, steps=12)
-    pool = S.TrajectoryPool(N, D, ps, slots=6, **SYS)
+    pool = S.TrajectoryPool(N, D, slots=6, **SYS)
     _pool_run(pool, {0: [(alone, 0)]}, 12)
     pool.close()
-    # the same request in slot 4, admitted at tick 3 next to others that come and go
-    rs = [S.TrajRequest(i, N, D, ps, seed=100 + i, steps=5 + 2 * i) for i in range(5)]
+    # the same request in slot 4, admitted at tick 3 next to others that come
+    # and go, some on other chains of the same shape
+    rs = [S.TrajRequest(i, N, D, ps if i % 2 else 20 + i, seed=100 + i, steps=5 + 2 * i) for i in range(5)]
     mix = S.TrajRequest(9, N, D, ps, seed=77, steps=12)
-    pool = S.TrajectoryPool(N, D, ps, slots=6, **SYS)
+    pool = S.TrajectoryPool(N, D, slots=6, **SYS)
     done = _pool_run(pool, {0: [(rs[0], 0), (rs[1], 1)], 2: [(rs[2], 2)], 3: [(mix, 4), (rs[3], 3)],
                             6: [(rs[4], 0)]}, 20)
     pool.close()
@@ -104,8 +105,8 @@ def test_pool_request_matches_oracle():
     # full bond dimension (N = 6, D = 8): every compression exact
     N, D, ps, seed, steps = 6, 8, 5, 31, 15
     r = S.TrajRequest(0, N, D, ps, seed=seed, steps=steps)
-    pool = S.TrajectoryPool(N, D, ps, slots=3, **SYS)
-    _pool_run(pool, {0: [(S.TrajRequest(1, N, D, ps, seed=1, steps=4), 0)], 1: [(r, 2)]}, steps + 1)
+    pool = S.TrajectoryPool(N, D, slots=3, **SYS)
+    _pool_run(pool, {0: [(S.TrajRequest(1, N, D, 9, seed=1, steps=4), 0)], 1: [(r, 2)]}, steps + 1)
     pool.close()
     from oracle import mps_oracle as O
     O.set_qr_rank_tol(S.QR_RANK_TOL)
@@ -121,18 +122,18 @@ def test_pool_request_matches_oracle():
 
 
 def test_pooled_server_poisson_stream():
-    systems = [(8, 8, 1), (10, 8, 2)]
-    reqs = S.poisson_trajectory_requests(14, rate=200.0, seed=6, systems=systems, steps_range=(3, 9))
-    stats = S.PooledServer(slots=4, max_pools=2, **SYS).run(reqs)
+    # three chains, two shapes: (8, 8) chains 1 and 5 share a pool
+    systems = [(8, 8, 1), (10, 8, 2), (8, 8, 5)]
+    reqs = S.poisson_trajectory_requests(16, rate=200.0, seed=6, systems=systems, steps_range=(3, 9))
+    stats = S.PooledServer(slots=4, max_pools=2, **SYS).run(reqs, warm=[(8, 8)])
     print(stats, flush=True)
     assert stats["requests"] == len(reqs) and stats["pools"] == 2
     assert 0 < stats["p50_latency_s"] <= stats["p99_latency_s"]
+    assert len({r.pos_seed for r in reqs if (r.N, r.D) == (8, 8)}) == 2
     # every response equals the same request run alone in slot 0 of a fresh pool
-    for key in systems:
-        mine = [r for r in reqs if (r.N, r.D, r.pos_seed) == key]
-        for r in mine:
-            a = S.TrajRequest(r.rid, r.N, r.D, r.pos_seed, r.seed, r.steps)
-            pool = S.TrajectoryPool(*key, slots=4, **SYS)
-            _pool_run(pool, {0: [(a, 0)]}, r.steps)
-            pool.close()
-            assert np.abs(a.result - r.result).max() <= SAME, r.rid
+    for r in reqs:
+        a = S.TrajRequest(r.rid, r.N, r.D, r.pos_seed, r.seed, r.steps)
+        pool = S.TrajectoryPool(r.N, r.D, slots=4, **SYS)
+        _pool_run(pool, {0: [(a, 0)]}, r.steps)
+        pool.close()
+        assert np.abs(a.result - r.result).max() <= SAME, r.rid
