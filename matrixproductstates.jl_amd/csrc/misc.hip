@@ -86,6 +86,36 @@ void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s, Bat bat
   HIPCHK(hipGetLastError());
 }
 
+// d (cols x rows, column major) = the transpose of s (rows x cols, column
+// major), through 32 x 32 LDS tiles so both sides are read and written in
+// contiguous column segments
+__global__ __launch_bounds__(256) void transpose_c_kernel(double2* __restrict__ d, const double2* __restrict__ s,
+                                                          int64_t rows, int64_t cols, int64_t bs) {
+  d = boff(d, blockIdx.z * bs);
+  s = boff(s, blockIdx.z * bs);
+  __shared__ double2 t[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t r0 = (int64_t)blockIdx.x * 32, c0 = (int64_t)blockIdx.y * 32;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = r0 + tx, c = c0 + k;
+    if (r < rows && c < cols) t[k][tx] = s[r + rows * c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t c = c0 + tx, r = r0 + k;
+    if (r < rows && c < cols) d[c + cols * r] = t[tx][k];
+  }
+}
+
+void launch_transpose_c(void* dst, const void* src, int64_t rows, int64_t cols, hipStream_t s, Bat bat) {
+  if (rows <= 0 || cols <= 0) return;
+  REQUIRE((rows + 31) / 32 < (1LL << 31) && (cols + 31) / 32 <= 65535, "transpose too large");
+  const dim3 grid((unsigned)((rows + 31) / 32), (unsigned)((cols + 31) / 32), (unsigned)bat.nb);
+  hipLaunchKernelGGL(transpose_c_kernel, grid, dim3(256), 0, s, (double2*)dst, (const double2*)src, rows, cols,
+                     bat.bs);
+  HIPCHK(hipGetLastError());
+}
+
 template <typename TD, typename TS>
 __global__ void convert_kernel(TD* d, const TS* s, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;

@@ -871,7 +871,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   __shared__ double s_val[NG * 8];
   __shared__ double2 s_u[NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int j = c0 + nbp + blockIdx.x;
+  // Workgroups are dealt to the XCDs round-robin (workgroup x on XCD x % 8)
+  // and the next panel's factorisation, a single workgroup, runs on XCD 0:
+  // where the grid allows, the next panel's columns go to workgroups 0, 8,
+  // 16, ... so they are written through that XCD's L2 (a bijection of
+  // [0, trail) otherwise in order).
+  const int trail = (int)gridDim.x;
+  const int P = min(NB, trail);
+  int col = blockIdx.x;
+  if (8 * (P - 1) < trail) {
+    const int x = blockIdx.x;
+    col = (x % 8 == 0 && x / 8 < P) ? x / 8 : P + x - min(P, (x + 7) / 8);
+  }
+  const int j = c0 + nbp + col;
   const int M = m - c0;
   const double2* V = Vs + (int64_t)c0 * m + c0;   // reflector q (panel-local), row c0 + r: V[r + q m]
   double2 ct = make_double2(0.0, 0.0);

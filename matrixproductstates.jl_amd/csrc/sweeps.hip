@@ -115,7 +115,18 @@ void qr_left(Plan& p, State* s, int j, void* Rout) {
 void qr_right(Plan& p, State* s, int j, void* Rout) {
   const i64 Dl = s->Dl(j), d = s->d[j], Dr = s->Dr(j);
   REQUIRE(d * Dr >= Dl, "rightorth!: d*Dr < Dl");
-  run_qr(p, s->site(j), d * Dr, Dl, Dl, 1, Rout);
+  if (Dl * d * Dr < 32768) {
+    run_qr(p, s->site(j), d * Dr, Dl, Dl, 1, Rout);
+    return;
+  }
+  // Large sites: the transposed view reads and writes whole rows a column
+  // apart (one cache line per element in the QR's column passes), so the
+  // panel is transposed into a column-major copy, factored there, and Q
+  // transposed back (the same arithmetic on the same values).
+  Ctx* c = p.ctx;
+  launch_transpose_c(p.tmp, s->site(j), Dl, d * Dr, c->stream, c->bat);
+  run_qr(p, p.tmp, d * Dr, Dl, 1, d * Dr, Rout);
+  launch_transpose_c(s->site(j), p.tmp, d * Dr, Dl, c->stream, c->bat);
 }
 
 // leftorthogonalizeQR! (orthonormalisation.jl:43-56); norm^2 left in dscal[slot]

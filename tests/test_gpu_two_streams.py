@@ -89,3 +89,18 @@ def test_rk4_rydberg_two_streams(eleven):
         assert rel(got[j], ref[j]) < TOL, j
     I = models.identity_vec_mps(N)
     assert abs(M.scal_prod_no_conj(M.MPS.from_host(I), g) - O.scal_prod_no_conj(I, ref)) < TOL
+
+
+def test_right_gauge_moves_large_sites(rng):
+    # rightorth! on sites of at least 32768 elements factors a transposed,
+    # column-major copy of the panel: every site against the oracle
+    N, D, d = 5, 64, 9
+    a, dims = O.mpsrandom(N, D, d, rng)
+    a = [s + 0.1 * crandn(rng, s.shape) for s in a]
+    ref = copy_sites(a)
+    no_ref = O.rightorthonormalizationQR_(ref)
+    g = M.MPS.from_host(a)
+    assert abs(M.rightorthonormalizationQR_(g) - no_ref) < TOL * no_ref
+    got = g.to_host()
+    for j in range(N):
+        assert rel(got[j], ref[j]) < TOL, j
