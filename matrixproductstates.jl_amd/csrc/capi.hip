@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 
 using namespace mpship;
@@ -32,6 +33,9 @@ struct mps_ctx_s {
   std::map<Key, hipGraphExec_t> graphs;
   std::map<Key, int> seen;
   size_t ws_gen = 0;
+  // second operator environment for the RK4 stages whose environments are
+  // built on the aux stream (same slots as the caller's envop)
+  std::unique_ptr<Env> shadow;
   void clear_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     graphs.clear();
@@ -1222,6 +1226,24 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     size_t before = c->ws_bytes;
     Plan p(c, {&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s}, {&H1->o, &H2->o, &H3->o}, 4);
     track_ws(x, before);
+    if (p.conc) {
+      const Env& like = envop->e;
+      if (!x->shadow || x->shadow->N != like.N || x->shadow->off != like.off || x->shadow->total != like.total) {
+        x->shadow.reset();
+        auto e = std::make_unique<Env>();
+        e->ctx = c;
+        e->N = like.N;
+        e->op = like.op;
+        e->d1 = like.d1;
+        e->wd = like.wd;
+        e->d2 = like.d2;
+        e->off = like.off;
+        e->total = like.total;
+        e->buf = (double2*)c->alloc(like.total * sizeof(double2), e->own);
+        x->shadow = std::move(e);
+      }
+      p.shadow = x->shadow.get();
+    }
     key.ws_gen = x->ws_gen;
     // Asynchronous on the context's stream (no host value is produced): the
     // caller's next call on this context, or a scalar-returning call, orders

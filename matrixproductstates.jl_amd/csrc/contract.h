@@ -32,6 +32,11 @@ struct Ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void ensure_side();
+  // third stream: builds the next RK4 stage's operator environments site by
+  // site behind the current sweep (default priority)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_site = nullptr, ev_aux = nullptr;
+  void ensure_aux();
   double* dscal = nullptr;      // 256 doubles of device scalars
   double* hscal = nullptr;      // pinned host mirror
   bool capturing = false;

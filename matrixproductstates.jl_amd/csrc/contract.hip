@@ -38,6 +38,14 @@ void Ctx::ensure_side() {
   HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
 }
 
+void Ctx::ensure_aux() {
+  if (aux) return;
+  REQUIRE(!capturing, "aux stream creation during graph capture");
+  HIPCHK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ev_site, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_aux, hipEventDisableTiming));
+}
+
 void* Ctx::alloc(size_t bytes, bool& owned) {
   bytes = (bytes + 255) & ~(size_t)255;
   if (slab != nullptr) {
@@ -58,6 +66,10 @@ Ctx::~Ctx() {
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (side) (void)hipStreamDestroy(side);
+  if (aux) (void)hipStreamSynchronize(aux);
+  if (ev_site) (void)hipEventDestroy(ev_site);
+  if (ev_aux) (void)hipEventDestroy(ev_aux);
+  if (aux) (void)hipStreamDestroy(aux);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
   if (slab) {
     (void)hipStreamSynchronize(stream);

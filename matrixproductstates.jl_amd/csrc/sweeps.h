@@ -31,6 +31,13 @@ struct Plan {
   void* splitk2 = nullptr;
   void fork();
   void join();
+  // The aux stream (conc plans): RK4 stages 2-4 find their operator
+  // environments built by it during the previous stage, into `shadow` or the
+  // caller's envop alternately (see rk4_half).  `aux_w` has its own T1 and
+  // split-K partials, `Ta` holds its products.
+  Work aux_w{};
+  void *Ta = nullptr, *splitk3 = nullptr;
+  Env* shadow = nullptr;
   Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
        int n_ovl, size_t extra_bytes = 0);
   Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl, 0); }
@@ -46,11 +53,24 @@ void left_orthogonalize(Plan& p, State* s, int slot);
 void right_orthogonalize(Plan& p, State* s, int slot);
 double read_scalar(Ctx* c, int slot);
 
-void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H);
+// The next sweep's operator-environment build, queued on the aux stream
+// behind this sweep: after site j of this sweep's output is final, the
+// environment step of the next sweep that reads it (update_re! for a right
+// sweep, update_le! for a left one, with that sweep's A = Ain = the output).
+struct NextInit {
+  Env* e;                       // the next sweep's envop
+  MPO* H;                       // the next sweep's operator
+  std::complex<double> coef;    // its far-end boundary coefficient
+};
+
+void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,
+                               const NextInit* nx = nullptr);
 void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
-                                   const std::complex<double>* coef, State** A, MPO* H);
+                                   const std::complex<double>* coef, State** A, MPO* H,
+                                   const NextInit* nx = nullptr, bool init_done = false);
 void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
-                                    const std::complex<double>* coef, State** A, MPO* H);
+                                    const std::complex<double>* coef, State** A, MPO* H,
+                                    const NextInit* nx = nullptr, bool init_done = false);
 void compress_var_apply_H(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H, int maxsweeps);
 void compress_var(Plan& p, State* A0, State* A, Env* env, int maxsweeps);
 void compress_sum_var(Plan& p, State* out, State* Ain, int numv, const std::complex<double>* coef,

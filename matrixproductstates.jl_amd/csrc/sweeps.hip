@@ -54,6 +54,8 @@ static void plan_layout(Plan* p, i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra
   add(p ? &p->envB : nullptr, big * cz);
   add(p ? &p->splitk : nullptr, splitk_bound(num_cus, nb));
   if (conc) add(p ? &p->splitk2 : nullptr, splitk_bound(num_cus, nb));
+  if (conc) add(p ? &p->splitk3 : nullptr, splitk_bound(num_cus, nb));
+  if (conc) add(p ? &p->Ta : nullptr, big * cz);
   if (extra_bytes) add(p ? &p->extra : nullptr, extra_bytes);
 }
 
@@ -72,7 +74,10 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
   // GPU (measured: D = 128, d = 9 gains; bond-16/32 chains served many to a
   // GPU lose, their streams then outnumber the hardware queues)
   conc = n_ovl > 0 && !c->profiling && site >= 32768 && c->bat.nb == 1;
-  if (conc) c->ensure_side();
+  if (conc) {
+    c->ensure_side();
+    c->ensure_aux();
+  }
   plan_layout(this, Dm, dm, wm, n_ovl, extra_bytes, c->num_cus, c->bat.nb, conc,
               [&](void** slot, size_t b) { scr.add(slot, b); });
   splitk_bytes = splitk_bound(c->num_cus, c->bat.nb);
@@ -86,6 +91,10 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
     side.splitk = splitk2;
     side.T1 = nullptr;   // the overlap terms need no T1
     side.stream = c->side;
+    aux_w = work;
+    aux_w.splitk = splitk3;
+    aux_w.T1 = T3;
+    aux_w.stream = c->aux;
   }
   op.cmp = (c->prec == 1) ? CMP_F32_3M : CMP_F64;
   op.tmp = (c->prec == 1) ? F32 : F64;
@@ -220,10 +229,67 @@ static void init_right_op(Plan& p, Env* envop, State* Ain, State* A, MPO* H, cd 
   set1(c, envop->slot(0), left_coef);
 }
 
+// ------------------------------------- next sweep's environments (aux stream)
+// The aux stream continues from the main stream's current point / the main
+// stream waits for everything queued on the aux stream.
+static void aux_follow(Plan& p) {
+  Ctx* c = p.ctx;
+  HIPCHK(hipEventRecord(c->ev_site, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->aux, c->ev_site, 0));
+}
+static void aux_wait(Plan& p) {
+  Ctx* c = p.ctx;
+  HIPCHK(hipEventRecord(c->ev_aux, c->aux));
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_aux, 0));
+}
+
+// After a left sweep's QR of site j (A0[j] final): init_right_op's step for
+// site j of the next (right) sweep, whose A and Ain are this sweep's A0.
+static void nx_right_begin(Plan& p, const NextInit* nx, int N) {
+  if (!nx) return;
+  aux_follow(p);   // nx->e is no longer in use by earlier work on the main stream
+  set1(p.aux_w, nx->e->slot(N), 1.0);
+}
+static void nx_right_step(Plan& p, const NextInit* nx, State* A0, int j) {
+  if (!nx) return;
+  aux_follow(p);
+  Env* e = nx->e;
+  MPO* H = nx->H;
+  prod_AOR(p.aux_w, p.op, p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->site(j), H->wl(j), H->wr(j),
+           e->slot(j + 1), A0->Dr(j));
+  update_re(p.aux_w, p.op, e->slot(j), p.Ta, A0->site(j), A0->Dl(j), H->wl(j), A0->d[j], A0->Dr(j),
+            A0->Dl(j));
+}
+static void nx_right_end(Plan& p, const NextInit* nx) {
+  if (nx) set1(p.aux_w, nx->e->slot(0), nx->coef);
+}
+// After a right sweep's QR of site j: init_left_op's step for site j of the
+// next (left) sweep.
+static void nx_left_begin(Plan& p, const NextInit* nx) {
+  if (!nx) return;
+  aux_follow(p);
+  set1(p.aux_w, nx->e->slot(0), 1.0);
+}
+static void nx_left_step(Plan& p, const NextInit* nx, State* A0, int j) {
+  if (!nx) return;
+  aux_follow(p);
+  Env* e = nx->e;
+  MPO* H = nx->H;
+  prod_AOL(p.aux_w, p.op, p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->site(j), H->wl(j), H->wr(j),
+           e->slot(j), A0->Dl(j));
+  update_le(p.aux_w, p.op, e->slot(j + 1), p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->wr(j),
+            A0->Dr(j));
+}
+static void nx_left_end(Plan& p, const NextInit* nx, int N) {
+  if (nx) set1(p.aux_w, nx->e->slot(N), nx->coef);
+}
+
 // compress_var_apply_H_left! (compression.jl:321-347)
-void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H) {
+void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,
+                               const NextInit* nx) {
   const int N = Ain->N;
   init_left_op(p, envop, Ain, A, H, 1.0);
+  nx_right_begin(p, nx, N);
   for (int j = N - 1; j >= 0; --j) {
     // AOR = prod_AOR(A[jj], H[jj], envop[jj+1]); prod_LR!(A0[jj], envop[jj], AOR)
     prod_AOR(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
@@ -232,16 +298,18 @@ void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State
                A0->Dr(j), false);
     if (j == 0) break;
     qr_right(p, A0, j, nullptr);
+    nx_right_step(p, nx, A0, j);
     update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), A->Dl(j), H->wl(j), A->d[j], A0->Dr(j),
               A0->Dl(j));
   }
+  nx_right_end(p, nx);
 }
 
 // compress_sum_var_apply_H_left! (compression.jl:889-944).  The overlap
 // terms (environment builds, per-site products, environment updates) run on
 // the side stream; the operator term joins them before accumulating into A0.
 void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
-                                   const cd* coef, State** A, MPO* H) {
+                                   const cd* coef, State** A, MPO* H, const NextInit* nx, bool init_done) {
   const int N = Ain->N;
   Work& ow = p.side;
   p.fork();
@@ -258,7 +326,11 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     set1(ow, e->slot(N), coef[ll]);
   }
   State* Ah = A[numv - 1];
-  init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  if (init_done)
+    aux_wait(p);   // envop built on the aux stream during the previous sweep
+  else
+    init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  nx_right_begin(p, nx, N);
   for (int j = N - 1; j >= 0; --j) {
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
     // overlap terms: AR = prod_LR(A[ll][jj], env[ll][jj+1]); A0 (+)= prod_LR(env[ll][jj], AR)
@@ -279,15 +351,17 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     p.fork();
     for (int ll = 0; ll + 1 < numv; ++ll)
       update_renv(ow, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
+    nx_right_step(p, nx, A0, j);
     update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
   }
+  nx_right_end(p, nx);
   p.join();
 }
 
 // compress_sum_var_apply_H_right! (compression.jl:1007-1063).  A0 may alias
 // A[0] (both touched only by the side stream's overlap chain, in order).
 void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
-                                    const cd* coef, State** A, MPO* H) {
+                                    const cd* coef, State** A, MPO* H, const NextInit* nx, bool init_done) {
   const int N = Ain->N;
   Work& ow = p.side;
   p.fork();
@@ -304,7 +378,11 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     set1(ow, e->slot(0), coef[ll]);
   }
   State* Ah = A[numv - 1];
-  init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  if (init_done)
+    aux_wait(p);   // envop built on the aux stream during the previous sweep
+  else
+    init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  nx_left_begin(p, nx);
   for (int j = 0; j < N; ++j) {
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
     // overlap terms first: AL[ll] reads A[ll][jj] before A0[jj] is written
@@ -324,8 +402,10 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     p.fork();
     for (int ll = 0; ll + 1 < numv; ++ll)
       update_lenv(ow, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
+    nx_left_step(p, nx, A0, j);
     update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
   }
+  nx_left_end(p, nx, N);
   p.join();
 }
 
@@ -518,26 +598,33 @@ void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const cd
 }
 
 // RK4stp_apply_H_half! (timeevolution.jl:178-196)
+//
+// With the aux stream (p.shadow set): stages 2-4 do not build their operator
+// environments between sweeps; the aux stream builds each one behind the
+// previous sweep, site by site as that sweep's output becomes final, into
+// the shadow environment (stages 2, 4) or envop (stage 3).  The steps and
+// their order per environment are those of init_{left,right}_op.
 void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
               State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop) {
-  compress_var_apply_H_left(p, psi1, envop, psi, psi, H1);
+  const bool ahead = p.conc && p.shadow != nullptr;
+  Env* E2 = ahead ? p.shadow : envop;   // stages 2 and 4
+  const cd c2[2] = {1.0, dt / 2}, c3[2] = {1.0, dt}, c4[4] = {-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3};
+  const NextInit n2{E2, H2, c2[1]}, n3{envop, H2, c3[1]}, n4{E2, H3, c4[3]};
+  compress_var_apply_H_left(p, psi1, envop, psi, psi, H1, ahead ? &n2 : nullptr);
   {
     Env* e[1] = {env1};
     State* A[2] = {psi, psi1};
-    cd coef[2] = {1.0, dt / 2};
-    compress_sum_var_apply_H_right(p, psi2, e, envop, psi1, 2, coef, A, H2);
+    compress_sum_var_apply_H_right(p, psi2, e, E2, psi1, 2, c2, A, H2, ahead ? &n3 : nullptr, ahead);
   }
   {
     Env* e[1] = {env1};
     State* A[2] = {psi, psi2};
-    cd coef[2] = {1.0, dt};
-    compress_sum_var_apply_H_left(p, psi3, e, envop, psi2, 2, coef, A, H2);
+    compress_sum_var_apply_H_left(p, psi3, e, envop, psi2, 2, c3, A, H2, ahead ? &n4 : nullptr, ahead);
   }
   {
     Env* e[3] = {env1, env2, env3};
     State* A[4] = {psi, psi1, psi2, psi3};
-    cd coef[4] = {-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3};
-    compress_sum_var_apply_H_right(p, out, e, envop, psi3, 4, coef, A, H3);
+    compress_sum_var_apply_H_right(p, out, e, E2, psi3, 4, c4, A, H3, nullptr, ahead);
   }
 }
 
