@@ -36,7 +36,7 @@ struct Plan {
   // caller's envop alternately (see rk4_half).  `aux_w` has its own T1 and
   // split-K partials, `Ta` holds its products.
   Work aux_w{};
-  void *Ta = nullptr, *splitk3 = nullptr;
+  void *Ta = nullptr, *ALa = nullptr, *splitk3 = nullptr;
   Env* shadow = nullptr;
   Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
        int n_ovl, size_t extra_bytes = 0);
@@ -61,6 +61,12 @@ struct NextInit {
   Env* e;                       // the next sweep's envop
   MPO* H;                       // the next sweep's operator
   std::complex<double> coef;    // its far-end boundary coefficient
+  // overlap environments of the next sweep built the same way: oe[k]
+  // between oA[k] and this sweep's output, far-end coefficient ocoef[k]
+  int n_ovl = 0;
+  Env* oe[3] = {nullptr, nullptr, nullptr};
+  State* oA[3] = {nullptr, nullptr, nullptr};
+  std::complex<double> ocoef[3] = {};
 };
 
 void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,

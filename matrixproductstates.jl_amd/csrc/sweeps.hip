@@ -56,6 +56,7 @@ static void plan_layout(Plan* p, i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra
   if (conc) add(p ? &p->splitk2 : nullptr, splitk_bound(num_cus, nb));
   if (conc) add(p ? &p->splitk3 : nullptr, splitk_bound(num_cus, nb));
   if (conc) add(p ? &p->Ta : nullptr, big * cz);
+  if (conc) add(p ? &p->ALa : nullptr, site * cz);
   if (extra_bytes) add(p ? &p->extra : nullptr, extra_bytes);
 }
 
@@ -247,8 +248,9 @@ static void aux_wait(Plan& p) {
 // site j of the next (right) sweep, whose A and Ain are this sweep's A0.
 static void nx_right_begin(Plan& p, const NextInit* nx, int N) {
   if (!nx) return;
-  aux_follow(p);   // nx->e is no longer in use by earlier work on the main stream
+  aux_follow(p);   // nx's environments are no longer in use by earlier work on the main stream
   set1(p.aux_w, nx->e->slot(N), 1.0);
+  for (int k = 0; k < nx->n_ovl; ++k) set1(p.aux_w, nx->oe[k]->slot(N), 1.0);
 }
 static void nx_right_step(Plan& p, const NextInit* nx, State* A0, int j) {
   if (!nx) return;
@@ -259,9 +261,18 @@ static void nx_right_step(Plan& p, const NextInit* nx, State* A0, int j) {
            e->slot(j + 1), A0->Dr(j));
   update_re(p.aux_w, p.op, e->slot(j), p.Ta, A0->site(j), A0->Dl(j), H->wl(j), A0->d[j], A0->Dr(j),
             A0->Dl(j));
+  // update_renv!(oe[j], A0[j], oA[j], oe[j+1]), as the next sweep's own build
+  for (int k = 0; k < nx->n_ovl; ++k) {
+    State* B = nx->oA[k];
+    prod_LR_32(p.aux_w, p.ALa, B->site(j), nx->oe[k]->slot(j + 1), B->Dl(j), B->d[j], B->Dr(j), A0->Dr(j),
+               false);
+    update_renv(p.aux_w, nx->oe[k]->slot(j), A0->site(j), p.ALa, A0->Dl(j), A0->d[j], A0->Dr(j), B->Dl(j));
+  }
 }
 static void nx_right_end(Plan& p, const NextInit* nx) {
-  if (nx) set1(p.aux_w, nx->e->slot(0), nx->coef);
+  if (!nx) return;
+  set1(p.aux_w, nx->e->slot(0), nx->coef);
+  for (int k = 0; k < nx->n_ovl; ++k) set1(p.aux_w, nx->oe[k]->slot(0), nx->ocoef[k]);
 }
 // After a right sweep's QR of site j: init_left_op's step for site j of the
 // next (left) sweep.
@@ -269,6 +280,7 @@ static void nx_left_begin(Plan& p, const NextInit* nx) {
   if (!nx) return;
   aux_follow(p);
   set1(p.aux_w, nx->e->slot(0), 1.0);
+  for (int k = 0; k < nx->n_ovl; ++k) set1(p.aux_w, nx->oe[k]->slot(0), 1.0);
 }
 static void nx_left_step(Plan& p, const NextInit* nx, State* A0, int j) {
   if (!nx) return;
@@ -279,9 +291,19 @@ static void nx_left_step(Plan& p, const NextInit* nx, State* A0, int j) {
            e->slot(j), A0->Dl(j));
   update_le(p.aux_w, p.op, e->slot(j + 1), p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->wr(j),
             A0->Dr(j));
+  // update_lenv!(oe[j+1], A0[j], oA[j], oe[j]), as the next sweep's own build
+  for (int k = 0; k < nx->n_ovl; ++k) {
+    State* B = nx->oA[k];
+    prod_LR_23(p.aux_w, p.ALa, nx->oe[k]->slot(j), B->site(j), A0->Dl(j), B->Dl(j), B->d[j], B->Dr(j), false,
+               false);
+    update_lenv(p.aux_w, nx->oe[k]->slot(j + 1), A0->site(j), p.ALa, A0->Dl(j), A0->d[j], A0->Dr(j), B->Dr(j),
+                0);
+  }
 }
 static void nx_left_end(Plan& p, const NextInit* nx, int N) {
-  if (nx) set1(p.aux_w, nx->e->slot(N), nx->coef);
+  if (!nx) return;
+  set1(p.aux_w, nx->e->slot(N), nx->coef);
+  for (int k = 0; k < nx->n_ovl; ++k) set1(p.aux_w, nx->oe[k]->slot(N), nx->ocoef[k]);
 }
 
 // compress_var_apply_H_left! (compression.jl:321-347)
@@ -312,8 +334,9 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
                                    const cd* coef, State** A, MPO* H, const NextInit* nx, bool init_done) {
   const int N = Ain->N;
   Work& ow = p.side;
+  if (init_done) aux_wait(p);   // environments built on the aux stream during the previous sweep
   p.fork();
-  for (int ll = 0; ll + 1 < numv; ++ll) {
+  for (int ll = 0; ll + 1 < numv && !init_done; ++ll) {
     Env* e = env[ll];
     set1(ow, e->slot(0), 1.0);
     for (int j = 0; j + 1 < N; ++j) {
@@ -326,10 +349,7 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     set1(ow, e->slot(N), coef[ll]);
   }
   State* Ah = A[numv - 1];
-  if (init_done)
-    aux_wait(p);   // envop built on the aux stream during the previous sweep
-  else
-    init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  if (!init_done) init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
   nx_right_begin(p, nx, N);
   for (int j = N - 1; j >= 0; --j) {
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
@@ -364,8 +384,9 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
                                     const cd* coef, State** A, MPO* H, const NextInit* nx, bool init_done) {
   const int N = Ain->N;
   Work& ow = p.side;
+  if (init_done) aux_wait(p);   // environments built on the aux stream during the previous sweep
   p.fork();
-  for (int ll = 0; ll + 1 < numv; ++ll) {
+  for (int ll = 0; ll + 1 < numv && !init_done; ++ll) {
     Env* e = env[ll];
     set1(ow, e->slot(N), 1.0);
     for (int j = N - 1; j >= 1; --j) {
@@ -378,10 +399,7 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     set1(ow, e->slot(0), coef[ll]);
   }
   State* Ah = A[numv - 1];
-  if (init_done)
-    aux_wait(p);   // envop built on the aux stream during the previous sweep
-  else
-    init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
+  if (!init_done) init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
   nx_left_begin(p, nx);
   for (int j = 0; j < N; ++j) {
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
@@ -609,7 +627,23 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
   const bool ahead = p.conc && p.shadow != nullptr;
   Env* E2 = ahead ? p.shadow : envop;   // stages 2 and 4
   const cd c2[2] = {1.0, dt / 2}, c3[2] = {1.0, dt}, c4[4] = {-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3};
-  const NextInit n2{E2, H2, c2[1]}, n3{envop, H2, c3[1]}, n4{E2, H3, c4[3]};
+  // Overlap environments: stage 2's env1 (psi vs psi1) is built behind stage
+  // 1; stage 3's env1 (psi vs psi2, coefficient 1 at both ends) is what
+  // stage 2's own environment updates leave in env1, operation for
+  // operation; stage 4's env1 (psi vs psi3) is what stage 3 leaves, up to the
+  // coefficient in slot 0; env2 and env3 are built behind stage 3.
+  NextInit n2{E2, H2, c2[1]}, n3{envop, H2, c3[1]}, n4{E2, H3, c4[3]};
+  n2.n_ovl = 1;
+  n2.oe[0] = env1;
+  n2.oA[0] = psi;
+  n2.ocoef[0] = c2[0];
+  n4.n_ovl = 2;
+  n4.oe[0] = env2;
+  n4.oA[0] = psi1;
+  n4.ocoef[0] = c4[1];
+  n4.oe[1] = env3;
+  n4.oA[1] = psi2;
+  n4.ocoef[1] = c4[2];
   compress_var_apply_H_left(p, psi1, envop, psi, psi, H1, ahead ? &n2 : nullptr);
   {
     Env* e[1] = {env1};
@@ -624,6 +658,7 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
   {
     Env* e[3] = {env1, env2, env3};
     State* A[4] = {psi, psi1, psi2, psi3};
+    if (ahead) set1(p.ctx, env1->slot(0), c4[0]);
     compress_sum_var_apply_H_right(p, out, e, E2, psi3, 4, c4, A, H3, nullptr, ahead);
   }
 }
