@@ -88,6 +88,7 @@ void Ctx::ensure_ws(size_t bytes) {
   REQUIRE(!capturing, "workspace growth during graph capture");
   HIPCHK(hipStreamSynchronize(stream));
   if (side) HIPCHK(hipStreamSynchronize(side));
+  if (aux) HIPCHK(hipStreamSynchronize(aux));
   if (ws) HIPCHK(hipFree(ws));
   ws = nullptr;
   size_t nb = bytes + bytes / 4 + (1 << 20);

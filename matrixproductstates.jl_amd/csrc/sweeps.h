@@ -70,7 +70,7 @@ struct NextInit {
 };
 
 void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,
-                               const NextInit* nx = nullptr);
+                               const NextInit* nx = nullptr, bool init_done = false);
 void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int numv,
                                    const std::complex<double>* coef, State** A, MPO* H,
                                    const NextInit* nx = nullptr, bool init_done = false);
@@ -83,8 +83,12 @@ void compress_sum_var(Plan& p, State* out, State* Ain, int numv, const std::comp
                       State** A, int maxsweeps, std::vector<Env*>& FL);
 void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const std::complex<double>* coef,
                               State** A, MPO* H, int maxsweeps, std::vector<Env*>& env);
+// s1_done: stage 1's operator environments are already in envop (built
+// behind the previous call's last sweep); next: build them for a following
+// call with psi = out, the same H1 and envop, behind this call's last sweep
 void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
-              State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop);
+              State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop, bool s1_done = false,
+              bool next = false);
 void scal_prod(Plan& p, State* A, State* B, int mode, int slot);
 void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot);
 

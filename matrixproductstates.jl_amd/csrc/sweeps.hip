@@ -308,9 +308,12 @@ static void nx_left_end(Plan& p, const NextInit* nx, int N) {
 
 // compress_var_apply_H_left! (compression.jl:321-347)
 void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,
-                               const NextInit* nx) {
+                               const NextInit* nx, bool init_done) {
   const int N = Ain->N;
-  init_left_op(p, envop, Ain, A, H, 1.0);
+  if (init_done)
+    aux_wait(p);
+  else
+    init_left_op(p, envop, Ain, A, H, 1.0);
   nx_right_begin(p, nx, N);
   for (int j = N - 1; j >= 0; --j) {
     // AOR = prod_AOR(A[jj], H[jj], envop[jj+1]); prod_LR!(A0[jj], envop[jj], AOR)
@@ -623,7 +626,7 @@ void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const cd
 // the shadow environment (stages 2, 4) or envop (stage 3).  The steps and
 // their order per environment are those of init_{left,right}_op.
 void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
-              State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop) {
+              State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop, bool s1_done, bool next) {
   const bool ahead = p.conc && p.shadow != nullptr;
   Env* E2 = ahead ? p.shadow : envop;   // stages 2 and 4
   const cd c2[2] = {1.0, dt / 2}, c3[2] = {1.0, dt}, c4[4] = {-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3};
@@ -644,7 +647,7 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
   n4.oe[1] = env3;
   n4.oA[1] = psi2;
   n4.ocoef[1] = c4[2];
-  compress_var_apply_H_left(p, psi1, envop, psi, psi, H1, ahead ? &n2 : nullptr);
+  compress_var_apply_H_left(p, psi1, envop, psi, psi, H1, ahead ? &n2 : nullptr, ahead && s1_done);
   {
     Env* e[1] = {env1};
     State* A[2] = {psi, psi1};
@@ -659,8 +662,12 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
     Env* e[3] = {env1, env2, env3};
     State* A[4] = {psi, psi1, psi2, psi3};
     if (ahead) set1(p.ctx, env1->slot(0), c4[0]);
-    compress_sum_var_apply_H_right(p, out, e, E2, psi3, 4, c4, A, H3, nullptr, ahead);
+    // a following call's stage 1 (init_left_op on out, H1, far-end
+    // coefficient 1) into envop, which stage 4 leaves alone
+    const NextInit n1{envop, H1, 1.0};
+    compress_sum_var_apply_H_right(p, out, e, E2, psi3, 4, c4, A, H3, (ahead && next) ? &n1 : nullptr, ahead);
   }
+  if (ahead && next) aux_wait(p);   // the call's work is complete on the context's stream
 }
 
 // ------------------------------------------------------ scalar products

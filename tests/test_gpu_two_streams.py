@@ -104,3 +104,48 @@ def test_right_gauge_moves_large_sites(rng):
     got = g.to_host()
     for j in range(N):
         assert rel(got[j], ref[j]) < TOL, j
+
+
+def test_rk4_consecutive_calls_reuse_and_invalidation():
+    # Each RK4 call at these sizes queues the next call's first environments
+    # behind its last sweep; they are used only when nothing was written in
+    # between.  Sequence: step, read-only query (kept), step, a write to the
+    # state (normalize_lo!: dropped), step, a different envop (dropped), step.
+    N, D, dt = 6, 64, 0.005
+    p = models.RydbergParams(n_exp=2)
+    z = models.positions(N, 6.0, 5)
+    L = models.rydberg_rk4_mpos(p, z, dt)
+    rho, dims = O.mpsrandom(N, D, 9, np.random.default_rng(21))
+    wop = [1] + [p.w] * (N - 1) + [1]
+    ref = copy_sites(rho)
+    envop = O.build_env_op(dims, dims, wop)
+    e = [O.build_env(dims, dims) for _ in range(3)]
+    r1, r2, r3 = copy_sites(rho), copy_sites(rho), copy_sites(rho)
+    gL = [M.MPO(h) for h in L]
+    g = M.MPS.from_host(rho)
+    g1, g2, g3 = (M.MPS.from_host(rho) for _ in range(3))
+    ge = [M.build_env(dims, dims) for _ in range(3)]
+    geop = M.build_env(dims, dims, wop)
+    geop2 = M.build_env(dims, dims, wop)
+    I = models.identity_vec_mps(N)
+    gI = M.MPS.from_host(I)
+
+    def both(envop_dev):
+        O.RK4stp_apply_H_half_(dt, L, ref, r1, r2, r3, *e, envop)
+        M.RK4stp_apply_H_half_(dt, gL, g, g1, g2, g3, *ge, envop_dev)
+        got = g.to_host()
+        for j in range(N):
+            assert rel(got[j], ref[j]) < TOL, j
+
+    both(geop)
+    assert abs(M.scal_prod_no_conj(gI, g) - O.scal_prod_no_conj(I, ref)) < TOL
+    both(geop)
+    assert abs(M.normalize_lo_(g) - O.normalize_lo_(ref)) < TOL
+    both(geop)
+    # a write to site 1 changes every left environment past it
+    op = np.eye(9, dtype=complex) + 0.3 * crandn(np.random.default_rng(3), (9, 9))
+    ref[1] = O.apply_site_operator_(ref[1], op)
+    M.apply_site_operator_(g, 1, op)
+    both(geop)
+    both(geop2)
+    both(geop2)
