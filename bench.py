@@ -253,10 +253,16 @@ def main():
     idm = M.MPS.from_host(MD.identity_vec_mps(args.N), ctx)
     nem = M.MPS.from_host(MD.sum_local_vec_mps(args.N, MD.HEE3), ctx)
     nrm = M.MPS.from_host(MD.sum_local_vec_mps(args.N, MD.HSS3), ctx)
+    pem = M.MPS.from_host(MD.pair_sum_vec_mps(args.N, MD.HEE3), ctx)
 
     def observables(r):
+        # total populations and the pair correlation of the excited level,
+        # g2 = 2 sum_{i<j} <n_i n_j> / (sum_i <n_i>)^2
         tr_ = M.scal_prod_no_conj(idm, r)
-        return {"trace": tr_, "n_e": M.scal_prod_no_conj(nem, r) / tr_, "n_r": M.scal_prod_no_conj(nrm, r) / tr_}
+        n_e = M.scal_prod_no_conj(nem, r) / tr_
+        pairs = M.scal_prod_no_conj(pem, r) / tr_
+        return {"trace": tr_, "n_e": n_e, "n_r": M.scal_prod_no_conj(nrm, r) / tr_,
+                "g2_e": (2.0 * pairs / n_e ** 2).real}
 
     tr = M.scal_prod_no_conj(idm, rho)
     ref_obs = observables(rho)
@@ -309,6 +315,7 @@ def main():
                 "value": round(args.steps / el, 4), "ms_per_step": round(1e3 * el / args.steps, 3),
                 "speedup_vs_fp64": round((args.steps / el) / sweeps, 3),
                 "err_n_e": abs(ob["n_e"] - ref_obs["n_e"]), "err_n_r": abs(ob["n_r"] - ref_obs["n_r"]),
+                "rel_err_g2_e": abs(ob["g2_e"] - ref_obs["g2_e"]) / abs(ref_obs["g2_e"]),
                 "infidelity": 1.0 - fidelity(rho, rho64)}
             print(f"{form}: {prec_forms[form]}", file=sys.stderr, flush=True)
         rho.upload(rho64)
@@ -361,7 +368,9 @@ def main():
                                f"w={p.w} ({args.n_exp} exponentials), D_max={args.D}, dt={dt}, "
                                f"RK4stp_apply_H_half! from mpsgroundstate",
                    "seed": args.seed, "model_gflop_per_step": round(model_flops(dims, 9, p.w) / 1e9, 1),
-                   "trace_rho": [round(tr.real, 12), round(tr.imag, 12)]},
+                   "trace_rho": [round(tr.real, 12), round(tr.imag, 12)],
+                   "observables": {"n_e": ref_obs["n_e"].real, "n_r": ref_obs["n_r"].real,
+                                   "g2_e": ref_obs["g2_e"]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "profile": prof,

@@ -324,3 +324,23 @@ def sum_local_vec_mps(N, op, d=3):
     if N == 1:
         return [(i * 0 + o).reshape(1, dd, 1)]
     return [first] + [bulk.copy() for _ in range(N - 2)] + [last]
+
+
+def pair_sum_vec_mps(N, op, d=3):
+    """Vectorised sum_{i<j} op_i op_j (bond dimension 3: the bond index counts
+    the factors of op placed so far, 0, 1 or 2), so that ``scal_prod_no_conj``
+    with a vectorised density matrix gives sum_{i<j} tr(op_i op_j rho); with
+    ``sum_local_vec_mps`` this gives the equal-time pair correlation
+    g2 = 2 tr(rho) sum_{i<j} <op_i op_j> / (sum_i <op_i>)^2 (N >= 2)."""
+    if N < 2:
+        raise ValueError("pair sums need N >= 2")
+    i = np.eye(d, dtype=CDT).reshape(-1, order="F")
+    o = np.asarray(op, dtype=CDT).reshape(-1, order="F")
+    dd = d * d
+    first = np.zeros((1, dd, 3), dtype=CDT)
+    first[0, :, 0], first[0, :, 1] = i, o
+    last = np.zeros((3, dd, 1), dtype=CDT)
+    last[1, :, 0], last[2, :, 0] = o, i
+    bulk = np.zeros((3, dd, 3), dtype=CDT)
+    bulk[0, :, 0], bulk[0, :, 1], bulk[1, :, 1], bulk[1, :, 2], bulk[2, :, 2] = i, o, i, o, i
+    return [first] + [bulk.copy() for _ in range(N - 2)] + [last]

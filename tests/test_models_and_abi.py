@@ -119,3 +119,33 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     from mpsamd import _lib
     assert set(names) == set(_lib.SIGNATURES) | {"mps_last_error"}
+
+
+def _mps_vector(mps):
+    """Full vector of a bond-index MPS [Dl, dd, Dr] (first site's index slowest)."""
+    v = mps[0][0]                                   # (dd, Dr)
+    for s in mps[1:]:
+        v = np.einsum("xa,ayb->xyb", v, s).reshape(-1, s.shape[2])
+    return v[:, 0]
+
+
+def test_vectorised_sum_and_pair_sum_observables():
+    # against the dense sums built from Kronecker products of vec(op) and
+    # vec(1): sum_j op_j and sum_{i<j} op_i op_j (N = 4, d = 3)
+    N, d = 4, 3
+    rng = np.random.default_rng(2)
+    op = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    i = np.eye(d, dtype=complex).reshape(-1, order="F")
+    o = op.reshape(-1, order="F")
+
+    def kron_of(factors):
+        v = np.ones(1, dtype=complex)
+        for f in factors:
+            v = np.kron(v, f)
+        return v
+
+    single = sum(kron_of([o if k == j else i for k in range(N)]) for j in range(N))
+    pairs = sum(kron_of([o if k in (a, b) else i for k in range(N)])
+                for a in range(N) for b in range(a + 1, N))
+    assert np.allclose(_mps_vector(models.sum_local_vec_mps(N, op)), single, atol=1e-12)
+    assert np.allclose(_mps_vector(models.pair_sum_vec_mps(N, op)), pairs, atol=1e-12)
