@@ -33,3 +33,28 @@ def test_trajectory_matches_oracle(N, D, seed, gam_eg, tol):
     assert len(tr.jumps) >= 1
     assert np.abs(tr.sigma_z() - ref.sigma_z()).max() < 1e-9
     M.default_context().set_qr_rank_tol(0.0)
+
+
+def test_trajectory_two_streams_matches_oracle():
+    # N = 16 at D = 128: the middle sites (128 x 2 x 128) run the RK4 step on
+    # three streams; the state is normalised and may jump between steps, so
+    # every step starts from environments rebuilt for the new state
+    N, D, seed, gam_eg, tol, dt, steps = 16, 128, 7, 0.3, 1e-13, 0.05, 8
+    import mpsamd as M
+    from oracle import mps_oracle as O
+    M.default_context().set_qr_rank_tol(tol)
+    O.set_qr_rank_tol(tol)
+    tr = SpinTrajectory(N, D, seed, dt=dt, gam_eg=gam_eg, init=(0.0, 1.0))
+    H = models.spin_rk4_mpos(tr.z, 0.0, dt)
+    jr, jl = models.spin_jump_mpos(tr.z)
+    ref = OT.OracleSpinTrajectory(N, D, seed, H, jr, jl, dt, gam_eg, init=(0.0, 1.0))
+    try:
+        for i in range(steps):
+            p = tr.step()
+            pr = ref.step()
+            print("step", i, p, pr, flush=True)
+            assert abs(p - pr) < 1e-9 * max(1.0, abs(pr)), (i, p, pr)
+            assert tr.jumps == ref.jumps, i
+        assert np.abs(tr.sigma_z() - ref.sigma_z()).max() < 1e-9
+    finally:
+        M.default_context().set_qr_rank_tol(0.0)

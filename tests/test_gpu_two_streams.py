@@ -50,12 +50,12 @@ def test_compress_sum_var_apply_H_two_streams(side, numv, rng):
         assert np.array_equal(out[0][j], out[1][j]), j
 
 
-@pytest.mark.parametrize("eleven", [False, True])
-def test_rk4_rydberg_two_streams(eleven):
-    N, D, dt, steps = 6, 64, 0.005, 2
-    print("case eleven", eleven, flush=True)
+@pytest.mark.parametrize("N,eleven", [(6, False), (6, True), (12, False)])
+def test_rk4_rydberg_two_streams(N, eleven):
+    D, dt, steps = 64, 0.005, 2
+    print("case", N, "eleven", eleven, flush=True)
     p = models.RydbergParams(n_exp=2)
-    z = models.positions(N, 6.0, 7)
+    z = models.positions(N, float(N), 7)
     L = models.rydberg_rk4_mpos(p, z, dt)
     rho, dims = O.mpsrandom(N, D, 9, np.random.default_rng(11))
     wop = [1] + [p.w] * (N - 1) + [1]
