@@ -149,3 +149,56 @@ def test_rk4_consecutive_calls_reuse_and_invalidation():
     both(geop)
     both(geop2)
     both(geop2)
+
+
+def test_rk4_fp32_two_streams():
+    # the precision form on the three-stream path: consecutive calls (the
+    # second reuses environments built behind the first) stay within the FP32
+    # level of the FP64 oracle and above FP64 round-off
+    N, D, dt, steps = 6, 64, 0.005, 2
+    p = models.RydbergParams(n_exp=2)
+    z = models.positions(N, float(N), 7)
+    L = models.rydberg_rk4_mpos(p, z, dt)
+    rho, dims = O.mpsrandom(N, D, 9, np.random.default_rng(13))
+    wop = [1] + [p.w] * (N - 1) + [1]
+    ref = copy_sites(rho)
+    envop = O.build_env_op(dims, dims, wop)
+    e = [O.build_env(dims, dims) for _ in range(3)]
+    r1, r2, r3 = copy_sites(rho), copy_sites(rho), copy_sites(rho)
+    for _ in range(steps):
+        O.RK4stp_apply_H_half_(dt, L, ref, r1, r2, r3, *e, envop)
+    ctx = M.default_context()
+    ctx.set_precision("fp32")
+    try:
+        gL = [M.MPO(h) for h in L]
+        g = M.MPS.from_host(rho)
+        g1, g2, g3 = (M.MPS.from_host(rho) for _ in range(3))
+        ge = [M.build_env(dims, dims) for _ in range(3)]
+        geop = M.build_env(dims, dims, wop)
+        for _ in range(steps):
+            M.RK4stp_apply_H_half_(dt, gL, g, g1, g2, g3, *ge, geop)
+        got = g.to_host()
+    finally:
+        ctx.set_precision("fp64")
+    err = max(rel(got[j], ref[j]) for j in range(N))
+    print("fp32 two-stream relative site error", err, flush=True)
+    assert 1e-12 < err < 1e-5
+
+
+def test_arnoldi_two_streams(rng):
+    # Arnoldi_expm! at D = 64, d = 9 (its variational sums on two streams)
+    N, D, d, numv = 5, 64, 9, 3
+    psi, dims = O.mpsrandom(N, D, d, rng)
+    psi = [s.real.astype(complex) for s in psi]
+    O.leftorthonormalizationQR_(psi)
+    H = [np.real(h).astype(complex) * 0.3 for h in random_mpo(rng, N, d, 2)]
+    wd = [1] + [2] * (N - 1) + [1]
+    rhov = [copy_sites(psi) for _ in range(numv)]
+    envop = O.build_env_op(dims, dims, wd)
+    cref = O.Arnoldi_expm_(H, rhov, numv, None, envop)
+    grhov = [M.MPS.from_host(psi) for _ in range(numv)]
+    cgot = M.Arnoldi_expm_(M.MPO(H), grhov, numv, None, M.build_env(dims, dims, wd))
+    assert np.abs(cgot - cref).max() < 1e-9
+    got = grhov[0].to_host()
+    for j in range(N):
+        assert rel(got[j], rhov[0][j]) < 1e-9, j
