@@ -224,8 +224,15 @@ int mps_normalize_lo_batch(mps_state_t s, double* norm2);
  * same context, downloads and scalar-returning calls order after it.  Small
  * sites run, from the second call with a given set of handles on, as one
  * captured graph; sites of 32768 elements or more run eagerly, their overlap
- * terms on a second stream of the context that is joined back into the
- * context's stream before the call returns. */
+ * terms on a second stream of the context, and the environment builds of
+ * stages 2-4 on a third, behind the previous stage's sweep, into envop and a
+ * second operator environment the context keeps; both streams are joined
+ * back into the context's stream before the call returns.  Such a call also
+ * builds, behind its last sweep, the stage-1 environments of a following
+ * call with psi = this call's result and the same H1 and envop; that call
+ * uses them only if no entry point that writes a state, MPO or environment
+ * ran in between.  envop is scratch: its content after the call is not part
+ * of the result. */
 int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
                             mps_state_t psi0, mps_state_t psi, mps_state_t psi1,
                             mps_state_t psi2, mps_state_t psi3, mps_env_t env1,
