@@ -58,3 +58,30 @@ def test_trajectory_two_streams_matches_oracle():
         assert np.abs(tr.sigma_z() - ref.sigma_z()).max() < 1e-9
     finally:
         M.default_context().set_qr_rank_tol(0.0)
+
+
+def test_config_spin_chain_n20_d32_100_steps():
+    # the CPU-runnable spin-chain case at its own size: N = 20, D = 32, L = 20,
+    # Omega = 0.5, Gamma = 1, k = 2 pi, dt = 0.01, 100 steps from |g...g>;
+    # the truncated product-state start needs the same rank cutoff on both sides
+    N, D, seed, dt, steps, tol = 20, 32, 1, 0.01, 100, 1e-13
+    import mpsamd as M
+    from oracle import mps_oracle as O
+    M.default_context().set_qr_rank_tol(tol)
+    O.set_qr_rank_tol(tol)
+    try:
+        tr = SpinTrajectory(N, D, seed, dt=dt, L=20.0)
+        H = models.spin_rk4_mpos(tr.z, 0.0, dt)
+        jr, jl = models.spin_jump_mpos(tr.z)
+        ref = OT.OracleSpinTrajectory(N, D, seed, H, jr, jl, dt)
+        worst = 0.0
+        for i in range(steps):
+            p = tr.step()
+            pr = ref.step()
+            worst = max(worst, abs(p - pr))
+            assert abs(p - pr) < 1e-9, (i, p, pr)
+            assert tr.jumps == ref.jumps, i
+        print("max |p_coh - oracle| over 100 steps", worst, "jumps", tr.jumps, flush=True)
+        assert np.abs(tr.sigma_z() - ref.sigma_z()).max() < 1e-9
+    finally:
+        M.default_context().set_qr_rank_tol(0.0)
