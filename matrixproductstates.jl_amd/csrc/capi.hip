@@ -937,12 +937,36 @@ static void check_mpo(const MPO& o, const State& s) {
   for (int j = 0; j < s.N; ++j) REQUIRE(o.d[j] == s.d[j], "MPO local dims differ");
 }
 
+// The H-applying sweeps, RK4 and Arnoldi index every state of a call with one
+// set of bonds, as the reference builds them all from one dims vector; states
+// of other bonds are rejected before anything is queued.
+static void check_same_bonds(std::initializer_list<const State*> v) {
+  const State* f = *v.begin();
+  for (const State* s : v) REQUIRE(s->dims == f->dims, "states differ in bond dimensions");
+}
+
+// Slot j of an environment receives the [a.dims[j], w[j], b.dims[j]] block
+// (w = 1 for overlaps) that a sweep pairing a with b writes there; env_create
+// gave it max(d1, d2)^2 wd elements.  A slot too small for the states' bonds
+// is an error here rather than a write past the slot.
+static void check_env_fits(const Env& e, const State& a, const State& b, const MPO* H) {
+  REQUIRE(e.N == a.N && e.N == b.N, "environment length differs from chain length");
+  for (int j = 0; j <= e.N; ++j) {
+    const i64 big = std::max(e.d1[j], e.d2[j]);
+    const i64 need = a.dims[j] * (H ? H->w[j] : 1) * b.dims[j];
+    REQUIRE(need <= big * e.wd[j] * big,
+            "environment slot " + std::to_string(j) + " is smaller than the states' bonds need");
+  }
+}
+
 int mps_compress_var_apply_H_left(mps_state_t A0, mps_env_t envop, mps_state_t Ain, mps_state_t A,
                                   mps_mpo_t H) {
   return guard([&] {
     check_states({&A0->s, &Ain->s, &A->s});
     check_op_env(envop->e, A0->s);
     check_mpo(H->o, A0->s);
+    check_same_bonds({&A0->s, &Ain->s, &A->s});
+    check_env_fits(envop->e, A0->s, A0->s, &H->o);
     Ctx* c = A0->s.ctx;
     set_device(c);
     size_t before = c->ws_bytes;
@@ -972,9 +996,13 @@ static void sum_var_common(mps_state_t A0, const mps_env_t* env, mps_env_t envop
   check_states({&A0->s, &Ain->s});
   check_op_env(envop->e, A0->s);
   check_mpo(H->o, A0->s);
+  check_same_bonds({&A0->s, &Ain->s});
+  for (int l = 0; l < numv; ++l) check_same_bonds({&A0->s, As[l]});
+  for (int l = 0; l + 1 < numv; ++l) check_env_fits(*Es[l], A0->s, A0->s, nullptr);
+  check_env_fits(envop->e, A0->s, A0->s, &H->o);
   Ctx* c = A0->s.ctx;
   set_device(c);
-  i64 Dm = std::max(A0->s.max_dim(), Ain->s.max_dim()), dm = 1;
+  i64 Dm =std::max(A0->s.max_dim(), Ain->s.max_dim()), dm = 1;
   for (int l = 0; l < numv; ++l) Dm = std::max(Dm, As[l]->max_dim());
   for (int j = 0; j < A0->s.N; ++j) dm = std::max(dm, A0->s.d[j]);
   size_t before = c->ws_bytes;
@@ -1008,6 +1036,8 @@ int mps_compress_var_apply_H(mps_state_t A0, mps_env_t envop, mps_state_t Ain, m
     check_op_env(envop->e, A0->s);
     check_mpo(H->o, A0->s);
     REQUIRE(maxsweeps >= 0, "maxsweeps >= 0");
+    check_same_bonds({&A0->s, &Ain->s, &A->s});
+    check_env_fits(envop->e, A0->s, A0->s, &H->o);
     Ctx* c = A0->s.ctx;
     set_device(c);
     size_t before = c->ws_bytes;
@@ -1024,6 +1054,7 @@ int mps_compress_var(mps_state_t A0, mps_state_t A, mps_env_t env, int maxsweeps
     no_batch(A0->s.ctx);
     check_ovl_env(env->e, A0->s);
     REQUIRE(maxsweeps >= 0, "maxsweeps >= 0");
+    check_env_fits(env->e, A0->s, A->s, nullptr);
     Ctx* c = A0->s.ctx;
     set_device(c);
     size_t before = c->ws_bytes;
@@ -1067,6 +1098,7 @@ int mps_compress_sum_var(mps_state_t A0out, mps_state_t Ain, int numv, const dou
       As[l] = &A[l]->s;
       cf[l] = cd(coef[2 * l], coef[2 * l + 1]);
       check_states({&A0out->s, As[l]});
+      check_same_bonds({&A0out->s, &Ain->s, As[l]});
       if (A[l] == A0out) alias = true;
     }
     Ctx* c = A0out->s.ctx;
@@ -1110,10 +1142,12 @@ int mps_compress_sum_var_apply_H(mps_state_t A0, mps_env_t envop, int numv, cons
       As[l] = &A[l]->s;
       cf[l] = cd(coef[2 * l], coef[2 * l + 1]);
       check_states({&A0->s, As[l]});
+      check_same_bonds({&A0->s, As[l]});
       REQUIRE(A[l] != A0, "compress_sum_var_apply_H!: A0 must not be one of A");
     }
     check_op_env(envop->e, A0->s);
     check_mpo(H->o, A0->s);
+    check_env_fits(envop->e, A0->s, A0->s, &H->o);
     Ctx* c = A0->s.ctx;
     set_device(c);
     TmpEnvs te;
@@ -1243,6 +1277,9 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     for (auto e : {env1, env2, env3}) check_ovl_env(e->e, psi->s);
     check_op_env(envop->e, psi->s);
     for (auto h : {H1, H2, H3}) check_mpo(h->o, psi->s);
+    check_same_bonds({&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s});
+    for (auto e : {env1, env2, env3}) check_env_fits(e->e, psi->s, psi->s, nullptr);
+    for (auto h : {H1, H2, H3}) check_env_fits(envop->e, psi->s, psi->s, &h->o);
     mps_ctx_s* x = psi->owner;
     Ctx* c = &x->c;
     set_device(c);
@@ -1513,6 +1550,8 @@ int mps_arnoldi_expm(mps_mpo_t A, int numv, const mps_state_t* rhov, mps_env_t /
     for (int l = 0; l < numv; ++l) check_states({&rhov[0]->s, &rhov[l]->s});
     check_mpo(A->o, rhov[0]->s);
     check_op_env(envop->e, rhov[0]->s);
+    for (int l = 0; l < numv; ++l) check_same_bonds({&rhov[0]->s, &rhov[l]->s});
+    check_env_fits(envop->e, rhov[0]->s, rhov[0]->s, &A->o);
     std::vector<cd> N((size_t)numv * numv, 0.0), H((size_t)numv * numv, 0.0);
     for (int i = 0; i < numv; ++i) N[i + numv * i] = 1.0;
     auto sop = [&](mps_state_t a, mps_state_t b) {
