@@ -15,7 +15,13 @@
  *   - sites and environment slots are 0-based: site j (Julia jj = j+1) has its
  *     left environment in slot j and its right environment in slot j+1;
  *   - every function returns 0 on success, a negative MPS_E_* code otherwise;
- *     mps_last_error() gives a message (thread-local).
+ *     mps_last_error() gives a message (thread-local);
+ *   - shapes are checked on the host before anything is queued: a chain
+ *     length, local dim or bond that does not fit, an environment slot smaller
+ *     than the states' bonds need, or (in the H-applying sweeps, RK4 and
+ *     Arnoldi, whose states the reference builds from one dims vector) states
+ *     of differing bonds return MPS_E_ARG (the reference's DimensionMismatch)
+ *     and leave every argument unchanged.
  *
  * All state (MPS sites, MPOs, environments) is device resident and owned by
  * the library.  Calls on one context are ordered on the context's HIP stream;
