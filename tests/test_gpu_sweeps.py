@@ -289,6 +289,28 @@ def test_compress_var_and_sum_var(rng):
     assert rel(Dn.mps_to_dense(got.to_host()), Dn.mps_to_dense(ref)) < TOL
 
 
+@pytest.mark.parametrize("N,D0,D,d", [(8, 4, 12, 2), (6, 3, 9, 3)])
+def test_compress_var_to_smaller_bonds(N, D0, D, d, rng):
+    # the fit compress_var! is for: a guess of smaller bond dimension fitted
+    # to a larger target; the environment pairs the two dims vectors
+    print("case", N, D0, D, d, flush=True)
+    target, dims = O.mpsrandom(N, D, d, rng)
+    guess, dims0 = O.mpsrandom(N, D0, d, rng)
+    assert max(dims0) < max(dims)
+    A0 = copy_sites(guess)
+    O.compress_var_(A0, target, O.build_env(dims0, dims), 3)
+    gA0, gt = dev_state(guess), dev_state(target)
+    M.compress_var_(gA0, gt, M.build_env(dims0, dims), 3)
+    assert list(gA0.dims) == list(dims0)
+    got = gA0.to_host()
+    assert rel(Dn.mps_to_dense(got), Dn.mps_to_dense(A0)) < TOL
+    # the fit raised the overlap with the target above the guess's
+    t = Dn.mps_to_dense(target)
+    ov = lambda s: abs(np.vdot(Dn.mps_to_dense(s), t)) / np.linalg.norm(Dn.mps_to_dense(s))
+    assert ov(got) > ov(guess)
+    assert abs(M.scal_prod(gA0, gt) - np.vdot(Dn.mps_to_dense(A0), t)) < TOL * np.linalg.norm(t) * 10
+
+
 def test_compress_var_apply_H_full_sweeps(rng):
     N, D, d = 6, 8, 2
     psi, dims = O.mpsrandom(N, D, d, rng)
