@@ -18,9 +18,9 @@ reports throughput and p50/p99 latency.  A request's result does not depend
 on what else is running: each solve has its own stream and workspace, and
 every kernel reduces in a fixed order.
 
-Trajectory requests of one shape (chain length N, bond dimension D; the
-chains themselves may differ) are batched in lockstep instead
-(``PooledServer``): a pool holds ``slots`` copies on one batched context, each
+Requests of one kind and shape (spin trajectories or density matrices; chain
+length N, bond dimension D; the chains themselves may differ) are batched in
+lockstep instead (``PooledServer``): a pool holds ``slots`` copies on one batched context, each
 with its own MPOs, every tick advances all of them with one launch per
 kernel, and a request joins a free slot as soon as it arrives and leaves it as
 soon as it is done (continuous batching).  Each copy's arithmetic reads only
@@ -264,6 +264,10 @@ class TrajRequest:
     done: float = 0.0
     result: np.ndarray | None = None
     left: int = 0
+    # "spin": quantum-jump trajectory (d = 2), result [<sigma_z^j>..., p_coh];
+    # "rydberg": density-matrix evolution (d = 9, no draws: seed unused),
+    # result [trace, Rydberg-level population / trace]
+    kind: str = "spin"
 
 
 class TrajectoryPool:
@@ -281,7 +285,7 @@ class TrajectoryPool:
     """
 
     def __init__(self, N, D, slots=64, pos_seed=0, **system):
-        self.key = (N, D)
+        self.key = ("spin", N, D)
         self.ens = SpinEnsemble(N, D, pos_seed, [0] * slots, qr_rank_tol=QR_RANK_TOL, **system)
         self.req = [None] * slots
 
@@ -327,14 +331,126 @@ class TrajectoryPool:
         self.ens.close()
 
 
+class DensityMatrixPool:
+    """Continuous lockstep batching of same-shape density-matrix evolutions.
+
+    ``slots`` copies of the vectorised Rydberg-ladder density matrix (N sites,
+    d = 9, bond D, from ``mpsgroundstate``) live on one batched context, each
+    with its own three RK4 Lindblad MPOs, so the slots may follow chains of
+    different positions.  Every tick advances all copies with one batched
+    ``RK4stp_apply_H_half!`` (rydberg_density_matrix_gpu.jl:211-222, the
+    10-argument form); a request takes a free slot (its MPOs set to the
+    request's chain, its copy reset to the ground state) and, after its
+    ``steps``, reads the trace and the Rydberg-level population of its copy through
+    a view.  ``RydbergEvolution`` is the same evolution on a plain context.
+    """
+
+    def __init__(self, N, D, slots=32, pos_seed=0, dt=0.005, n_exp=2, device=0):
+        from .trajectory import _env_bytes, _mpo_bytes, _state_bytes
+        self.key = ("rydberg", N, D)
+        self.N, self.D, self.dt = N, D, dt
+        self.p = MD.RydbergParams(n_exp=n_exp)
+        Ls = MD.rydberg_rk4_mpos(self.p, MD.positions(N, float(N), pos_seed), dt)
+        dims = M._dims_rule(N, D, 9)
+        w = max(h.shape[0] for hs in Ls for h in hs)
+        copy_bytes = (4 * (_state_bytes(dims, 9) + 256) + 3 * (_env_bytes(dims, 1) + 256)
+                      + _env_bytes(dims, w) + 256 + sum(_mpo_bytes(h) + 256 for h in Ls) + 256 * 64)
+        ws = M.batch_workspace_bytes(slots, max(dims), 9, w, device)
+        self.ctx = M.Context.batched(slots, copy_bytes, ws, device)
+        self.ctx.set_qr_rank_tol(QR_RANK_TOL)
+        self.comp = self.ctx.companion()
+        self.comp.set_qr_rank_tol(QR_RANK_TOL)
+        self.L = [M.MPO(h, self.ctx) for h in Ls]
+        self.rho, self.dims = M.mpsgroundstate(N, D, 9, self.ctx)
+        self.r = [self.rho.similar() for _ in range(3)]
+        for s in self.r:
+            s.copy_from(self.rho)
+        self.env = [M.build_env(self.dims, self.dims, ctx=self.ctx) for _ in range(3)]
+        self.envop = M.build_env(self.dims, self.dims, [1] + [w] * (N - 1) + [1], ctx=self.ctx)
+        self._init, _ = M.mpsgroundstate(N, D, 9, self.comp)
+        self.idm = M.MPS.from_host(MD.identity_vec_mps(N), self.comp)
+        self.nrm = M.MPS.from_host(MD.sum_local_vec_mps(N, MD.HSS3), self.comp)
+        self.rv = [self.rho.view(b, self.comp) for b in range(slots)]
+        self.system = [pos_seed] * slots
+        self.req = [None] * slots
+
+    def set_system(self, b, pos_seed):
+        """Copy b follows the chain with positions ``models.positions(N, N,
+        pos_seed)`` from now on (its RK4 MPOs rewritten in place)."""
+        if self.system[b] == pos_seed:
+            return
+        Ls = MD.rydberg_rk4_mpos(self.p, MD.positions(self.N, float(self.N), pos_seed), self.dt)
+        for h_dev, h_host in zip(self.L, Ls):
+            v = h_dev.view(b, self.comp)
+            for j, site in enumerate(h_host):
+                v.upload_site(j, site)
+            v.__del__()
+        self.system[b] = pos_seed
+
+    def free_slots(self):
+        return [b for b, r in enumerate(self.req) if r is None]
+
+    def pick_slot(self, r: TrajRequest, free):
+        for b in free:
+            if self.system[b] == r.pos_seed:
+                return b
+        return free[0]
+
+    def admit(self, r: TrajRequest, b: int):
+        self.set_system(b, r.pos_seed)
+        self.rv[b].copy_from(self._init)
+        self.req[b] = r
+        r.left = r.steps
+
+    def busy(self):
+        return any(r is not None for r in self.req)
+
+    def launch(self):
+        """Queue the batched RK4 step of every copy (no host wait)."""
+        M.RK4stp_apply_H_half_(self.dt, self.L, self.rho, *self.r, *self.env, self.envop)
+
+    def result(self, b):
+        tr = M.scal_prod_no_conj(self.idm, self.rv[b])
+        return np.array([tr.real, (M.scal_prod_no_conj(self.nrm, self.rv[b]) / tr).real])
+
+    def finish(self):
+        """Wait for the step; retire the requests that are done."""
+        self.ctx.synchronize()
+        out = []
+        for b, r in enumerate(self.req):
+            if r is None:
+                continue
+            r.left -= 1
+            if r.left == 0:
+                r.result = self.result(b)
+                self.req[b] = None
+                out.append(r)
+        return out
+
+    def close(self):
+        for v in self.rv:
+            v.__del__()
+        for o in (self._init, self.idm, self.nrm, self.rho, *self.r, *self.env, self.envop, *self.L):
+            o.__del__()
+        self.comp.close()
+        self.ctx.close()
+
+
 def solve_alone(r: TrajRequest, **system):
-    """The same trajectory on its own plain context (the sequential baseline)."""
+    """The same request on its own plain context (the sequential baseline)."""
     ctx = M.Context(0)
     ctx.set_qr_rank_tol(QR_RANK_TOL)
-    tr = SpinTrajectory(r.N, r.D, r.seed, ctx=ctx, pos_seed=r.pos_seed, **system)
-    for _ in range(r.steps):
-        tr.step()
-    r.result = np.concatenate([tr.sigma_z(), [tr.p_coh[-1]]])
+    if r.kind == "rydberg":
+        tr = RydbergEvolution(r.N, r.D, r.pos_seed, ctx=ctx)
+        for _ in range(r.steps):
+            tr.launch_coherent()
+            tr.finish_step()
+        r.result = tr.result()
+    else:
+        tr = SpinTrajectory(r.N, r.D, r.seed, ctx=ctx, pos_seed=r.pos_seed, **system)
+        for _ in range(r.steps):
+            tr.step()
+        r.result = np.concatenate([tr.sigma_z(), [tr.p_coh[-1]]])
     for v in list(vars(tr).values()):
         for o in (v if isinstance(v, (list, tuple)) else [v]):
             if isinstance(o, (M.MPS, M.MPO, M.Env)):
@@ -344,10 +460,13 @@ def solve_alone(r: TrajRequest, **system):
 
 
 class PooledServer:
-    """Continuous batching of trajectory requests over per-shape pools.
+    """Continuous batching of trajectory and density-matrix requests over
+    per-shape pools.
 
     Requests arrive with arrival times; each tick admits the arrived requests
-    into free slots of the pool of their shape (N, D) (a pool is created,
+    into free slots of the pool of their kind and shape (kind, N, D)
+    (``TrajectoryPool`` of ``slots`` copies for spin trajectories,
+    ``DensityMatrixPool`` of ``dm_slots`` copies for density matrices; a pool is created,
     within ``max_pools``, when a shape first arrives), queues one batched step of
     every busy pool (each pool has its own stream), then collects the norms,
     runs the jump branches and retires finished requests; a pool's next step
@@ -357,17 +476,25 @@ class PooledServer:
     shared by every pool.
     """
 
-    def __init__(self, slots=64, max_pools=8, **system):
-        self.slots, self.max_pools, self.system = slots, max_pools, system
+    def __init__(self, slots=64, max_pools=8, dm_slots=32, **system):
+        self.slots, self.max_pools, self.dm_slots, self.system = slots, max_pools, dm_slots, system
+
+    def _pool(self, key):
+        kind, N, D = key
+        if kind == "rydberg":
+            return DensityMatrixPool(N, D, self.dm_slots)
+        return TrajectoryPool(N, D, self.slots, **self.system)
 
     def run(self, requests, warm=()):
-        """Serve ``requests``; pools of the shapes in ``warm`` [(N, D)] are
-        built and stepped twice (capturing their step graphs) before the clock
-        starts, as a running server keeps them."""
+        """Serve ``requests``; pools of the shapes in ``warm`` [(N, D) for
+        spin trajectories, or (kind, N, D)] are built and stepped twice
+        (capturing their step graphs) before the clock starts, as a running
+        server keeps them."""
         pending = sorted(requests, key=lambda r: r.arrival)
         pools = {}
         for key in warm:
-            pool = pools[key] = TrajectoryPool(*key, self.slots, **self.system)
+            key = tuple(key) if len(key) == 3 else ("spin",) + tuple(key)
+            pool = pools[key] = self._pool(key)
             for _ in range(2):
                 pool.launch()
                 pool.finish()
@@ -381,7 +508,7 @@ class PooledServer:
                 if r.arrival > now:
                     keep.append(r)
                     continue
-                key = (r.N, r.D)
+                key = (r.kind, r.N, r.D)
                 pool = pools.get(key)
                 if pool is None and len(pools) >= self.max_pools:
                     # make room: release a pool with nothing in flight or queued
@@ -389,7 +516,7 @@ class PooledServer:
                     if idle:
                         pools.pop(idle[0]).close()
                 if pool is None and len(pools) < self.max_pools:
-                    pool = pools[key] = TrajectoryPool(r.N, r.D, self.slots, **self.system)
+                    pool = pools[key] = self._pool(key)
                 # a pool with a step in flight takes requests only once that
                 # step is collected (the step's norms belong to the old slots)
                 free = pool.free_slots() if pool is not None and key not in inflight else []
@@ -427,14 +554,17 @@ class PooledServer:
 
 
 def poisson_trajectory_requests(n, rate, seed, systems, steps_range=(10, 40)):
-    """Seeded stream of trajectory requests over ``systems`` [(N, D, pos_seed)]:
-    exponential inter-arrival times at ``rate`` per second."""
+    """Seeded stream of requests over ``systems`` [(N, D, pos_seed) for spin
+    trajectories, or (kind, N, D, pos_seed)]: exponential inter-arrival times
+    at ``rate`` per second."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(n):
         t += rng.exponential(1.0 / rate)
-        N, D, ps = systems[int(rng.integers(len(systems)))]
+        s = systems[int(rng.integers(len(systems)))]
+        kind, (N, D, ps) = (s[0], s[1:]) if len(s) == 4 else ("spin", s)
         out.append(TrajRequest(i, N, D, ps, int(rng.integers(1 << 30)),
-                               int(rng.integers(steps_range[0], steps_range[1], endpoint=True)), t))
+                               int(rng.integers(steps_range[0], steps_range[1], endpoint=True)), t,
+                               kind=kind))
     return out

@@ -137,3 +137,79 @@ def test_pooled_server_poisson_stream():
         _pool_run(pool, {0: [(a, 0)]}, r.steps)
         pool.close()
         assert np.abs(a.result - r.result).max() <= SAME, r.rid
+
+
+# ------------------------------------------------ lockstep density-matrix pools
+def test_density_matrix_pool_same_alone_and_in_the_mix():
+    N, D, ps, steps = 6, 9, 3, 6
+    alone = S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg")
+    pool = S.DensityMatrixPool(N, D, slots=4)
+    _pool_run(pool, {0: [(alone, 0)]}, steps)
+    pool.close()
+    # the same request in slot 2, admitted at tick 2 next to requests on
+    # other chains of the same shape that come and go
+    rs = [S.TrajRequest(i, N, D, 10 + i, seed=0, steps=3 + i, kind="rydberg") for i in range(4)]
+    mix = S.TrajRequest(9, N, D, ps, seed=0, steps=steps, kind="rydberg")
+    pool = S.DensityMatrixPool(N, D, slots=4)
+    done = _pool_run(pool, {0: [(rs[0], 0), (rs[1], 1)], 2: [(mix, 2), (rs[2], 3)], 4: [(rs[3], 0)]}, 12)
+    pool.close()
+    assert mix in done and all(r in done for r in rs)
+    d = np.abs(mix.result - alone.result).max()
+    print("density-matrix pool alone vs mix max difference", d, mix.result, flush=True)
+    assert d <= SAME
+    # and the lockstep copy agrees with the evolution on a plain context
+    single = S.solve_alone(S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg"))
+    assert np.abs(single.result - alone.result).max() < 1e-10
+
+
+def test_density_matrix_pool_matches_oracle():
+    N, D, ps, steps, dt = 5, 9, 4, 4, 0.005
+    r = S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg")
+    pool = S.DensityMatrixPool(N, D, slots=3, dt=dt)
+    _pool_run(pool, {0: [(S.TrajRequest(1, N, D, 8, seed=0, steps=2, kind="rydberg"), 0)], 1: [(r, 1)]},
+              steps + 1)
+    got = pool.rv[1].to_host()          # slot 1 after the request's last step
+    pool.close()
+    from oracle import mps_oracle as O
+    from oracle import dense as Dn
+    from helpers import copy_sites, rel
+    O.set_qr_rank_tol(S.QR_RANK_TOL)
+    try:
+        p = models.RydbergParams(n_exp=2)
+        L = models.rydberg_rk4_mpos(p, models.positions(N, float(N), ps), dt)
+        g = np.zeros(9, dtype=complex)
+        g[0] = 1.0
+        rho, dims = O.mpsproductstate(N, D, 9, g)
+        st = [copy_sites(rho) for _ in range(3)]
+        envs = [O.build_env(dims, dims) for _ in range(3)]
+        eop = O.build_env_op(dims, dims, [1] + [p.w] * (N - 1) + [1])
+        for _ in range(steps):
+            O.RK4stp_apply_H_half_(dt, L, rho, *st, *envs, eop)
+    finally:
+        O.set_qr_rank_tol(0.0)
+    tr = O.scal_prod_no_conj(models.identity_vec_mps(N), rho)
+    ne = O.scal_prod_no_conj(models.sum_local_vec_mps(N, models.HSS3), rho) / tr
+    print("pool", r.result, "oracle", tr.real, ne.real, flush=True)
+    assert rel(Dn.mps_to_dense(got), Dn.mps_to_dense(rho)) < 1e-10
+    assert abs(r.result[0] - tr.real) < 1e-10
+    assert abs(r.result[1] - ne.real) < 1e-10 * max(1.0, abs(ne))
+
+
+def test_pooled_server_mixes_trajectories_and_density_matrices():
+    systems = [(8, 8, 1), ("rydberg", 5, 9, 2), ("rydberg", 5, 9, 6)]
+    reqs = S.poisson_trajectory_requests(12, rate=200.0, seed=8, systems=systems, steps_range=(3, 7))
+    kinds = {r.kind for r in reqs}
+    assert kinds == {"spin", "rydberg"}
+    stats = S.PooledServer(slots=4, max_pools=2, dm_slots=3, **SYS).run(reqs, warm=[("rydberg", 5, 9)])
+    print(stats, flush=True)
+    assert stats["requests"] == len(reqs) and stats["pools"] == 2
+    # every response equals the same request run alone in slot 0 of a fresh pool
+    for r in reqs:
+        a = S.TrajRequest(r.rid, r.N, r.D, r.pos_seed, r.seed, r.steps, kind=r.kind)
+        if r.kind == "rydberg":
+            pool = S.DensityMatrixPool(r.N, r.D, slots=3)
+        else:
+            pool = S.TrajectoryPool(r.N, r.D, slots=4, **SYS)
+        _pool_run(pool, {0: [(a, 0)]}, r.steps)
+        pool.close()
+        assert np.abs(a.result - r.result).max() <= SAME, (r.rid, r.kind)
