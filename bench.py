@@ -131,6 +131,50 @@ def ensemble_leg(seed, N=60, D=32, B=256, steps=10, n_seq=4):
     return out
 
 
+def density_matrix_pool_leg(seed, N=40, D=32, B=64, steps=10, n_seq=4):
+    """Lockstep pool of B Rydberg density-matrix evolutions (d=9; one batched
+    context, each copy on its own chain's positions) against the same
+    evolutions one after another on a plain context."""
+    import mpsamd as M
+    from mpsamd import serving as S
+    ctx = M.Context(0)
+    ctx.set_qr_rank_tol(S.QR_RANK_TOL)
+    evs = [S.RydbergEvolution(N, D, seed + s, ctx=ctx) for s in range(n_seq)]
+    for ev in evs:          # first two steps: eager, then graph capture
+        for _ in range(2):
+            ev.launch_coherent()
+            ev.finish_step()
+    t0 = time.perf_counter()
+    for ev in evs:
+        for _ in range(steps):
+            ev.launch_coherent()
+            ev.finish_step()
+    seq = (time.perf_counter() - t0) / (n_seq * steps)
+    for ev in evs:
+        for v in list(vars(ev).values()):
+            for o in (v if isinstance(v, (list, tuple)) else [v]):
+                if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                    o.__del__()
+    ctx.close()
+    pool = S.DensityMatrixPool(N, D, slots=B, pos_seed=seed)
+    for b in range(B):
+        pool.admit(S.TrajRequest(b, N, D, seed + b, 0, steps + 2, kind="rydberg"), b)
+    for _ in range(2):
+        pool.launch()
+        pool.finish()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pool.launch()
+        pool.finish()
+    per = (time.perf_counter() - t0) / (B * steps)
+    pool.close()
+    out = {"evolutions": B, "N": N, "D": D, "d": 9, "steps_timed": steps,
+           "evolution_steps_per_s": round(1.0 / per, 1), "sequential_evolution_steps_per_s": round(1.0 / seq, 1),
+           "speedup_vs_sequential": round(seq / per, 2)}
+    print(f"density-matrix pool: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_and_serve(seed):
     """Batched quantum-jump trajectories (concurrent per-stream solves vs the same
     solves one after another) and a Poisson request stream through the server."""
@@ -344,7 +388,8 @@ def main():
     batched = serving = None
     if not args.no_serve:
         uneven, per_solve = batch_and_serve(args.seed)
-        batched = {"lockstep_ensemble": ensemble_leg(args.seed), "concurrent_uneven": uneven}
+        batched = {"lockstep_ensemble": ensemble_leg(args.seed), "concurrent_uneven": uneven,
+                   "density_matrix_pool": density_matrix_pool_leg(args.seed)}
         serving = {"lockstep_pools": pooled_serving_leg(args.seed), "per_solve_streams": per_solve}
 
     cpu = None
