@@ -158,7 +158,7 @@ def density_matrix_pool_leg(seed, N=40, D=32, B=64, steps=10, n_seq=4):
     ctx.close()
     pool = S.DensityMatrixPool(N, D, slots=B, pos_seed=seed)
     for b in range(B):
-        pool.admit(S.TrajRequest(b, N, D, seed + b, 0, steps + 2, kind="rydberg"), b)
+        pool.admit(S.TrajRequest(b, N, D, seed + b, 0, steps + 3, kind="rydberg"), b)   # none retires inside the clock
     for _ in range(2):
         pool.launch()
         pool.finish()
