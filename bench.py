@@ -227,6 +227,25 @@ def pooled_serving_leg(seed, n_burst=480, n=320, rate=30.0, systems=4, N=60, D=3
     out["poisson"] = {"requests": st["requests"], "arrival_rate_req_s": rate, "slots": slots_poisson,
                       "throughput_req_s": round(st["throughput_req_s"], 3),
                       "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4)}
+    # one stream of both kinds: spin trajectories and d=9 density matrices,
+    # each kind continuously batched in its own lockstep pool
+    mix_rate, dm_shape = 50.0, (40, 32)
+    mix_sy = sy + [("rydberg", *dm_shape, seed + 10 + i) for i in range(2)]
+    reqs = S.poisson_trajectory_requests(300, rate=mix_rate, seed=seed + 2, systems=mix_sy, steps_range=(10, 40))
+    st = S.PooledServer(slots=slots_poisson, dm_slots=32, dt=0.01).run(reqs, warm=[(N, D), ("rydberg", *dm_shape)])
+    print(f"pooled serving, mixed Poisson {mix_rate}/s: {st}", file=sys.stderr, flush=True)
+    kinds = {}
+    for k in ("spin", "rydberg"):
+        lat = np.array([r.done - r.arrival for r in reqs if r.kind == k])
+        kinds[k] = {"requests": int(lat.size), "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                    "p99_latency_s": round(float(np.percentile(lat, 99)), 4)}
+    out["mixed_poisson"] = {
+        "mix": f"{len(sy)} spin chains (d=2, N={N}, D={D}, {slots_poisson} slots) and 2 Rydberg density-matrix "
+               f"chains (d=9, N={dm_shape[0]}, D={dm_shape[1]}, 32 slots), drawn uniformly; 10-40 steps each",
+        "requests": st["requests"], "arrival_rate_req_s": mix_rate,
+        "throughput_req_s": round(st["throughput_req_s"], 3),
+        "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4),
+        "by_kind": kinds}
     return out
 
 
