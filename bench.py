@@ -175,6 +175,46 @@ def density_matrix_pool_leg(seed, N=40, D=32, B=64, steps=10, n_seq=4):
     return out
 
 
+def uneven_pools_leg(seed, n=512, steps=10, Ns=(40, 60, 80, 100, 120), Ds=(16, 32, 64), slots=64):
+    """n quantum-jump trajectories of uneven shape (N and D drawn per
+    trajectory, one chain per shape) as one burst through the pooled server:
+    a lockstep pool per shape, the pools advancing concurrently on their own
+    streams.  The sequential rate is measured on one trajectory per shape on a
+    plain context and weighted by the number of trajectories of that shape."""
+    import mpsamd as M
+    from mpsamd import serving as S
+    from mpsamd.trajectory import SpinTrajectory
+    rng = np.random.default_rng(seed)
+    shapes = [(N, D) for N in Ns for D in Ds]
+    pick = rng.integers(len(shapes), size=n)
+    reqs = [S.TrajRequest(i, *shapes[k], seed + k, int(rng.integers(1 << 30)), steps) for i, k in enumerate(pick)]
+    st = S.PooledServer(slots=slots, max_pools=len(shapes), dt=0.01).run(reqs)
+    print(f"uneven pools: {st}", file=sys.stderr, flush=True)
+    ctx = M.Context(0)
+    ctx.set_qr_rank_tol(S.QR_RANK_TOL)
+    t_seq = 0.0
+    for k, (N, D) in enumerate(shapes):
+        tr = SpinTrajectory(N, D, seed, dt=0.01, ctx=ctx, pos_seed=seed + k)
+        tr.step(), tr.step()       # eager, then graph capture
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tr.step()
+        t_seq += (time.perf_counter() - t0) * int((pick == k).sum())
+        for v in list(vars(tr).values()):
+            for o in (v if isinstance(v, (list, tuple)) else [v]):
+                if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                    o.__del__()
+    ctx.close()
+    out = {"trajectories": n, "steps_each": steps, "N_choices": list(Ns), "D_choices": list(Ds),
+           "pools": st["pools"], "slots_per_pool": slots, "wall_s": round(st["wall_s"], 3),
+           "trajectories_per_s": round(n / st["wall_s"], 2),
+           "sequential_trajectories_per_s": round(n / t_seq, 3),
+           "speedup_vs_sequential": round(t_seq / st["wall_s"], 2),
+           "note": "wall time includes building each pool and capturing its step graph"}
+    print(f"uneven pools: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_and_serve(seed):
     """Batched quantum-jump trajectories (concurrent per-stream solves vs the same
     solves one after another) and a Poisson request stream through the server."""
@@ -408,7 +448,8 @@ def main():
     if not args.no_serve:
         uneven, per_solve = batch_and_serve(args.seed)
         batched = {"lockstep_ensemble": ensemble_leg(args.seed), "concurrent_uneven": uneven,
-                   "density_matrix_pool": density_matrix_pool_leg(args.seed)}
+                   "density_matrix_pool": density_matrix_pool_leg(args.seed),
+                   "uneven_pools": uneven_pools_leg(args.seed)}
         serving = {"lockstep_pools": pooled_serving_leg(args.seed), "per_solve_streams": per_solve}
 
     cpu = None
