@@ -360,12 +360,13 @@ def main():
 
     def observables(r):
         # total populations and the pair correlation of the excited level,
-        # g2 = 2 sum_{i<j} <n_i n_j> / (sum_i <n_i>)^2
+        # g2 = 2 sum_{i<j} <n_i n_j> / (sum_i <n_i>)^2 (undefined, None, on a
+        # state with no excitation, e.g. the ground state before any step)
         tr_ = M.scal_prod_no_conj(idm, r)
         n_e = M.scal_prod_no_conj(nem, r) / tr_
         pairs = M.scal_prod_no_conj(pem, r) / tr_
         return {"trace": tr_, "n_e": n_e, "n_r": M.scal_prod_no_conj(nrm, r) / tr_,
-                "g2_e": (2.0 * pairs / n_e ** 2).real}
+                "g2_e": (2.0 * pairs / n_e ** 2).real if n_e != 0 else None}
 
     tr = M.scal_prod_no_conj(idm, rho)
     ref_obs = observables(rho)
