@@ -1318,9 +1318,17 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     // for the side stream runs eagerly (its launches are hidden behind the
     // kernels, and measured, a second live graph with side-stream branches in
     // the same context ran at half speed).
+    // A graph is keyed by the handles and holds their device pointers; a
+    // handle's destruction clears the graphs of its own context only.  So a
+    // call whose MPOs, environments or states belong to another context runs
+    // eagerly: a graph of it could outlive one of them.
+    bool own_all = true;
+    for (auto h : {H1, H2, H3}) own_all = own_all && h->owner == x;
+    for (auto e : {env1, env2, env3, envop}) own_all = own_all && e->owner == x;
+    for (auto s : {out, psi1, psi2, psi3}) own_all = own_all && s->owner == x;
     const bool had_spec = x->spec;
     x->spec = false;
-    if (!c->graphs || c->profiling || p.conc) {
+    if (!c->graphs || c->profiling || p.conc || !own_all) {
       if (p.conc && p.shadow) {
         const bool hit = had_spec && g_epoch.load() == x->spec_epoch + 1 && x->spec_H1 == H1 &&
                          x->spec_psi == psi && x->spec_envop == envop && x->spec_prec == c->prec;

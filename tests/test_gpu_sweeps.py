@@ -186,6 +186,35 @@ def test_rk4_product_state_truncated_with_rank_cutoff():
     assert np.abs(ez_dev - ez_ref).max() < TOL
 
 
+def test_rk4_mpos_of_another_context_replaced_between_runs():
+    # MPOs owned by a second context: such a call runs eagerly (a graph of it
+    # could outlive the MPOs), so MPOs destroyed and re-made with other
+    # contents, possibly at the same host address, are the ones used
+    N, D, steps = 8, 8, 3
+    ctx2 = M.Context(0)
+    try:
+        for dt in (0.01, 0.02, 0.01):
+            print("case dt", dt, flush=True)
+            z, H, psi, dims = _spin_setup(N, D, dt, "random")
+            wop = [1] + [4] * (N - 1) + [1]
+            ref, rn = _oracle_rk4(H, psi, dims, dt, steps, wop)
+            gH = [M.MPO(h, ctx2) for h in H]
+            g = dev_state(psi)
+            g1, g2, g3 = dev_state(psi), dev_state(psi), dev_state(psi)
+            e1, e2, e3 = (M.build_env(dims, dims) for _ in range(3))
+            eop = M.build_env(dims, dims, wop)
+            gn = []
+            for _ in range(steps):
+                M.RK4stp_apply_H_half_(dt, gH, g, g1, g2, g3, e1, e2, e3, eop)
+                gn.append(M.normalize_lo_(g))
+            assert np.allclose(gn, rn, rtol=TOL, atol=0)
+            assert rel(Dn.mps_to_dense(g.to_host()), Dn.mps_to_dense(ref)) < TOL
+            for h in gH:
+                h.__del__()
+    finally:
+        ctx2.close()
+
+
 def _sz(psi):
     N = len(psi)
     env = np.ones((1, 1), dtype=complex)
