@@ -229,7 +229,8 @@ int mps_normalize_lo_batch(mps_state_t s, double* norm2);
  * Returns once the step is queued on the context's stream; later calls on the
  * same context, downloads and scalar-returning calls order after it.  Small
  * sites run, from the second call with a given set of handles on, as one
- * captured graph; sites of 32768 elements or more run eagerly, their overlap
+ * captured graph, when every handle of the call was created on this context
+ * (otherwise eagerly); sites of 32768 elements or more run eagerly, their overlap
  * terms on a second stream of the context, and the environment builds of
  * stages 2-4 on a third, behind the previous stage's sweep, into envop and a
  * second operator environment the context keeps; both streams are joined
