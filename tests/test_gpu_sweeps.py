@@ -180,10 +180,26 @@ def test_rk4_product_state_truncated_with_rank_cutoff():
         g, gn = _device_rk4(H, psi, dims, dt, steps, wop)
     finally:
         ctx.set_qr_rank_tol(0.0)
+        O.set_qr_rank_tol(0.0)
     assert np.allclose(gn, rn, rtol=TOL, atol=0)
     ez_ref = _sz(ref)
     ez_dev = M.measure_excitations_(None, g, models.HEE2)
     assert np.abs(ez_dev - ez_ref).max() < TOL
+
+
+@pytest.mark.parametrize("N,D", [(2, 4), (3, 4), (4, 2)])
+def test_rk4_short_chains(N, D):
+    # the shortest chains: every slot of the environments is a boundary slot
+    # or next to one, and the sweeps run one or two sites
+    dt, steps = 0.01, 4
+    print("case", N, D, flush=True)
+    z, H, psi, dims = _spin_setup(N, D, dt, "random", seed=3)
+    wop = [1] + [4] * (N - 1) + [1]
+    ref, rn = _oracle_rk4(H, psi, dims, dt, steps, wop)
+    g, gn = _device_rk4(H, psi, dims, dt, steps, wop)
+    assert np.allclose(gn, rn, rtol=TOL, atol=0)
+    assert rel(Dn.mps_to_dense(g.to_host()), Dn.mps_to_dense(ref)) < TOL
+    assert np.abs(M.measure_excitations_(None, g, models.HEE2) - _sz(ref)).max() < TOL
 
 
 def test_rk4_mpos_of_another_context_replaced_between_runs():
