@@ -419,6 +419,37 @@ def test_compression_svd(rng, N, D, d, dmax, tol):
         assert np.abs(M2 @ M2.conj().T - np.eye(B.shape[0])).max() < 1e-12
 
 
+@pytest.mark.parametrize("tol", [1e-10, 1e-6])
+def test_compression_svd_of_an_evolved_spin_chain(tol):
+    # the spin-chain case's truncation (N = 20, D = 32, cutoff 1e-10) on a
+    # state evolved 40 steps from |g...g>, whose spectra decay fast enough
+    # for the cutoff, not D, to set the bonds
+    from oracle import trajectory as OT
+    N, D, dt = 20, 32, 0.01
+    print("case", tol, flush=True)
+    z = models.positions(N, 20.0, 1)
+    H = models.spin_rk4_mpos(z, 0.0, dt)
+    jr, jl = models.spin_jump_mpos(z)
+    O.set_qr_rank_tol(1e-13)
+    try:
+        tr = OT.OracleSpinTrajectory(N, D, 1, H, jr, jl, dt)
+        for _ in range(40):
+            tr.step()
+    finally:
+        O.set_qr_rank_tol(0.0)
+    ref = copy_sites(tr.A)
+    g = dev_state(tr.A)
+    eg_r, et_r, ee_r = O.compressionSVD_(ref, D, tol)
+    assert max(s.shape[2] for s in ref) < D
+    eg, et, ee = M.compressionSVD_(g, D, tol)
+    got = g.to_host()
+    assert [s.shape for s in got] == [s.shape for s in ref]
+    assert abs(eg - eg_r) < 1e-12
+    assert np.abs(et - et_r).max() < 1e-12
+    assert np.abs(ee - ee_r).max() < 1e-10
+    assert _phase_aligned_rel(Dn.mps_to_dense(got), Dn.mps_to_dense(ref)) < 1e-10
+
+
 @pytest.mark.parametrize("graphs", [False, True])
 def test_rk4_fp32_operator_terms(graphs):
     # Precision form: the operator contractions run in complex FP32 (3M MFMA,
