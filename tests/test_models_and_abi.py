@@ -149,3 +149,21 @@ def test_vectorised_sum_and_pair_sum_observables():
                 for a in range(N) for b in range(a + 1, N))
     assert np.allclose(_mps_vector(models.sum_local_vec_mps(N, op)), single, atol=1e-12)
     assert np.allclose(_mps_vector(models.pair_sum_vec_mps(N, op)), pairs, atol=1e-12)
+
+
+def test_request_streams_mix_kinds_and_replay():
+    # the pooled server's request stream: seeded (the same stream twice),
+    # arrival times increasing at about the given rate, and systems given as
+    # (N, D, pos_seed) for spin trajectories or (kind, N, D, pos_seed)
+    from mpsamd import serving as S
+    systems = [(8, 8, 1), ("rydberg", 5, 9, 2), ("spin", 10, 8, 3)]
+    a = S.poisson_trajectory_requests(400, rate=50.0, seed=4, systems=systems, steps_range=(3, 7))
+    b = S.poisson_trajectory_requests(400, rate=50.0, seed=4, systems=systems, steps_range=(3, 7))
+    assert [(r.kind, r.N, r.D, r.pos_seed, r.seed, r.steps, r.arrival) for r in a] == \
+           [(r.kind, r.N, r.D, r.pos_seed, r.seed, r.steps, r.arrival) for r in b]
+    t = np.array([r.arrival for r in a])
+    assert np.all(np.diff(t) > 0)
+    assert abs(len(a) / t[-1] - 50.0) < 10.0
+    shapes = {(r.kind, r.N, r.D, r.pos_seed) for r in a}
+    assert shapes == {("spin", 8, 8, 1), ("rydberg", 5, 9, 2), ("spin", 10, 8, 3)}
+    assert all(3 <= r.steps <= 7 for r in a)
