@@ -36,7 +36,7 @@ import numpy as np
 
 from . import core as M
 from . import models as MD
-from .trajectory import SpinEnsemble, SpinTrajectory
+from .trajectory import SpinEnsemble, SpinTrajectory, slab_bytes, slab_sizes, spin_ensemble_bytes
 
 
 class RydbergEvolution:
@@ -346,16 +346,13 @@ class DensityMatrixPool:
     """
 
     def __init__(self, N, D, slots=32, pos_seed=0, dt=0.005, n_exp=2, device=0):
-        from .trajectory import _env_bytes, _mpo_bytes, _state_bytes
         self.key = ("rydberg", N, D)
         self.N, self.D, self.dt = N, D, dt
         self.p = MD.RydbergParams(n_exp=n_exp)
         Ls = MD.rydberg_rk4_mpos(self.p, MD.positions(N, float(N), pos_seed), dt)
         dims = M._dims_rule(N, D, 9)
-        w = max(h.shape[0] for hs in Ls for h in hs)
-        copy_bytes = (4 * (_state_bytes(dims, 9) + 256) + 3 * (_env_bytes(dims, 1) + 256)
-                      + _env_bytes(dims, w) + 256 + sum(_mpo_bytes(h) + 256 for h in Ls) + 256 * 64)
-        ws = M.batch_workspace_bytes(slots, max(dims), 9, w, device)
+        copy_bytes, ws, w = slab_sizes(dims, 9, Ls, 4, slots, device)
+        self.hbm_bytes = slab_bytes(copy_bytes, ws, slots)
         self.ctx = M.Context.batched(slots, copy_bytes, ws, device)
         self.ctx.set_qr_rank_tol(QR_RANK_TOL)
         self.comp = self.ctx.companion()
@@ -459,6 +456,14 @@ def solve_alone(r: TrajRequest, **system):
     return r
 
 
+def density_pool_bytes(N, D, slots, n_exp=2, dt=0.005, device=0):
+    """HBM bytes of the slab of a ``DensityMatrixPool``, before building it
+    (the MPO shapes do not depend on the positions)."""
+    Ls = MD.rydberg_rk4_mpos(MD.RydbergParams(n_exp=n_exp), MD.positions(N, float(N), 0), dt)
+    copy_bytes, ws, _ = slab_sizes(M._dims_rule(N, D, 9), 9, Ls, 4, slots, device)
+    return slab_bytes(copy_bytes, ws, slots)
+
+
 class PooledServer:
     """Continuous batching of trajectory and density-matrix requests over
     per-shape pools.
@@ -472,12 +477,28 @@ class PooledServer:
     runs the jump branches and retires finished requests; a pool's next step
     is queued as soon as its last one is collected.  A pool with nothing in
     flight is released when ``max_pools`` are open and another shape
-    arrives.  Latency = completion - arrival.  ``system`` (dt, Omega, Gamma, gam_eg, init ...) is
-    shared by every pool.
+    arrives.  With ``hbm_budget_bytes``, a pool opens only while the slabs of
+    the open pools and its own (computed before it is built) stay within the
+    budget: idle pools are released to make room, and otherwise its requests
+    wait until room frees; a shape whose pool alone exceeds the budget is an
+    error.  Latency = completion - arrival.  ``system`` (dt, Omega, Gamma,
+    gam_eg, init ...) is shared by every spin pool.
     """
 
-    def __init__(self, slots=64, max_pools=8, dm_slots=32, **system):
+    def __init__(self, slots=64, max_pools=8, dm_slots=32, hbm_budget_bytes=None, **system):
         self.slots, self.max_pools, self.dm_slots, self.system = slots, max_pools, dm_slots, system
+        self.budget = hbm_budget_bytes
+        self._bytes = {}
+
+    def pool_bytes(self, key):
+        """HBM bytes of the slab of the pool for (kind, N, D)."""
+        if key not in self._bytes:
+            kind, N, D = key
+            if kind == "rydberg":
+                self._bytes[key] = density_pool_bytes(N, D, self.dm_slots)
+            else:
+                self._bytes[key] = spin_ensemble_bytes(N, D, self.slots, self.system.get("dt", 0.01))
+        return self._bytes[key]
 
     def _pool(self, key):
         kind, N, D = key
@@ -491,10 +512,35 @@ class PooledServer:
         (capturing their step graphs) before the clock starts, as a running
         server keeps them."""
         pending = sorted(requests, key=lambda r: r.arrival)
+        warm = [tuple(k) if len(k) == 3 else ("spin",) + tuple(k) for k in warm]
+        if self.budget is not None:
+            big = [k for k in {(r.kind, r.N, r.D) for r in requests} | set(warm) if self.pool_bytes(k) > self.budget]
+            if big:
+                raise ValueError(f"pools larger than the HBM budget: {big}")
         pools = {}
+        acct = {"used": 0, "peak": 0, "opened": 0}
+
+        def open_pool(key):
+            pools[key] = pool = self._pool(key)
+            if self.budget is not None:
+                acct["used"] += self.pool_bytes(key)
+                acct["peak"] = max(acct["peak"], acct["used"])
+            acct["opened"] += 1
+            return pool
+
+        def close_pool(key):
+            pools.pop(key).close()
+            if self.budget is not None:
+                acct["used"] -= self.pool_bytes(key)
+
+        def fits(key):
+            return len(pools) < self.max_pools and (
+                self.budget is None or acct["used"] + self.pool_bytes(key) <= self.budget)
+
         for key in warm:
-            key = tuple(key) if len(key) == 3 else ("spin",) + tuple(key)
-            pool = pools[key] = self._pool(key)
+            if not fits(key):
+                continue
+            pool = open_pool(key)
             for _ in range(2):
                 pool.launch()
                 pool.finish()
@@ -510,13 +556,13 @@ class PooledServer:
                     continue
                 key = (r.kind, r.N, r.D)
                 pool = pools.get(key)
-                if pool is None and len(pools) >= self.max_pools:
-                    # make room: release a pool with nothing in flight or queued
+                if pool is None and not fits(key):
+                    # make room: release pools with nothing in flight or queued
                     idle = [k for k, q in pools.items() if k not in inflight and not q.busy()]
-                    if idle:
-                        pools.pop(idle[0]).close()
-                if pool is None and len(pools) < self.max_pools:
-                    pool = pools[key] = self._pool(key)
+                    while idle and not fits(key):
+                        close_pool(idle.pop(0))
+                if pool is None and fits(key):
+                    pool = open_pool(key)
                 # a pool with a step in flight takes requests only once that
                 # step is collected (the step's norms belong to the old slots)
                 free = pool.free_slots() if pool is not None and key not in inflight else []
@@ -529,8 +575,10 @@ class PooledServer:
 
         # Pools advance independently: a pool's next step is queued as soon as
         # its previous one is collected, so the other pools' queued steps keep
-        # the GPU busy while the host handles one pool.
-        while pending or inflight:
+        # the GPU busy while the host handles one pool.  A pool opened by an
+        # admission during a pass is queued on the next pass, so the loop runs
+        # while any pool holds requests.
+        while pending or inflight or any(q.busy() for q in pools.values()):
             pending = admit(time.perf_counter() - t0)
             for key, p in list(pools.items()):
                 if key in inflight:
@@ -548,9 +596,12 @@ class PooledServer:
         for p in pools.values():
             p.close()
         lat = np.array([r.done - r.arrival for r in done])
-        return {"requests": len(done), "wall_s": wall, "pools": len(pools),
-                "throughput_req_s": len(done) / wall,
-                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+        out = {"requests": len(done), "wall_s": wall, "pools": len(pools), "pools_opened": acct["opened"],
+               "throughput_req_s": len(done) / wall,
+               "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+        if self.budget is not None:
+            out.update(hbm_budget_bytes=self.budget, peak_pool_bytes=acct["peak"])
+        return out
 
 
 def poisson_trajectory_requests(n, rate, seed, systems, steps_range=(10, 40)):

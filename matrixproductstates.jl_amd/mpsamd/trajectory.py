@@ -149,6 +149,32 @@ def _mpo_bytes(sites):
     return 16 * sum(s.size for s in sites)
 
 
+def slab_sizes(dims, d, Hs, n_states, B, device=0):
+    """(copy_bytes, ws_bytes, w) of a lockstep batch of B copies, each holding
+    ``n_states`` states of ``dims`` with local dim d, three overlap
+    environments, one operator environment and the MPOs ``Hs``."""
+    w = max(h.shape[0] for hs in Hs for h in hs)
+    copy_bytes = (n_states * (_state_bytes(dims, d) + 256) + 3 * (_env_bytes(dims, 1) + 256)
+                  + _env_bytes(dims, w) + 256 + sum(_mpo_bytes(h) + 256 for h in Hs) + 256 * 64)
+    ws = M.batch_workspace_bytes(B, max(dims), d, w, device)
+    return copy_bytes, ws, w
+
+
+def slab_bytes(copy_bytes, ws, B):
+    """HBM bytes of the batched context's slab (the library's per-copy stride
+    times B)."""
+    ws = (ws + 255) & ~255
+    return B * ((copy_bytes + ws + 4096 + 255) & ~255)
+
+
+def spin_ensemble_bytes(N, D, B, dt=0.01, device=0):
+    """HBM bytes of the slab of a ``SpinEnsemble`` of B copies of an (N, D)
+    chain, before building it (the MPO shapes do not depend on the positions)."""
+    Hs = MD.spin_rk4_mpos(MD.positions(N, float(N), 0), 0.0, dt)
+    copy_bytes, ws, _ = slab_sizes(M._dims_rule(N, D, 2), 2, Hs, 5, B, device)
+    return slab_bytes(copy_bytes, ws, B)
+
+
 class SpinEnsemble:
     """A lockstep ensemble of B trajectories of one driven spin chain.
 
@@ -173,11 +199,8 @@ class SpinEnsemble:
         Hs = MD.spin_rk4_mpos(self.z, 0.0, dt, Omega, Gamma, k)
         jr, jl = MD.spin_jump_mpos(self.z, Gamma, k)
         dims = M._dims_rule(N, D, 2)
-        w = max(h.shape[0] for hs in Hs for h in hs)
-        slack = 256 * 64
-        copy_bytes = (5 * (_state_bytes(dims, 2) + 256) + 3 * (_env_bytes(dims, 1) + 256)
-                      + _env_bytes(dims, w) + 256 + sum(_mpo_bytes(h) + 256 for h in Hs) + slack)
-        ws = M.batch_workspace_bytes(B, max(dims), 2, w, device)
+        copy_bytes, ws, w = slab_sizes(dims, 2, Hs, 5, B, device)
+        self.hbm_bytes = slab_bytes(copy_bytes, ws, B)
         self.ctx = M.Context.batched(B, copy_bytes, ws, device)
         self.ctx.set_qr_rank_tol(qr_rank_tol)
         self.comp = self.ctx.companion()

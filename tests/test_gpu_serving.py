@@ -213,3 +213,34 @@ def test_pooled_server_mixes_trajectories_and_density_matrices():
         _pool_run(pool, {0: [(a, 0)]}, r.steps)
         pool.close()
         assert np.abs(a.result - r.result).max() <= SAME, (r.rid, r.kind)
+
+
+def test_pooled_server_hbm_budget():
+    # the pool sizes the server plans with are the slabs the pools allocate
+    srv = S.PooledServer(slots=4, max_pools=4, dm_slots=3, **SYS)
+    b1, b2 = srv.pool_bytes(("spin", 8, 8)), srv.pool_bytes(("spin", 10, 8))
+    pool = S.TrajectoryPool(8, 8, 4, **SYS)
+    assert pool.ens.hbm_bytes == b1
+    pool.close()
+    dm = S.DensityMatrixPool(5, 9, slots=3)
+    assert dm.hbm_bytes == srv.pool_bytes(("rydberg", 5, 9))
+    dm.close()
+    # a budget that holds one of the two pools at a time: they take turns
+    budget = max(b1, b2) + min(b1, b2) // 2
+    systems = [(8, 8, 1), (10, 8, 2)]
+    reqs = S.poisson_trajectory_requests(10, rate=200.0, seed=9, systems=systems, steps_range=(3, 6))
+    assert {(r.N, r.D) for r in reqs} == {(8, 8), (10, 8)}
+    stats = S.PooledServer(slots=4, max_pools=4, hbm_budget_bytes=budget, **SYS).run(reqs)
+    print(stats, flush=True)
+    assert stats["requests"] == len(reqs)
+    assert stats["pools_opened"] >= 2
+    assert stats["peak_pool_bytes"] <= budget
+    for r in reqs:
+        a = S.TrajRequest(r.rid, r.N, r.D, r.pos_seed, r.seed, r.steps)
+        pool = S.TrajectoryPool(r.N, r.D, slots=4, **SYS)
+        _pool_run(pool, {0: [(a, 0)]}, r.steps)
+        pool.close()
+        assert np.abs(a.result - r.result).max() <= SAME, r.rid
+    # a pool larger than the whole budget is refused before anything runs
+    with pytest.raises(ValueError):
+        S.PooledServer(slots=4, hbm_budget_bytes=min(b1, b2) - 1, **SYS).run(reqs)
