@@ -272,7 +272,8 @@ def pooled_serving_leg(seed, n_burst=480, n=320, rate=30.0, systems=4, N=60, D=3
     mix_rate, dm_shape = 50.0, (40, 32)
     mix_sy = sy + [("rydberg", *dm_shape, seed + 10 + i) for i in range(2)]
     reqs = S.poisson_trajectory_requests(300, rate=mix_rate, seed=seed + 2, systems=mix_sy, steps_range=(10, 40))
-    st = S.PooledServer(slots=slots_poisson, dm_slots=32, dt=0.01).run(reqs, warm=[(N, D), ("rydberg", *dm_shape)])
+    st = S.PooledServer(slots=slots_poisson, dm_slots=32, hbm_budget_bytes=64 << 30, dt=0.01).run(
+        reqs, warm=[(N, D), ("rydberg", *dm_shape)])
     print(f"pooled serving, mixed Poisson {mix_rate}/s: {st}", file=sys.stderr, flush=True)
     kinds = {}
     for k in ("spin", "rydberg"):
@@ -285,6 +286,7 @@ def pooled_serving_leg(seed, n_burst=480, n=320, rate=30.0, systems=4, N=60, D=3
         "requests": st["requests"], "arrival_rate_req_s": mix_rate,
         "throughput_req_s": round(st["throughput_req_s"], 3),
         "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4),
+        "hbm_budget_bytes": st["hbm_budget_bytes"], "peak_pool_bytes": st["peak_pool_bytes"],
         "by_kind": kinds}
     return out
 
