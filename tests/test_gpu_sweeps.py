@@ -285,6 +285,38 @@ def _local(psi, op):
     return ep
 
 
+@pytest.mark.parametrize("N,D,d", [(6, 8, 2), (5, 9, 3)])
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_orthogonalize_without_normalising(N, D, d, side, rng):
+    # leftorthogonalizeQR! / rightorthogonalizeQR! (orthonormalisation.jl:43-56,
+    # 77-90): the gauge moves alone, on a non-canonical state of norm != 1.  The
+    # state is unchanged, the returned value is its norm^2, every site but the
+    # last (first) is an isometry, and the sites equal the oracle's.
+    print("case", N, D, d, side, flush=True)
+    a, _ = O.mpsrandom(N, D, d, rng)
+    a = [1.3 * s + 0.1 * crandn(rng, s.shape) for s in a]
+    dense = Dn.mps_to_dense(a)
+    ref = copy_sites(a)
+    fo = O.leftorthogonalizeQR_ if side == "left" else O.rightorthogonalizeQR_
+    fm = M.leftorthogonalizeQR_ if side == "left" else M.rightorthogonalizeQR_
+    n_ref = fo(ref)
+    g = dev_state(a)
+    n_dev = fm(g)
+    got = g.to_host()
+    assert abs(n_dev - n_ref) < TOL * n_ref
+    assert abs(n_dev - np.vdot(dense, dense).real) < TOL * n_dev
+    assert rel(Dn.mps_to_dense(got), dense) < TOL
+    for j in range(N):
+        assert rel(got[j], ref[j]) < 1e-9, j
+    for B in (got[:-1] if side == "left" else got[1:]):
+        if side == "left":
+            Q = B.reshape(-1, B.shape[2], order="F")
+            assert np.abs(Q.conj().T @ Q - np.eye(Q.shape[1])).max() < 1e-12
+        else:
+            Q = B.reshape(B.shape[0], -1, order="F")
+            assert np.abs(Q @ Q.conj().T - np.eye(Q.shape[0])).max() < 1e-12
+
+
 def test_gauge_moves(rng):
     N, D, d = 6, 8, 2
     a, dims = O.mpsrandom(N, D, d, rng)
