@@ -285,6 +285,27 @@ def _local(psi, op):
     return ep
 
 
+def test_mpo_set_block(rng):
+    # mps_mpo_set_block rewrites O[j][wl, :, wr, :] in place (the per-step
+    # writes of update_h! / update_L_Ryd!); through <A|O|B> its effect equals
+    # the same rewrite of the host MPO in the oracle
+    N, D, d, w = 6, 6, 3, 3
+    a, _ = O.mpsrandom(N, D, d, rng)
+    b, _ = O.mpsrandom(N, D, d, rng)
+    H = random_mpo(rng, N, d, w)
+    gH = M.MPO(H)
+    ga, gb = dev_state(a), dev_state(b)
+    before = O.scal_op_prod(a, H, b)
+    assert abs(M.scal_op_prod(ga, gH, gb) - before) < TOL * max(1.0, abs(before))
+    for j, wl, wr in [(0, 0, 2), (2, 1, 0), (2, 2, 2), (5, 1, 0)]:
+        op = crandn(rng, (d, d))
+        H[j][wl, :, wr, :] = op
+        gH.set_block(j, wl, wr, op)
+    after = O.scal_op_prod(a, H, b)
+    assert abs(after - before) > 1e-3 * abs(before)
+    assert abs(M.scal_op_prod(ga, gH, gb) - after) < TOL * max(1.0, abs(after))
+
+
 @pytest.mark.parametrize("N,D,d", [(6, 8, 2), (5, 9, 3)])
 @pytest.mark.parametrize("side", ["left", "right"])
 def test_orthogonalize_without_normalising(N, D, d, side, rng):
