@@ -524,3 +524,47 @@ def test_rk4_fp32_operator_terms(graphs):
     print("fp32 relative state error", err, flush=True)
     assert 1e-12 < err < 1e-5
     assert np.allclose(gn, rn, rtol=1e-5, atol=0)
+
+
+def test_environment_slots_after_compress_var_init(rng):
+    # mps_env_download_slot: compress_var! with no sweeps only builds its left
+    # overlap environments (compression.jl:65-103, the initialisation loop);
+    # every slot equals the oracle's, boundaries are 1
+    N, D0, D, d = 6, 4, 8, 2
+    target, dims = O.mpsrandom(N, D, d, rng)
+    guess, dims0 = O.mpsrandom(N, D0, d, rng)
+    env_o = O.build_env(dims0, dims)
+    O.compress_var_(copy_sites(guess), target, env_o, 0)
+    env_d = M.build_env(dims0, dims)
+    M.compress_var_(dev_state(guess), dev_state(target), env_d, 0)
+    for j in range(N + 1):
+        ref = np.asarray(env_o[j])
+        got = env_d.slot(j, ref.shape)
+        assert rel(got, ref) < TOL, j
+    assert env_d.slot(0, (1, 1))[0, 0] == 1.0 and env_d.slot(N, (1, 1))[0, 0] == 1.0
+
+
+def test_kernel_stats_count_the_algorithmic_gemm_work(rng):
+    # the measurement extension: with profiling on, every GEMM launch is timed
+    # and its algorithmic flops (8 per complex multiply-add) are summed; for
+    # <A|B> that is, per site, L*B (Da_l x d Db_r x Db_l) and the environment
+    # update (Da_r x Db_r x Da_l d)
+    N, D, d = 7, 8, 3
+    a, da = O.mpsrandom(N, D, d, rng)
+    b, db = O.mpsrandom(N, 5, d, rng)
+    ga, gb = dev_state(a), dev_state(b)
+    ctx = M.default_context()
+    ref = M.scal_prod(ga, gb)
+    ctx.set_profiling(True)
+    try:
+        ctx.kernel_stats()
+        got = M.scal_prod(ga, gb)
+        st = ctx.kernel_stats()
+    finally:
+        ctx.set_profiling(False)
+    flops = sum(8.0 * da[j] * d * db[j + 1] * db[j] + 8.0 * da[j + 1] * db[j + 1] * da[j] * d for j in range(N))
+    print(st, flops, flush=True)
+    assert got == ref
+    assert st["gemm_launches"] == 2 * N
+    assert st["gemm_flops"] == flops
+    assert st["gemm_ms"] > 0
