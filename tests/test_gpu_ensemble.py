@@ -70,3 +70,37 @@ def test_batched_context_refuses_single_copy_calls():
     v = ens.Av[0]
     assert abs(M.scal_prod(v, v) - 1.0) < 1e-12
     ens.close()
+
+
+def test_normalize_lo_batch_per_copy():
+    # normalize_lo! of every lockstep copy in one call (orthonormalisation.jl:97-121):
+    # copies hold different left-canonical states scaled differently; each
+    # returned norm^2 and each normalised copy equal the oracle's on that copy
+    from oracle import mps_oracle as O
+    from helpers import copy_sites
+    from mpsamd.trajectory import _state_bytes
+    N, D, d, nb = 6, 8, 3, 3
+    rng = np.random.default_rng(21)
+    states = []
+    for b in range(nb):
+        s, dims = O.mpsrandom(N, D, d, rng)
+        s[-1] = s[-1] * (0.5 + b)
+        states.append(s)
+    ctx = M.Context.batched(nb, _state_bytes(dims, d) + 4096, M.batch_workspace_bytes(nb, D, d, 1), 0)
+    comp = ctx.companion()
+    g = M.MPS(N, d, dims, ctx)
+    views = [g.view(b, comp) for b in range(nb)]
+    for v, s in zip(views, states):
+        v.upload(s)
+    norms = M.normalize_lo_batch(g)
+    for b in range(nb):
+        ref = copy_sites(states[b])
+        n_ref = O.normalize_lo_(ref)
+        assert abs(norms[b] - n_ref) < 1e-12 * n_ref, b
+        got = views[b].to_host()
+        assert max(np.abs(x - y).max() for x, y in zip(got, ref)) < 1e-12, b
+    for v in views:
+        v.__del__()
+    g.__del__()
+    comp.close()
+    ctx.close()
