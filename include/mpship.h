@@ -44,7 +44,12 @@ extern "C" {
 #define MPS_E_STATE (-4)    /* call not valid in the current state          */
 #define MPS_E_NUMERIC (-5)  /* numerical failure (e.g. SVD did not converge) */
 
-#define MPS_PREC_FP64 0     /* plain path: complex FP64 on MFMA (4M)        */
+/* Both precisions form complex products with three real MFMA (3M: ArBr,
+ * AiBi, (Ar+Ai)(Br+Bi)).  The rounding bound is normwise, about
+ * 3 K eps |a||b| per complex dot product of length K, not componentwise: a
+ * real or imaginary part far below |a||b| has an absolute error of that
+ * size (eps = 2^-53 for FP64, 2^-24 for FP32). */
+#define MPS_PREC_FP64 0     /* plain path: complex FP64 on MFMA (3M)        */
 #define MPS_PREC_FP32 1     /* precision form: operator terms in complex FP32 (3M) */
 
 typedef struct mps_ctx_s* mps_ctx_t;
@@ -83,6 +88,17 @@ int mps_ctx_synchronize(mps_ctx_t ctx);
 int mps_ctx_set_qr_rank_tol(mps_ctx_t ctx, double tol);
 /* MPS_PREC_FP64 (default) or MPS_PREC_FP32 for the operator (MPO) contractions. */
 int mps_ctx_set_precision(mps_ctx_t ctx, int prec);
+/* Refinement of the precision form (no counterpart in the reference; it is
+ * the reference's own compress_sum_var_apply_H_left! / _right!,
+ * compression.jl:889-944, 1007-1063, called again with the last RK4 stage's
+ * arguments): with n > 0 (even, at most 64), every mps_rk4stp_apply_H_half
+ * call on this context follows stage 4 with n half-sweeps (left, right, ...)
+ * in complex FP64 that refit stage 4's target -1/3 psi + 1/3 psi1 +
+ * 2/3 psi2 + 1/3 H3 psi3, each started from the previous result; the result
+ * stays left canonical.  The context keeps one or two states of psi's shape
+ * for it (the 10-argument form's copy of psi and the refit's intermediate).
+ * Default 0.  MPS_E_STATE on a batched context. */
+int mps_ctx_set_refine_sweeps(mps_ctx_t ctx, int n);
 /* Enable / disable hipGraph capture of whole RK4 steps (default on). */
 int mps_ctx_set_graphs(mps_ctx_t ctx, int enable);
 /* Measurement extension (no counterpart in the reference): when enabled, every

@@ -23,11 +23,13 @@ struct mps_ctx_s {
     int prec;
     double tol;
     size_t ws_gen;
+    int refine;
     bool operator<(const Key& o) const {
       if (h != o.h) return h < o.h;
       if (dt != o.dt) return dt < o.dt;
       if (prec != o.prec) return prec < o.prec;
       if (tol != o.tol) return tol < o.tol;
+      if (refine != o.refine) return refine < o.refine;
       return ws_gen < o.ws_gen;
     }
   };
@@ -37,6 +39,9 @@ struct mps_ctx_s {
   // second operator environment for the RK4 stages whose environments are
   // built on the aux stream (same slots as the caller's envop)
   std::unique_ptr<Env> shadow;
+  // RK4 refinement states (mps_ctx_set_refine_sweeps): psi's copy for the
+  // 10-argument form and the refit's intermediate state
+  std::unique_ptr<State> rkeep, rtmp;
   // the next call's stage-1 environments, queued by the last RK4 call: valid
   // for a call with these handles and precision if no writing entry point
   // ran since (g_epoch == spec_epoch + 1 inside that call)
@@ -134,6 +139,35 @@ static void track_ws(mps_ctx_s* x, size_t before) {
     x->ws_gen++;
     x->clear_graphs();
   }
+}
+
+// A context-owned state shaped like `like` (its bonds at most the capacity);
+// reallocated, and the context's graphs dropped, when it does not fit.
+static State* internal_like(mps_ctx_s* x, std::unique_ptr<State>& s, const State& like) {
+  bool fits = s && s->N == like.N && s->d == like.d;
+  for (int j = 0; fits && j < like.N; ++j) fits = like.dims[j] <= s->cap_l[j] && like.dims[j + 1] <= s->cap_r[j];
+  if (!fits) {
+    x->clear_graphs();
+    s.reset();
+    auto n = std::make_unique<State>();
+    n->ctx = &x->c;
+    n->N = like.N;
+    n->d = like.d;
+    n->dims = like.dims;
+    n->cap_l.assign(like.dims.begin(), like.dims.end() - 1);
+    n->cap_r.assign(like.dims.begin() + 1, like.dims.end());
+    n->off.resize(like.N);
+    size_t tot = 0;
+    for (int j = 0; j < like.N; ++j) {
+      n->off[j] = (int64_t)tot;
+      tot += (size_t)like.dims[j] * like.d[j] * like.dims[j + 1];
+    }
+    n->total = tot;
+    n->buf = (double2*)x->c.alloc(tot * sizeof(double2), n->own);
+    s = std::move(n);
+  }
+  s->dims = like.dims;
+  return s.get();
 }
 
 extern "C" {
@@ -268,6 +302,16 @@ int mps_ctx_set_precision(mps_ctx_t ctx, int prec) {
   return guard([&] {
     REQUIRE(prec == MPS_PREC_FP64 || prec == MPS_PREC_FP32, "unknown precision");
     ctx->c.prec = prec;
+  });
+}
+
+int mps_ctx_set_refine_sweeps(mps_ctx_t ctx, int n) {
+  return guard([&] {
+    REQUIRE(n >= 0 && n <= 64 && n % 2 == 0, "refinement half-sweeps: an even number in [0, 64]");
+    if (n > 0 && ctx->c.bat.nb != 1)
+      throw Error(MPS_E_STATE, "RK4 refinement is not available on a batched context");
+    ctx->c.refine = n;
+    invalidate(ctx);
   });
 }
 
@@ -1284,7 +1328,7 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     Ctx* c = &x->c;
     set_device(c);
     mps_ctx_s::Key key{{H1, H2, H3, psi0, psi, psi1, psi2, psi3, env1, env2, env3, envop}, dt, c->prec,
-                       c->qr_tol, x->ws_gen};
+                       c->qr_tol, x->ws_gen, c->refine};
     auto run = [&](Plan& p) {
       rk4_half(p, dt, &H1->o, &H2->o, &H3->o, &out->s, &psi->s, &psi1->s, &psi2->s, &psi3->s, &env1->e,
                &env2->e, &env3->e, &envop->e);
@@ -1292,6 +1336,11 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     size_t before = c->ws_bytes;
     Plan p(c, {&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s}, {&H1->o, &H2->o, &H3->o}, 4);
     track_ws(x, before);
+    if (c->refine > 0) {
+      if (c->bat.nb != 1) throw Error(MPS_E_STATE, "RK4 refinement is not available on a batched context");
+      p.rtmp = internal_like(x, x->rtmp, psi->s);
+      if (out == psi) p.rkeep = internal_like(x, x->rkeep, psi->s);
+    }
     if (p.conc) {
       const Env& like = envop->e;
       if (!x->shadow || x->shadow->N != like.N || x->shadow->off != like.off || x->shadow->total != like.total) {
@@ -1325,7 +1374,7 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     bool own_all = true;
     for (auto h : {H1, H2, H3}) own_all = own_all && h->owner == x;
     for (auto e : {env1, env2, env3, envop}) own_all = own_all && e->owner == x;
-    for (auto s : {out, psi1, psi2, psi3}) own_all = own_all && s->owner == x;
+    for (auto s : {out, psi, psi1, psi2, psi3}) own_all = own_all && s->owner == x;
     const bool had_spec = x->spec;
     x->spec = false;
     if (!c->graphs || c->profiling || p.conc || !own_all) {

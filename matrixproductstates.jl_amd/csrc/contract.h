@@ -14,6 +14,7 @@ struct Ctx {
   int num_cus = 256;
   double qr_tol = 0.0;
   int prec = 0;          // 0 FP64, 1 FP32 operator terms
+  int refine = 0;        // FP64 half-sweeps refitting the last RK4 stage (even)
   bool graphs = true;
   // workspace arena (grow only, outside graph capture)
   char* ws = nullptr;

@@ -5,12 +5,16 @@
 // offset is given by an IdxMap, so the permutedims/reshape chains of the
 // reference (mps_base.jl) cost nothing.
 //
-//   CMP_F64     complex FP64, 4 real MFMA per complex MAC
-//               (v_mfma_f64_16x16x4_f64, FP64 accumulation)
-//   CMP_F32_3M  complex FP32, 3 real MFMA per complex MAC (Gauss/3M:
+//   CMP_F64     complex FP64, 3 real MFMA per complex MAC (Gauss/3M:
 //               P1 = Ar Br, P2 = Ai Bi, P3 = (Ar+Ai)(Br+Bi);
-//               Re = P1 - P2, Im = P3 - P1 - P2), v_mfma_f32_16x16x4_f32,
+//               Re = P1 - P2, Im = P3 - P1 - P2), v_mfma_f64_16x16x4_f64,
+//               FP64 accumulation
+//   CMP_F32_3M  complex FP32, the same 3M products on v_mfma_f32_16x16x4_f32,
 //               FP32 accumulation.
+// 3M rounding is normwise: |C - fl(C)| <= c K eps |A| |B| elementwise in the
+// moduli of whole complex rows/columns (not componentwise in Re/Im), with c
+// about 3 against 2 for the four-product form; an imaginary part much
+// smaller than |A||B| carries an absolute, not relative, error of that size.
 //
 // Workgroup tile TM x TM (complex; TM = 64 or 32), K step 16, 256 threads =
 // 4 waves of 64, each wave a TM/2 x TM/2 block of MFMA 16x16 tiles (2x2 or

@@ -38,6 +38,10 @@ struct Plan {
   Work aux_w{};
   void *Ta = nullptr, *ALa = nullptr, *splitk3 = nullptr;
   Env* shadow = nullptr;
+  // RK4 refinement (ctx->refine > 0): a copy of psi taken before the 10-argument
+  // form overwrites it, and the intermediate state of the refit sweeps
+  State* rkeep = nullptr;
+  State* rtmp = nullptr;
   Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
        int n_ovl, size_t extra_bytes = 0);
   Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl, 0); }
@@ -67,6 +71,8 @@ struct NextInit {
   Env* oe[3] = {nullptr, nullptr, nullptr};
   State* oA[3] = {nullptr, nullptr, nullptr};
   std::complex<double> ocoef[3] = {};
+  // arithmetic of these builds (nullptr: the sweep's own, Plan::op)
+  const Op* op = nullptr;
 };
 
 void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,

@@ -257,9 +257,10 @@ static void nx_right_step(Plan& p, const NextInit* nx, State* A0, int j) {
   aux_follow(p);
   Env* e = nx->e;
   MPO* H = nx->H;
-  prod_AOR(p.aux_w, p.op, p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->site(j), H->wl(j), H->wr(j),
+  const Op& op = nx->op ? *nx->op : p.op;
+  prod_AOR(p.aux_w, op, p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->site(j), H->wl(j), H->wr(j),
            e->slot(j + 1), A0->Dr(j));
-  update_re(p.aux_w, p.op, e->slot(j), p.Ta, A0->site(j), A0->Dl(j), H->wl(j), A0->d[j], A0->Dr(j),
+  update_re(p.aux_w, op, e->slot(j), p.Ta, A0->site(j), A0->Dl(j), H->wl(j), A0->d[j], A0->Dr(j),
             A0->Dl(j));
   // update_renv!(oe[j], A0[j], oA[j], oe[j+1]), as the next sweep's own build
   for (int k = 0; k < nx->n_ovl; ++k) {
@@ -287,9 +288,10 @@ static void nx_left_step(Plan& p, const NextInit* nx, State* A0, int j) {
   aux_follow(p);
   Env* e = nx->e;
   MPO* H = nx->H;
-  prod_AOL(p.aux_w, p.op, p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->site(j), H->wl(j), H->wr(j),
+  const Op& op = nx->op ? *nx->op : p.op;
+  prod_AOL(p.aux_w, op, p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->site(j), H->wl(j), H->wr(j),
            e->slot(j), A0->Dl(j));
-  update_le(p.aux_w, p.op, e->slot(j + 1), p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->wr(j),
+  update_le(p.aux_w, op, e->slot(j + 1), p.Ta, A0->site(j), A0->Dl(j), A0->d[j], A0->Dr(j), H->wr(j),
             A0->Dr(j));
   // update_lenv!(oe[j+1], A0[j], oA[j], oe[j]), as the next sweep's own build
   for (int k = 0; k < nx->n_ovl; ++k) {
@@ -625,9 +627,32 @@ void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const cd
 // previous sweep, site by site as that sweep's output becomes final, into
 // the shadow environment (stages 2, 4) or envop (stage 3).  The steps and
 // their order per environment are those of init_{left,right}_op.
+//
+// Refinement (ctx->refine = n > 0, even): after stage 4, n FP64 half-sweeps
+// (left, right, ...) refit stage 4's target -1/3 psi + 1/3 psi1 + 2/3 psi2 +
+// 1/3 H3 psi3, each started from the previous one's result, exactly as a
+// caller would by calling compress_sum_var_apply_H_left! / _right!
+// (compression.jl:889-944, 1007-1063) again with stage 4's arguments; the
+// last (right) sweep leaves `out` left canonical as stage 4 does.  The
+// 10-argument form overwrites psi in stage 4, so psi is first copied into
+// p.rkeep, which then stands for psi in stage 4 and the refit (the same
+// values: stage 4's arithmetic is unchanged).
 void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
               State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop, bool s1_done, bool next) {
   const bool ahead = p.conc && p.shadow != nullptr;
+  const int nref = p.ctx->refine;
+  State* psi_t = psi;   // psi as the first term of stage 4 (and of the refit)
+  if (nref > 0) {
+    REQUIRE(nref % 2 == 0 && p.rtmp != nullptr && (out != psi || p.rkeep != nullptr),
+            "RK4 refinement: an even number of half-sweeps and their states");
+    if (out == psi) {
+      p.rkeep->dims = psi->dims;
+      for (int j = 0; j < psi->N; ++j)
+        launch_copy_c(p.rkeep->site(j), psi->site(j), psi->size(j), p.ctx->stream, p.ctx->bat);
+      psi_t = p.rkeep;
+    }
+    p.rtmp->dims = psi->dims;
+  }
   Env* E2 = ahead ? p.shadow : envop;   // stages 2 and 4
   const cd c2[2] = {1.0, dt / 2}, c3[2] = {1.0, dt}, c4[4] = {-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3};
   // Overlap environments: stage 2's env1 (psi vs psi1) is built behind stage
@@ -658,14 +683,33 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
     State* A[2] = {psi, psi2};
     compress_sum_var_apply_H_left(p, psi3, e, envop, psi2, 2, c3, A, H2, ahead ? &n4 : nullptr, ahead);
   }
+  // a following call's stage 1 (init_left_op on out, H1, far-end
+  // coefficient 1, the context's precision) into envop, which stage 4 and
+  // the refit leave alone; queued behind the call's last sweep
+  const Op ctx_op = p.op;
+  NextInit n1{envop, H1, 1.0};
+  n1.op = &ctx_op;
   {
     Env* e[3] = {env1, env2, env3};
-    State* A[4] = {psi, psi1, psi2, psi3};
+    State* A[4] = {psi_t, psi1, psi2, psi3};
     if (ahead) set1(p.ctx, env1->slot(0), c4[0]);
-    // a following call's stage 1 (init_left_op on out, H1, far-end
-    // coefficient 1) into envop, which stage 4 leaves alone
-    const NextInit n1{envop, H1, 1.0};
-    compress_sum_var_apply_H_right(p, out, e, E2, psi3, 4, c4, A, H3, (ahead && next) ? &n1 : nullptr, ahead);
+    compress_sum_var_apply_H_right(p, out, e, E2, psi3, 4, c4, A, H3,
+                                   (ahead && next && nref == 0) ? &n1 : nullptr, ahead);
+  }
+  if (nref > 0) {
+    p.op = Op{CMP_F64, F64};
+    Env* e[3] = {env1, env2, env3};
+    State* A[4] = {psi_t, psi1, psi2, psi3};
+    for (int k = 0; k < nref; ++k) {
+      if (k % 2 == 0) {
+        compress_sum_var_apply_H_left(p, p.rtmp, e, E2, out, 4, c4, A, H3);
+      } else {
+        const bool last = k == nref - 1;
+        compress_sum_var_apply_H_right(p, out, e, E2, p.rtmp, 4, c4, A, H3,
+                                       (ahead && next && last) ? &n1 : nullptr, false);
+      }
+    }
+    p.op = ctx_op;
   }
   if (ahead && next) aux_wait(p);   // the call's work is complete on the context's stream
 }

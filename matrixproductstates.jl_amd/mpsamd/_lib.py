@@ -41,6 +41,7 @@ SIGNATURES = {
     "mps_ctx_set_qr_rank_tol": [vp, dbl],
     "mps_ctx_set_precision": [vp, i32],
     "mps_ctx_set_graphs": [vp, i32],
+    "mps_ctx_set_refine_sweeps": [vp, i32],
     "mps_ctx_set_profiling": [vp, i32],
     "mps_ctx_kernel_stats": [vp, PD],
     "mps_state_create": [vp, i32, P64, P64, PVP],

@@ -78,6 +78,10 @@ class Context:
     def set_precision(self, prec: str):
         call("mps_ctx_set_precision", self.h, {"fp64": MPS_PREC_FP64, "fp32": MPS_PREC_FP32}[prec])
 
+    def set_refine_sweeps(self, n: int):
+        """FP64 half-sweeps refitting the last RK4 stage after every RK4 call (even n; 0 = off)."""
+        call("mps_ctx_set_refine_sweeps", self.h, int(n))
+
     def set_graphs(self, enable: bool):
         call("mps_ctx_set_graphs", self.h, 1 if enable else 0)
 
