@@ -271,6 +271,19 @@ int mps_scal_prod(mps_state_t A, mps_state_t B, double* out);
 int mps_scal_op_prod(mps_state_t A, mps_mpo_t O, mps_state_t B, double* out);
 /* scal_prod_no_conj (MatrixProductStates.jl:366-379): bilinear A . B */
 int mps_scal_prod_no_conj(mps_state_t A, mps_state_t B, double* out);
+/* scal_prod_no_conj without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as mps_scal_prod_no_conj (its GEMMs do not split
+ * K, so the last bits may differ).  It works in a fixed 24 MiB device buffer
+ * of the library, one per GPU, held until the process ends: the two running
+ * environments and the per-site product need
+ * 32 (max_j dims_A[j] dims_B[j]) + 16 max_j (Dl_A(j) d_j Dr_B(j)) bytes
+ * (D = 256, d = 9: about 11.5 MiB); larger inputs return MPS_E_ARG.  Calls
+ * on one GPU are serialised by a lock and synchronise before returning, so
+ * they never overlap on the buffer; nothing in it must start at zero.
+ * MPS_E_STATE on a batched context or with kernel timing on. */
+int mps_scal_prod_no_conj_noalloc(mps_state_t A, mps_state_t B, double* out);
 /* measure_excitations! (spin_model_quantum_trajectory_rk4.jl:220-231):
  * out[n] = real(local_exp(A[n] R, A[n], op)) for a left-canonical, normalised state. */
 int mps_measure_local(mps_state_t A, const double* op /* d x d complex */, double* out /* N */);

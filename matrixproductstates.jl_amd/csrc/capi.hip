@@ -141,6 +141,43 @@ static void track_ws(mps_ctx_s* x, size_t before) {
   }
 }
 
+// ----------------------------------------------------- the no-alloc buffer
+// mps_scal_prod_no_conj_noalloc works in this fixed device array instead of
+// the context's workspace: the two running environments, the per-site
+// product and the result of one call.  Every GPU the library runs on holds
+// its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for it with the device current
+// (scratch_on(device)).  Calls through the entry point on one GPU are
+// serialised by a per-device lock held from the first launch to the final
+// synchronisation, so they cannot overlap on the buffer.
+static constexpr size_t kNoAllocScratchBytes = (size_t)24 << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_init_mu;
+static std::mutex g_scratch_call_mu[kMaxDevices];
+static char* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device`, resolved once per device with
+// that device current (called at context creation, the library's set-up)
+static char* scratch_on(int device) {
+  REQUIRE(device >= 0 && device < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    HIPCHK(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess || ptr == nullptr)
+      throw Error(MPS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (char*)ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
 // A context-owned state shaped like `like` (its bonds at most the capacity);
 // reallocated, and the context's graphs dropped, when it does not fit.
 static State* internal_like(mps_ctx_s* x, std::unique_ptr<State>& s, const State& like) {
@@ -182,6 +219,7 @@ int mps_ctx_create(int device, mps_ctx_t* out) {
     try {
       x->c.device = device;
       HIPCHK(hipSetDevice(device));
+      (void)scratch_on(device);
       hipDeviceProp_t prop;
       HIPCHK(hipGetDeviceProperties(&prop, device));
       x->c.num_cus = prop.multiProcessorCount;
@@ -1447,6 +1485,41 @@ int mps_scal_prod(mps_state_t A, mps_state_t B, double* out) {
 
 int mps_scal_prod_no_conj(mps_state_t A, mps_state_t B, double* out) {
   return guard_ro([&] { scal_common(A, B, 2, out); });
+}
+
+// scal_prod_no_conj in the fixed buffer `scratch`: no device allocation, no
+// split-K partials (each GEMM sums K in one pass), result through the
+// context's pinned host scalars.
+int mps_scal_prod_no_conj_noalloc(mps_state_t A, mps_state_t B, double* out) {
+  return guard_ro([&] {
+    REQUIRE(A && B && out, "invalid arguments");
+    State& a = A->s;
+    State& b = B->s;
+    check_states({&a, &b});
+    Ctx* c = a.ctx;
+    if (c->bat.nb != 1) throw Error(MPS_E_STATE, "not available on a batched context (use a view of one copy)");
+    if (c->profiling) throw Error(MPS_E_STATE, "kernel timing is on; it creates events on every launch");
+    i64 env = 1, al = 1;
+    for (int j = 0; j <= a.N; ++j) env = std::max(env, a.dims[j] * b.dims[j]);
+    for (int j = 0; j < a.N; ++j) al = std::max(al, a.Dl(j) * b.d[j] * b.Dr(j));
+    auto r256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t eb = r256((size_t)env * sizeof(double2)), ab = r256((size_t)al * sizeof(double2));
+    const size_t need = 2 * eb + ab + 256;
+    REQUIRE(need <= kNoAllocScratchBytes,
+            "scal_prod_no_conj without allocation needs " + std::to_string(need) +
+                " bytes of the library's fixed device buffer, which holds " +
+                std::to_string(kNoAllocScratchBytes) + " (use mps_scal_prod_no_conj)");
+    set_device(c);
+    char* base = scratch_on(c->device);
+    std::lock_guard<std::mutex> lk(g_scratch_call_mu[c->device]);
+    Work w;
+    w.c = c;   // no split-K buffer: GEMMs are not split
+    scal_prod_ws(w, &a, &b, 2, base, base + eb, base + 2 * eb, base + 2 * eb + ab);
+    HIPCHK(hipMemcpyAsync(c->hscal, base + 2 * eb + ab, sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    out[0] = c->hscal[0];
+    out[1] = c->hscal[1];
+  });
 }
 
 int mps_scal_op_prod(mps_state_t A, mps_mpo_t O, mps_state_t B, double* out) {

@@ -68,6 +68,7 @@ struct Gemm {
   double alpha_re = 1, alpha_im = 0, beta_re = 0, beta_im = 0;
   double diag_re = 0;   // added to C[i][i] after alpha/beta (Q = I - V W)
   Bat bat;              // lockstep copies (A, B, C and the split-K partials all offset)
+  int max_splits = 0;   // cap on the split-K depth (0: none; 1: no partials buffer needed)
 };
 
 enum Compute { CMP_F64 = 0, CMP_F32_3M = 1 };

@@ -121,7 +121,7 @@ struct Op {
   DType tmp = F64;   // storage of T1/T2 intermediates
 };
 
-// on stream s (nullptr: the context's stream)
+// on stream s (nullptr: the context's stream); splitk == nullptr: K is not split
 void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s = nullptr);
 
 // --------------------------------------------- L1 contractions (mps_base.jl)

@@ -143,6 +143,7 @@ static double esize(DType t) { return t == F64 ? 16.0 : 8.0; }
 
 void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s) {
   g.bat = c->bat;
+  if (splitk == nullptr) g.max_splits = 1;   // no partials buffer: K is not split
   if (!c->profiling || c->capturing) {
     launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, s ? s : c->stream);
     return;

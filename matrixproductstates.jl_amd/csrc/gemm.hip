@@ -403,6 +403,7 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
     if (splits > 32) splits = 32;
     if (splits < 1) splits = 1;
   }
+  if (g.max_splits > 0 && splits > g.max_splits) splits = g.max_splits;
   int64_t kchunk = (g.K + splits - 1) / splits;
   kchunk = ((kchunk + BK - 1) / BK) * BK;
   splits = (int)((g.K + kchunk - 1) / kchunk);

@@ -96,6 +96,7 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
               State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop, bool s1_done = false,
               bool next = false);
 void scal_prod(Plan& p, State* A, State* B, int mode, int slot);
+void scal_prod_ws(Work& w, State* A, State* B, int mode, void* envA, void* envB, void* AL, void* out);
 void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot);
 
 }  // namespace mpship

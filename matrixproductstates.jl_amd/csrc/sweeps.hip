@@ -717,17 +717,24 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
 // ------------------------------------------------------ scalar products
 // scal_prod / scal_prod_no_conj (MatrixProductStates.jl:345-379) -> dscal[slot..slot+1]
 void scal_prod(Plan& p, State* A, State* B, int mode, int slot) {
-  Ctx* c = p.ctx;
-  double2* cur = (double2*)p.envA;
-  double2* nxt = (double2*)p.envB;
-  set1(c, cur, 1.0);
+  scal_prod_ws(p.work, A, B, mode, p.envA, p.envB, p.AL, p.ctx->dscal + slot);
+}
+
+// The same chain on caller-provided buffers: envA / envB hold the running
+// environment (at most max_j Dl_A(j) Dl_B(j) complex), AL the per-site
+// product (at most max_j Dl_A(j) d_j Dr_B(j) complex); the scalar goes to out.
+void scal_prod_ws(Work& w, State* A, State* B, int mode, void* envA, void* envB, void* AL, void* out) {
+  Ctx* c = w.c;
+  double2* cur = (double2*)envA;
+  double2* nxt = (double2*)envB;
+  set1(w, cur, 1.0);
   for (int j = 0; j < A->N; ++j) {
     // AL = prod_LR(lenv, B[j]); lenv = update_lenv(A[j], AL)
-    prod_LR_23(p.work, p.AL, cur, B->site(j), A->Dl(j), B->Dl(j), B->d[j], B->Dr(j), false, false);
-    update_lenv(p.work, nxt, A->site(j), p.AL, A->Dl(j), A->d[j], A->Dr(j), B->Dr(j), mode);
+    prod_LR_23(w, AL, cur, B->site(j), A->Dl(j), B->Dl(j), B->d[j], B->Dr(j), false, false);
+    update_lenv(w, nxt, A->site(j), AL, A->Dl(j), A->d[j], A->Dr(j), B->Dr(j), mode);
     std::swap(cur, nxt);
   }
-  launch_copy_c(c->dscal + slot, cur, 1, c->stream, c->bat);
+  launch_copy_c(out, cur, 1, w.stream ? w.stream : c->stream, c->bat);
 }
 
 // scal_op_prod (MatrixProductStates.jl:327-340): conj(<A|O|B>) -> dscal[slot..]

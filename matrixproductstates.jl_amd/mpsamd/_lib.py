@@ -92,6 +92,7 @@ SIGNATURES = {
     "mps_scal_prod": [vp, vp, PD],
     "mps_scal_op_prod": [vp, vp, vp, PD],
     "mps_scal_prod_no_conj": [vp, vp, PD],
+    "mps_scal_prod_no_conj_noalloc": [vp, vp, PD],
     "mps_measure_local": [vp, PD, PD],
     "mps_apply_site_operator": [vp, i32, PD],
 }

@@ -532,6 +532,11 @@ def diff_norm(mps1, mps2):
     return normMPS(mps1) + normMPS(mps2) - 2 * scal_prod(mps1, mps2).real
 
 
+def scal_prod_no_conj_noalloc(mps1, mps2):
+    """scal_prod_no_conj in the library's fixed device buffer (no device allocation)."""
+    return _c2("mps_scal_prod_no_conj_noalloc", mps1.h, mps2.h)
+
+
 def measure_excitations_(out, mps, op):
     """spin_model_quantum_trajectory_rk4.jl:220-231 (left-canonical, normalised mps)."""
     keep, p = _cbuf(np.asarray(op, dtype=CDT))
