@@ -70,6 +70,21 @@ class RydbergEvolution:
         tr = M.scal_prod_no_conj(self.idm, self.rho)
         return np.array([tr.real, (M.scal_prod_no_conj(self.nrm, self.rho) / tr).real])
 
+    def checkpoint(self, path, **meta):
+        """Snapshot of rho between steps (the reference's _temp.mat role,
+        rydberg_density_matrix_gpu.jl:251-255), in the mpsamd.checkpoint format."""
+        from .checkpoint import save_state
+        save_state(path, self.rho, kind="rydberg_evolution", N=self.N, D=self.D, dt=self.dt, **meta)
+
+    def resume(self, path):
+        """Continue from a snapshot (rydberg_pollution_density_matrix_gpu_restart.jl:
+        151-161): later steps equal the uninterrupted run's bit for bit."""
+        from .checkpoint import load_state
+        _, meta = load_state(path, into=self.rho)
+        if meta.get("kind") != "rydberg_evolution" or meta["N"] != self.N or meta["D"] != self.D:
+            raise ValueError(f"{path}: not a snapshot of this evolution's shape")
+        return meta
+
     def footprint_bytes(self):
         site = sum(self.dims[j] * 9 * self.dims[j + 1] for j in range(self.N))
         ovl = sum(self.dims[j] ** 2 for j in range(self.N + 1))
@@ -135,9 +150,47 @@ def _result(solve):
     return solve.result()
 
 
-def run_batch(reqs, sequential=False):
-    """Run every request to completion; returns wall seconds.  ``sequential``
-    solves them one after another on one context (the plain baseline)."""
+def snapshot_requests(reqs, directory):
+    """Snapshot of in-flight requests between ticks: each request's solve state
+    (``<directory>/req<rid>.npz``) and a manifest of the requests (steps left,
+    kind, shape, seed) in ``<directory>/requests.json``."""
+    import json
+    import os
+    os.makedirs(directory, exist_ok=True)
+    man = []
+    for r in reqs:
+        path = os.path.join(directory, f"req{r.rid}.npz")
+        r.solve.checkpoint(path)
+        man.append({"rid": r.rid, "kind": r.kind, "N": r.N, "D": r.D, "steps": r.steps, "seed": r.seed,
+                    "left": r.left, "arrival": r.arrival, "state": os.path.basename(path)})
+    tmp = os.path.join(directory, "requests.json.tmp")
+    with open(tmp, "w") as f:
+        json.dump(man, f)
+    os.replace(tmp, os.path.join(directory, "requests.json"))
+
+
+def restore_requests(directory):
+    """The requests of a snapshot, each with its solve rebuilt on a fresh
+    context and resumed, ready for further ticks (``run_batch(..., resumed=True)``)."""
+    import json
+    import os
+    with open(os.path.join(directory, "requests.json")) as f:
+        man = json.load(f)
+    out = []
+    for m in man:
+        r = Request(m["rid"], m["kind"], m["N"], m["D"], m["steps"], m["seed"], m["arrival"])
+        r.solve = make_solve(r, M.Context(0))
+        r.solve.resume(os.path.join(directory, m["state"]))
+        r.left = m["left"]
+        out.append(r)
+    return out
+
+
+def run_batch(reqs, sequential=False, max_ticks=None, resumed=False):
+    """Run every request to completion (or for ``max_ticks`` ticks, leaving the
+    unfinished ones in flight); returns wall seconds.  ``sequential`` solves
+    them one after another on one context (the plain baseline); ``resumed``:
+    the requests already carry their solves (``restore_requests``)."""
     t0 = time.perf_counter()
     if sequential:
         ctx = M.Context(0)
@@ -156,10 +209,12 @@ def run_batch(reqs, sequential=False):
         return time.perf_counter() - t0
     active = []
     for r in reqs:
-        r.solve = make_solve(r, M.Context(0))
-        r.left = r.steps
+        if not resumed:
+            r.solve = make_solve(r, M.Context(0))
+            r.left = r.steps
         active.append(r)
-    while active:
+    ticks = 0
+    while active and (max_ticks is None or ticks < max_ticks):
         for r in active:
             r.solve.launch_coherent()
         for r in active:
@@ -170,6 +225,7 @@ def run_batch(reqs, sequential=False):
             release(r.solve)
             r.solve = None
             active.remove(r)
+        ticks += 1
     return time.perf_counter() - t0
 
 

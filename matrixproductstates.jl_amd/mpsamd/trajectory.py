@@ -94,6 +94,26 @@ class SpinTrajectory:
         self.launch_coherent()
         return self.finish_step()
 
+    # ---------------------------------------------------- checkpoint / resume
+    def checkpoint(self, path):
+        """Snapshot between steps: the state A, the draws' generator state, the
+        time, the p_coh record and the jump record (mpsamd.checkpoint format)."""
+        from .checkpoint import save_state
+        save_state(path, self.A, kind="spin_trajectory", N=self.N, D=self.D, seed=self.seed, t=self.t,
+                   rng=self.rng.bit_generator.state, p_coh=self.p_coh, jumps=self.jumps)
+
+    def resume(self, path):
+        """Continue from a snapshot of a trajectory of the same system: the
+        following steps equal the uninterrupted run's, bit for bit."""
+        from .checkpoint import load_state
+        _, meta = load_state(path, into=self.A)
+        if meta.get("kind") != "spin_trajectory" or meta["N"] != self.N or meta["D"] != self.D:
+            raise ValueError(f"{path}: not a snapshot of this trajectory's shape")
+        self.rng.bit_generator.state = meta["rng"]
+        self.t = float(meta["t"])
+        self.p_coh = [float(x) for x in meta["p_coh"]]
+        self.jumps = [(int(i), str(k), int(j)) for i, k, j in meta["jumps"]]
+
     def sigma_z(self):
         """<sigma_z^j> = 2 <n_e^j> - 1 (measure_excitations!, :220-231)."""
         return 2.0 * M.measure_excitations_(None, self.A, HEE) - 1.0
