@@ -473,6 +473,7 @@ def main():
                 "launches": st["gemm_launches"], "avg_launch_us": round(st["gemm_ms"] * 1e3 / max(st["gemm_launches"], 1), 2)}
         prof = {"gemm_ms_per_step": st["gemm_ms"] / args.steps, "qr_ms_per_step": st["qr_ms"] / args.steps,
                 "qr_launches_per_step": st["qr_launches"] / args.steps,
+                "launches_checked_in_process": st["launches_checked"],
                 "gemm_gflop_per_step": st["gemm_flops"] / args.steps / 1e9}
         print(f"profile: {prof}", file=sys.stderr, flush=True)
 

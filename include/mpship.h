@@ -105,7 +105,11 @@ int mps_ctx_set_graphs(mps_ctx_t ctx, int enable);
  * GEMM and QR launch is bracketed by HIP events on the context stream and
  * graphs are not used.  mps_ctx_kernel_stats synchronises, fills
  * out[0..7] = {gemm_ms, gemm_flops, gemm_bytes, gemm_launches, qr_ms,
- * qr_flops, qr_launches, reserved} (algorithmic flops/bytes) and resets. */
+ * qr_flops, qr_launches, launches} (algorithmic flops/bytes) and resets;
+ * launches = kernel launches the library has checked and queued in this
+ * process so far (every launch is checked for a non-empty grid and block, the
+ * kernel's thread limit, 32-bit grid sizes and the device's LDS limit, and
+ * refused with MPS_E_STATE otherwise). */
 int mps_ctx_set_profiling(mps_ctx_t ctx, int enable);
 int mps_ctx_kernel_stats(mps_ctx_t ctx, double* out);
 
@@ -117,6 +121,12 @@ int mps_state_destroy(mps_state_t s);
 int mps_state_dims(mps_state_t s, int64_t* dims_out /* N+1 */);
 int mps_state_upload_site(mps_state_t s, int site, const double* host, int64_t dl, int64_t d, int64_t dr);
 int mps_state_download_site(mps_state_t s, int site, double* host, int64_t dl, int64_t d, int64_t dr);
+/* The same with ComplexF32 host buffers (interleaved floats), the
+ * TN = Complex{Float32} arrays of the reference's GPU examples
+ * (rydberg_density_matrix_gpu.jl:124-125): widened to / narrowed from the
+ * library's complex FP64 storage on the device.  Plain contexts only. */
+int mps_state_upload_site_f32(mps_state_t s, int site, const float* host, int64_t dl, int64_t d, int64_t dr);
+int mps_state_download_site_f32(mps_state_t s, int site, float* host, int64_t dl, int64_t d, int64_t dr);
 /* dst .= copy.(src)  (e.g. spin_model_quantum_trajectory_rk4.jl:130) */
 int mps_state_copy(mps_state_t dst, mps_state_t src);
 /* mpsproductstate (constructors.jl:8-32, 37-61): local state vectors
@@ -127,6 +137,8 @@ int mps_state_product(mps_state_t s, const double* states);
 int mps_mpo_create(mps_ctx_t ctx, int N, const int64_t* d, const int64_t* wdims /* N+1 */, mps_mpo_t* out);
 int mps_mpo_destroy(mps_mpo_t o);
 int mps_mpo_upload_site(mps_mpo_t o, int site, const double* host, int64_t wl, int64_t d, int64_t wr);
+/* ComplexF32 MPO site (makempo of a Complex{Float32} TN), widened on the device. */
+int mps_mpo_upload_site_f32(mps_mpo_t o, int site, const float* host, int64_t wl, int64_t d, int64_t wr);
 /* O[site][wl_idx, :, wr_idx, :] = op (d x d) -- the per-step writes of
  * update_h! (spin_model_quantum_trajectory_rk4.jl:274-284) and
  * update_L_Ryd! (rydberg_density_matrix_gpu.jl:393-403). */

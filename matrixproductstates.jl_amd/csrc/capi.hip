@@ -388,6 +388,7 @@ int mps_ctx_kernel_stats(mps_ctx_t ctx, double* out) {
         v[6] += 1;
       }
     }
+    v[7] = (double)launches_checked();
     std::memcpy(out, v, sizeof(v));
     c.prof.clear();
     c.ev_used = 0;
@@ -466,6 +467,46 @@ int mps_state_download_site(mps_state_t s, int site, double* host, int64_t dl, i
     set_device(st.ctx);
     HIPCHK(hipMemcpyAsync(host, st.site(site), dl * d * dr * sizeof(double2), hipMemcpyDeviceToHost,
                           st.ctx->stream));
+    st.ctx->sync();
+  });
+}
+
+// ComplexF32 host buffers (Julia Array{ComplexF32}, the TN = Complex{Float32}
+// of the reference's GPU examples): staged in the context's workspace and
+// widened / narrowed on the device; the library stores and computes complex
+// FP64 (or FP32 in the precision form's operator terms).
+static void* f32_stage(Ctx* c, int64_t n) {
+  REQUIRE(c->bat.nb == 1, "ComplexF32 transfers run on plain contexts (use a view of one copy)");
+  c->ensure_ws((size_t)n * sizeof(float2) + 256);
+  return c->ws;
+}
+
+int mps_state_upload_site_f32(mps_state_t s, int site, const float* host, int64_t dl, int64_t d, int64_t dr) {
+  return guard([&] {
+    State& st = s->s;
+    REQUIRE(host != nullptr, "null host buffer");
+    REQUIRE(site >= 0 && site < st.N, "site out of range");
+    REQUIRE(dl == st.Dl(site) && d == st.d[site] && dr == st.Dr(site), "site shape mismatch");
+    set_device(st.ctx);
+    const int64_t n = dl * d * dr;
+    void* stage = f32_stage(st.ctx, n);
+    HIPCHK(hipMemcpyAsync(stage, host, n * sizeof(float2), hipMemcpyHostToDevice, st.ctx->stream));
+    launch_convert(st.site(site), F64, stage, F32, n, st.ctx->stream);
+    st.ctx->sync();
+  });
+}
+
+int mps_state_download_site_f32(mps_state_t s, int site, float* host, int64_t dl, int64_t d, int64_t dr) {
+  return guard_ro([&] {
+    State& st = s->s;
+    REQUIRE(host != nullptr, "null host buffer");
+    REQUIRE(site >= 0 && site < st.N, "site out of range");
+    REQUIRE(dl == st.Dl(site) && d == st.d[site] && dr == st.Dr(site), "site shape mismatch");
+    set_device(st.ctx);
+    const int64_t n = dl * d * dr;
+    void* stage = f32_stage(st.ctx, n);
+    launch_convert(stage, F32, st.site(site), F64, n, st.ctx->stream);
+    HIPCHK(hipMemcpyAsync(host, stage, n * sizeof(float2), hipMemcpyDeviceToHost, st.ctx->stream));
     st.ctx->sync();
   });
 }
@@ -563,6 +604,21 @@ int mps_mpo_upload_site(mps_mpo_t o, int site, const double* host, int64_t wl, i
     REQUIRE(wl == m.wl(site) && d == m.d[site] && wr == m.wr(site), "MPO site shape mismatch");
     set_device(m.ctx);
     upload_copies(m.ctx, m.site(site), host, wl * d * wr * d * sizeof(double2));
+    m.ctx->sync();
+  });
+}
+
+int mps_mpo_upload_site_f32(mps_mpo_t o, int site, const float* host, int64_t wl, int64_t d, int64_t wr) {
+  return guard([&] {
+    MPO& m = o->o;
+    REQUIRE(host != nullptr, "null host buffer");
+    REQUIRE(site >= 0 && site < m.N, "site out of range");
+    REQUIRE(wl == m.wl(site) && d == m.d[site] && wr == m.wr(site), "MPO site shape mismatch");
+    set_device(m.ctx);
+    const int64_t n = wl * d * wr * d;
+    void* stage = f32_stage(m.ctx, n);
+    HIPCHK(hipMemcpyAsync(stage, host, n * sizeof(float2), hipMemcpyHostToDevice, m.ctx->stream));
+    launch_convert(m.site(site), F64, stage, F32, n, m.ctx->stream);
     m.ctx->sync();
   });
 }

@@ -27,6 +27,23 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::mpship::Error(-1, std::string("DimensionMismatch: ") + (msg)); \
   } while (0)
 
+// ------------------------------------------------------- launch checks
+// Every kernel launch goes through MPS_LAUNCH, which checks the dispatch
+// before it is queued: each grid and block dimension at least 1, the block
+// within the kernel's maximum threads (hipFuncGetAttributes, cached per
+// kernel), grid x block within 32 bits per dimension (the AQL packet's grid
+// size is in work-items), and static plus dynamic LDS within the device's
+// per-workgroup limit.  A failed check throws (MPS_E_STATE) instead of
+// queueing a malformed packet.
+void check_launch(const void* kernel, const char* name, dim3 grid, dim3 block, size_t dyn_lds);
+uint64_t launches_checked();
+
+#define MPS_LAUNCH(K, G, B, L, S, ...)                                                          \
+  do {                                                                                          \
+    ::mpship::check_launch(reinterpret_cast<const void*>(&(K)), #K, dim3(G), dim3(B), (size_t)(L)); \
+    hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                            \
+  } while (0)
+
 // ------------------------------------------------------------- index maps
 // A tensor index i (a flattened group of reference indices) maps to the
 // element offset (i % n0) * s0 + (i / n0) * s1.  n0 = 0 means a single

@@ -427,9 +427,9 @@ static void launch_kf(const GemmParams& p, bool two, bool kf, dim3 grid, hipStre
 #define MPS_GEMM_LAUNCH(TWO_, KF_)                                                                    \
   do {                                                                                            \
     if constexpr (F64)                                                                            \
-      hipLaunchKernelGGL((zgemm_f64_kernel<TM, TA, TB, TC, TWO_, KF_>), grid, dim3(NT), 0, s, p);   \
+      MPS_LAUNCH((zgemm_f64_kernel<TM, TA, TB, TC, TWO_, KF_>), grid, dim3(NT), 0, s, p);   \
     else                                                                                          \
-      hipLaunchKernelGGL((zgemm_f32_3m_kernel<TM, TA, TB, TC, TWO_, KF_>), grid, dim3(NT), 0, s, p); \
+      MPS_LAUNCH((zgemm_f32_3m_kernel<TM, TA, TB, TC, TWO_, KF_>), grid, dim3(NT), 0, s, p); \
   } while (0)
   if (two) {
     if (kf) MPS_GEMM_LAUNCH(true, true); else MPS_GEMM_LAUNCH(true, false);
@@ -492,9 +492,9 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
     p.splits = 0;
     dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536), (unsigned)g.bat.nb);
     if (g.tc == F64)
-      hipLaunchKernelGGL(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
+      MPS_LAUNCH(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
     else
-      hipLaunchKernelGGL(splitk_reduce_kernel<float2>, rg, dim3(256), 0, s, p);
+      MPS_LAUNCH(splitk_reduce_kernel<float2>, rg, dim3(256), 0, s, p);
     return;
   }
   if (pl.splits > 1) {
@@ -515,9 +515,9 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
   if (pl.splits > 1) {
     dim3 rg((unsigned)std::min<int64_t>((g.M * g.N * g.batch + 255) / 256, 65536), (unsigned)g.bat.nb);
     if (g.tc == F64)
-      hipLaunchKernelGGL(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
+      MPS_LAUNCH(splitk_reduce_kernel<double2>, rg, dim3(256), 0, s, p);
     else
-      hipLaunchKernelGGL(splitk_reduce_kernel<float2>, rg, dim3(256), 0, s, p);
+      MPS_LAUNCH(splitk_reduce_kernel<float2>, rg, dim3(256), 0, s, p);
     HIPCHK(hipGetLastError());
   }
 }

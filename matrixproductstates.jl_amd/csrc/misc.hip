@@ -1,7 +1,59 @@
 // Small device kernels: boundary-coefficient writes, norms, rescaling, copies.
 #include "common.h"
 
+#include <atomic>
+#include <mutex>
+#include <unordered_map>
+
 namespace mpship {
+
+namespace {
+struct LaunchLimits {
+  int max_threads;
+  size_t static_lds;
+};
+std::mutex g_launch_mu;
+std::unordered_map<const void*, LaunchLimits> g_launch_cache;
+std::atomic<uint64_t> g_launches{0};
+size_t g_device_lds = 0;
+}  // namespace
+
+void check_launch(const void* kernel, const char* name, dim3 grid, dim3 block, size_t dyn_lds) {
+  auto fail = [&](const std::string& why) {
+    throw Error(-4, std::string("invalid launch of ") + name + ": " + why);
+  };
+  if (grid.x < 1 || grid.y < 1 || grid.z < 1) fail("empty grid");
+  if (block.x < 1 || block.y < 1 || block.z < 1) fail("empty block");
+  const uint64_t threads = (uint64_t)block.x * block.y * block.z;
+  if ((uint64_t)grid.x * block.x > 0xffffffffull || (uint64_t)grid.y * block.y > 0xffffffffull ||
+      (uint64_t)grid.z * block.z > 0xffffffffull)
+    fail("grid size exceeds 32 bits");
+  LaunchLimits lim;
+  {
+    std::lock_guard<std::mutex> lk(g_launch_mu);
+    auto it = g_launch_cache.find(kernel);
+    if (it == g_launch_cache.end()) {
+      hipFuncAttributes a;
+      HIPCHK(hipFuncGetAttributes(&a, kernel));
+      if (g_device_lds == 0) {
+        int dev = 0, v = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        g_device_lds = (size_t)v;
+      }
+      it = g_launch_cache.emplace(kernel, LaunchLimits{a.maxThreadsPerBlock, a.sharedSizeBytes}).first;
+    }
+    lim = it->second;
+  }
+  if (threads > (uint64_t)lim.max_threads)
+    fail("block of " + std::to_string(threads) + " threads, kernel maximum " + std::to_string(lim.max_threads));
+  if (lim.static_lds + dyn_lds > g_device_lds)
+    fail("LDS " + std::to_string(lim.static_lds + dyn_lds) + " bytes, device limit " + std::to_string(g_device_lds));
+  g_launches.fetch_add(1, std::memory_order_relaxed);
+}
+
+uint64_t launches_checked() { return g_launches.load(std::memory_order_relaxed); }
+
 
 // Batched copies: blockIdx.y selects the copy (byte offset blockIdx.y * bs).
 __global__ void set_scalar_kernel(double2* p, double re, double im, int64_t bs) {
@@ -9,7 +61,7 @@ __global__ void set_scalar_kernel(double2* p, double re, double im, int64_t bs) 
 }
 
 void launch_set_scalar(void* p, double re, double im, hipStream_t s, Bat bat) {
-  hipLaunchKernelGGL(set_scalar_kernel, dim3(1, bat.nb), dim3(64), 0, s, (double2*)p, re, im, bat.bs);
+  MPS_LAUNCH(set_scalar_kernel, dim3(1, bat.nb), dim3(64), 0, s, (double2*)p, re, im, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
@@ -23,7 +75,7 @@ __global__ void fill_zero_kernel(double2* p, int64_t n, int64_t bs) {
 void launch_fill_zero(void* p, int64_t n, hipStream_t s, Bat bat) {
   if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(fill_zero_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)p, n, bat.bs);
+  MPS_LAUNCH(fill_zero_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)p, n, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
@@ -48,7 +100,7 @@ __global__ __launch_bounds__(1024) void sum_abs2_kernel(const double2* p, int64_
 }
 
 void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s, Bat bat) {
-  hipLaunchKernelGGL(sum_abs2_kernel, dim3(1, bat.nb), dim3(1024), 0, s, (const double2*)p, n, out, bat.bs);
+  MPS_LAUNCH(sum_abs2_kernel, dim3(1, bat.nb), dim3(1024), 0, s, (const double2*)p, n, out, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
@@ -65,8 +117,9 @@ __global__ void scale_rsqrt_kernel(double2* p, int64_t n, const double* norm2, i
 }
 
 void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s, Bat bat) {
+  if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(scale_rsqrt_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)p, n, norm2, bat.bs);
+  MPS_LAUNCH(scale_rsqrt_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)p, n, norm2, bat.bs);
   HIPCHK(hipGetLastError());
 }
 
@@ -81,7 +134,7 @@ __global__ void copy_c_kernel(double2* d, const double2* s, int64_t n, int64_t b
 void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s, Bat bat) {
   if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(copy_c_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)dst, (const double2*)src, n,
+  MPS_LAUNCH(copy_c_kernel, dim3(g, bat.nb), dim3(256), 0, s, (double2*)dst, (const double2*)src, n,
                      bat.bs);
   HIPCHK(hipGetLastError());
 }
@@ -111,7 +164,7 @@ void launch_transpose_c(void* dst, const void* src, int64_t rows, int64_t cols, 
   if (rows <= 0 || cols <= 0) return;
   REQUIRE((rows + 31) / 32 < (1LL << 31) && (cols + 31) / 32 <= 65535, "transpose too large");
   const dim3 grid((unsigned)((rows + 31) / 32), (unsigned)((cols + 31) / 32), (unsigned)bat.nb);
-  hipLaunchKernelGGL(transpose_c_kernel, grid, dim3(256), 0, s, (double2*)dst, (const double2*)src, rows, cols,
+  MPS_LAUNCH(transpose_c_kernel, grid, dim3(256), 0, s, (double2*)dst, (const double2*)src, rows, cols,
                      bat.bs);
   HIPCHK(hipGetLastError());
 }
@@ -129,16 +182,16 @@ void launch_convert(void* dst, DType td, const void* src, DType ts, int64_t n, h
   if (n <= 0) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
   if (td == F32 && ts == F64)
-    hipLaunchKernelGGL((convert_kernel<float2, double2>), dim3(g), dim3(256), 0, s, (float2*)dst,
+    MPS_LAUNCH((convert_kernel<float2, double2>), dim3(g), dim3(256), 0, s, (float2*)dst,
                        (const double2*)src, n);
   else if (td == F64 && ts == F32)
-    hipLaunchKernelGGL((convert_kernel<double2, float2>), dim3(g), dim3(256), 0, s, (double2*)dst,
+    MPS_LAUNCH((convert_kernel<double2, float2>), dim3(g), dim3(256), 0, s, (double2*)dst,
                        (const float2*)src, n);
   else if (td == F64)
-    hipLaunchKernelGGL((convert_kernel<double2, double2>), dim3(g), dim3(256), 0, s, (double2*)dst,
+    MPS_LAUNCH((convert_kernel<double2, double2>), dim3(g), dim3(256), 0, s, (double2*)dst,
                        (const double2*)src, n);
   else
-    hipLaunchKernelGGL((convert_kernel<float2, float2>), dim3(g), dim3(256), 0, s, (float2*)dst,
+    MPS_LAUNCH((convert_kernel<float2, float2>), dim3(g), dim3(256), 0, s, (float2*)dst,
                        (const float2*)src, n);
   HIPCHK(hipGetLastError());
 }
@@ -156,7 +209,7 @@ __global__ void trace_kernel(const double2* M, int64_t d, double2* out) {
 }
 
 void launch_trace_small(const void* M, int64_t d, void* out, hipStream_t s) {
-  hipLaunchKernelGGL(trace_kernel, dim3(1), dim3(64), 0, s, (const double2*)M, d, (double2*)out);
+  MPS_LAUNCH(trace_kernel, dim3(1), dim3(64), 0, s, (const double2*)M, d, (double2*)out);
   HIPCHK(hipGetLastError());
 }
 

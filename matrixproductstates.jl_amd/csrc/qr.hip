@@ -1209,15 +1209,15 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
   const unsigned nb = (unsigned)bat.nb;
   const int64_t bs = bat.bs;
   if (tol > 0.0)
-    hipLaunchKernelGGL(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(64), 0, s, A, m, rs, cs, colnrm, bs);
+    MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(64), 0, s, A, m, rs, cs, colnrm, bs);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
-    hipLaunchKernelGGL((qr_panel_kernel<R, NB, NT, PT>), dim3(1, nb), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
+    MPS_LAUNCH((qr_panel_kernel<R, NB, NT, PT>), dim3(1, nb), dim3(NT), 0, s, A, m, n, rs, cs, c0, nbp,
                        tol, colnrm, Vs, tau, Tp, bs);
     const int trail = n - c0 - nbp;
     if (trail > 0)
-      hipLaunchKernelGGL((qr_apply_col_kernel<RA, NB, NTA, !PT>), dim3((unsigned)trail, nb), dim3(NTA), 0, s, A,
+      MPS_LAUNCH((qr_apply_col_kernel<RA, NB, NTA, !PT>), dim3((unsigned)trail, nb), dim3(NTA), 0, s, A,
                          m, rs, cs, c0, nbp, Vs, Tp, tau, bs);
   }
   QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
@@ -1227,22 +1227,22 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
     launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
     const dim3 grid((unsigned)((n + 3) / 4), nb);
     if (n <= 64)
-      hipLaunchKernelGGL((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
+      MPS_LAUNCH((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
                          bs);
     else if (n <= 128)
-      hipLaunchKernelGGL((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
+      MPS_LAUNCH((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
                          bs);
     else
-      hipLaunchKernelGGL((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
+      MPS_LAUNCH((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
                          bs);
     launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
   } else {
     if (!PT) {
       launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
-      hipLaunchKernelGGL((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB), nb), dim3(64), 0, s, q.G,
+      MPS_LAUNCH((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB), nb), dim3(64), 0, s, q.G,
                          n, tau, Tall, bs);
     }
-    hipLaunchKernelGGL((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4), nb), dim3(WY_NT), 0, s, A, m,
+    MPS_LAUNCH((qr_formq_kernel<MAXR, NB>), dim3((unsigned)((n + 3) / 4), nb), dim3(WY_NT), 0, s, A, m,
                        rs, cs, Vs, Tall, R_out, n, bs);
   }
 }
@@ -1315,7 +1315,7 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
   else if (m <= 2560)
     qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else
-    hipLaunchKernelGGL(qr_householder_kernel, dim3(1, (unsigned)bat.nb), dim3(QR_NT), 0, s, A, (int)m, (int)n, rs,
+    MPS_LAUNCH(qr_householder_kernel, dim3(1, (unsigned)bat.nb), dim3(QR_NT), 0, s, A, (int)m, (int)n, rs,
                        cs, tol, q.tau, R, bat.bs);
   HIPCHK(hipGetLastError());
 }

@@ -153,21 +153,21 @@ __global__ void svd_gather_kernel(double2* __restrict__ out, const double2* __re
 static unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 4096); }
 
 void svd_load(void* Y, const void* X, int64_t m, int64_t n, bool conj_transpose, hipStream_t s) {
-  hipLaunchKernelGGL(svd_load_kernel, dim3(grid_for(m * n)), dim3(256), 0, s, (double2*)Y, (const double2*)X,
+  MPS_LAUNCH(svd_load_kernel, dim3(grid_for(m * n)), dim3(256), 0, s, (double2*)Y, (const double2*)X,
                      (int)m, (int)n, conj_transpose ? 1 : 0);
   HIPCHK(hipGetLastError());
 }
 
 int svd_jacobi(void* Y, int64_t m, int64_t n, void* J, double tol, int* flag_dev, int* flag_host,
                int max_sweeps, hipStream_t s) {
-  hipLaunchKernelGGL(svd_identity_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, (double2*)J, (int)n, flag_dev);
+  MPS_LAUNCH(svd_identity_kernel, dim3(grid_for(n * n)), dim3(256), 0, s, (double2*)J, (int)n, flag_dev);
   HIPCHK(hipGetLastError());
   if (n < 2) return 0;
   const int np = (int)(n + (n & 1));
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (sweep > 0) hipLaunchKernelGGL(svd_clear_flag_kernel, dim3(1), dim3(64), 0, s, flag_dev);
+    if (sweep > 0) MPS_LAUNCH(svd_clear_flag_kernel, dim3(1), dim3(64), 0, s, flag_dev);
     for (int round = 0; round < np - 1; ++round)
-      hipLaunchKernelGGL(svd_jacobi_round_kernel, dim3((unsigned)(np / 2)), dim3(SVD_NT), 0, s, (double2*)Y,
+      MPS_LAUNCH(svd_jacobi_round_kernel, dim3((unsigned)(np / 2)), dim3(SVD_NT), 0, s, (double2*)Y,
                          (int)m, (int)n, (double2*)J, round, np, tol, flag_dev);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(flag_host, flag_dev, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -178,13 +178,13 @@ int svd_jacobi(void* Y, int64_t m, int64_t n, void* J, double tol, int* flag_dev
 }
 
 void svd_colnorms(const void* Y, int64_t m, int64_t n, double* sigma, hipStream_t s) {
-  hipLaunchKernelGGL(svd_colnorm_kernel, dim3((unsigned)n), dim3(SVD_NT), 0, s, (const double2*)Y, (int)m, sigma);
+  MPS_LAUNCH(svd_colnorm_kernel, dim3((unsigned)n), dim3(SVD_NT), 0, s, (const double2*)Y, (int)m, sigma);
   HIPCHK(hipGetLastError());
 }
 
 void svd_gather(void* out, const void* S, int64_t ld, const int* perm, const double* scale, int64_t rows,
                 int64_t cols, bool conj, bool by_col, hipStream_t s) {
-  hipLaunchKernelGGL(svd_gather_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, (double2*)out,
+  MPS_LAUNCH(svd_gather_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, (double2*)out,
                      (const double2*)S, ld, perm, scale, (int)rows, (int)cols, conj ? 1 : 0, by_col ? 1 : 0);
   HIPCHK(hipGetLastError());
 }

@@ -22,6 +22,7 @@ i32 = C.c_int
 dbl = C.c_double
 P64 = C.POINTER(C.c_int64)
 PD = C.POINTER(C.c_double)
+PF = C.POINTER(C.c_float)
 PVP = C.POINTER(C.c_void_p)
 
 # name -> argtypes (all return int status except mps_last_error)
@@ -49,6 +50,9 @@ SIGNATURES = {
     "mps_state_dims": [vp, P64],
     "mps_state_upload_site": [vp, i32, PD, i64, i64, i64],
     "mps_state_download_site": [vp, i32, PD, i64, i64, i64],
+    "mps_state_upload_site_f32": [vp, i32, PF, i64, i64, i64],
+    "mps_state_download_site_f32": [vp, i32, PF, i64, i64, i64],
+    "mps_mpo_upload_site_f32": [vp, i32, PF, i64, i64, i64],
     "mps_state_copy": [vp, vp],
     "mps_state_product": [vp, PD],
     "mps_mpo_create": [vp, i32, P64, P64, PVP],

@@ -100,7 +100,7 @@ class Context:
         out = np.zeros(8)
         call("mps_ctx_kernel_stats", self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return {"gemm_ms": out[0], "gemm_flops": out[1], "gemm_bytes": out[2], "gemm_launches": int(out[3]),
-                "qr_ms": out[4], "qr_flops": out[5], "qr_launches": int(out[6])}
+                "qr_ms": out[4], "qr_flops": out[5], "qr_launches": int(out[6]), "launches_checked": int(out[7])}
 
 
 def batch_workspace_bytes(nb: int, Dmax: int, d: int, wmax: int, device: int = 0) -> int:
@@ -173,6 +173,23 @@ class MPS:
             call("mps_state_upload_site", self.h, j, p, *a.shape)
         return self
 
+    def upload_f32(self, sites: Sequence[np.ndarray]):
+        """Upload complex64 sites (the reference GPU examples' Complex{Float32})."""
+        for j, a in enumerate(sites):
+            a = np.asarray(a, dtype=np.complex64)
+            if a.shape != self.shape(j):
+                raise _lib.DimensionMismatch(-1, f"site {j}: {a.shape} != {self.shape(j)}")
+            flat = np.ascontiguousarray(a.reshape(-1, order="F")).view(np.float32)
+            call("mps_state_upload_site_f32", self.h, j, flat.ctypes.data_as(C.POINTER(C.c_float)), *a.shape)
+        return self
+
+    def site_f32(self, j) -> np.ndarray:
+        """Site j as complex64 (rounded on the device)."""
+        sh = self.shape(j)
+        buf = np.empty(int(np.prod(sh)) * 2, dtype=np.float32)
+        call("mps_state_download_site_f32", self.h, j, buf.ctypes.data_as(C.POINTER(C.c_float)), *sh)
+        return buf.view(np.complex64).reshape(sh, order="F")
+
     def site(self, j) -> np.ndarray:
         sh = self.shape(j)
         buf = np.empty(int(np.prod(sh)) * 2, dtype=np.float64)
@@ -196,9 +213,12 @@ class MPS:
 
     @classmethod
     def from_host(cls, sites, ctx=None):
+        """A device MPS of host sites; complex64 sites go through the ComplexF32 upload."""
         N = len(sites)
         dims = [s.shape[0] for s in sites] + [sites[-1].shape[2]]
         m = cls(N, [s.shape[1] for s in sites], dims, ctx)
+        if all(np.asarray(s).dtype == np.complex64 for s in sites):
+            return m.upload_f32(sites)
         return m.upload(sites)
 
 
@@ -221,6 +241,12 @@ class MPO:
             self.upload_site(j, s)
 
     def upload_site(self, j, s):
+        if np.asarray(s).dtype == np.complex64:
+            s = np.asarray(s)
+            flat = np.ascontiguousarray(s.reshape(-1, order="F")).view(np.float32)
+            call("mps_mpo_upload_site_f32", self.h, j, flat.ctypes.data_as(C.POINTER(C.c_float)),
+                 s.shape[0], s.shape[1], s.shape[2])
+            return
         s = np.asarray(s, dtype=CDT)
         keep, p = _cbuf(s)
         call("mps_mpo_upload_site", self.h, j, p, s.shape[0], s.shape[1], s.shape[2])
