@@ -276,6 +276,24 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
 int mps_arnoldi_expm(mps_mpo_t A, int numv, const mps_state_t* rhov, mps_env_t env,
                      mps_env_t envop, double* coef_out);
 
+/* steady_state_search! (steadystate.jl:6-86): sweeps of local
+ * smallest-eigenvalue problems of <At| L^dag L |At> over the sites of At
+ * (left canonical on entry, not necessarily normalised; overwritten with the
+ * result).  W is the MPO of Lconj * L that the reference contracts through
+ * its two-operator environments: W = applyMPOtoMPO(L, Lconj)
+ * (MatrixProductStates.jl:67-99), bond w_L * w_Lconj; envop is an operator
+ * environment for W's bonds.  Each local problem (the reference's
+ * eigs(...; nev = 1, which = :SR, v0 = At[jj], tol = eigs_tol)) is solved
+ * densely on the device for local spaces of at most 1536 elements (one-sided
+ * Jacobi SVD of the positive semidefinite local operator), otherwise by a
+ * Lanczos iteration from the current site to a residual below
+ * eigs_tol * max(|theta|, the largest Ritz value).  Stops as the reference
+ * does, after maxsweeps or when |ae| > converge_rat |ae at the sweep's first
+ * site| or |ae| < 1e-10.  evalue_out: the last local eigenvalue;
+ * sweeps_out: sweeps run. */
+int mps_steady_state_search(mps_state_t At, mps_mpo_t W, mps_env_t envop, int maxsweeps, double converge_rat,
+                            double eigs_tol, double* evalue_out, int* sweeps_out);
+
 /* ------------------------------------ scalar products and observables (a22-a24) */
 /* scal_prod (MatrixProductStates.jl:345-358): <A|B> */
 int mps_scal_prod(mps_state_t A, mps_state_t B, double* out);

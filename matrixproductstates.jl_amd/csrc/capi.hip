@@ -1405,6 +1405,31 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
   });
 }
 
+// steady_state_search! (steadystate.jl:6-86)
+int mps_steady_state_search(mps_state_t At, mps_mpo_t W, mps_env_t envop, int maxsweeps, double converge_rat,
+                            double eigs_tol, double* evalue_out, int* sweeps_out) {
+  return guard([&] {
+    REQUIRE(At && W && envop, "invalid arguments");
+    State& s = At->s;
+    no_batch(s.ctx);
+    check_op_env(envop->e, s);
+    check_mpo(W->o, s);
+    check_env_fits(envop->e, s, s, &W->o);
+    REQUIRE(maxsweeps >= 0 && eigs_tol > 0.0, "maxsweeps >= 0 and eigs_tol > 0 required");
+    Ctx* c = s.ctx;
+    set_device(c);
+    constexpr int kmax = 48;
+    size_t before = c->ws_bytes;
+    Plan p(c, {&s}, {&W->o}, 0, steady_state_extra_bytes(s.max_site(), kmax));
+    track_ws(At->owner, before);
+    p.op = Op{CMP_F64, F64};
+    int sw = 0;
+    const double ae = steady_state_search(p, &s, &W->o, &envop->e, maxsweeps, converge_rat, eigs_tol, kmax, &sw);
+    if (evalue_out) *evalue_out = ae;
+    if (sweeps_out) *sweeps_out = sw;
+  });
+}
+
 // ----------------------------------------------------------------- RK4
 int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3, mps_state_t psi0,
                             mps_state_t psi, mps_state_t psi1, mps_state_t psi2, mps_state_t psi3,

@@ -131,6 +131,8 @@ void launch_fill_zero(void* p, int64_t n_complex, hipStream_t s, Bat bat = Bat()
 void launch_sum_abs2(const void* p, int64_t n, double* out, hipStream_t s, Bat bat = Bat());
 void launch_scale_rsqrt(void* p, int64_t n, const double* norm2, hipStream_t s, Bat bat = Bat());
 void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s, Bat bat = Bat());
+void launch_conj_c(void* p, int64_t n, hipStream_t s);   // p <- conj(p), in place
+void launch_scale_c(void* p, int64_t n, double f, hipStream_t s);   // p <- f p, in place
 // dst (cols x rows) = transpose of src (rows x cols), both column major, complex FP64
 void launch_transpose_c(void* dst, const void* src, int64_t rows, int64_t cols, hipStream_t s, Bat bat = Bat());
 void launch_convert(void* dst, DType td, const void* src, DType ts, int64_t n, hipStream_t s);

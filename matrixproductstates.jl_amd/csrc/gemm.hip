@@ -405,7 +405,7 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   }
   if (g.max_splits > 0 && splits > g.max_splits) splits = g.max_splits;
   int64_t kchunk = (g.K + splits - 1) / splits;
-  kchunk = ((kchunk + BK - 1) / BK) * BK;
+  kchunk = std::max<int64_t>(((kchunk + BK - 1) / BK) * BK, BK);   // K = 0: one empty chunk
   splits = (int)((g.K + kchunk - 1) / kchunk);
   if (splits < 1) splits = 1;
   pl.splits = splits;

@@ -139,6 +139,30 @@ void launch_copy_c(void* dst, const void* src, int64_t n, hipStream_t s, Bat bat
   HIPCHK(hipGetLastError());
 }
 
+__global__ void scale_c_kernel(double2* p, int64_t n, double f) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_double2(f * p[i].x, f * p[i].y);
+}
+
+void launch_scale_c(void* p, int64_t n, double f, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  MPS_LAUNCH(scale_c_kernel, dim3(g), dim3(256), 0, s, (double2*)p, n, f);
+  HIPCHK(hipGetLastError());
+}
+
+__global__ void conj_c_kernel(double2* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i].y = -p[i].y;
+}
+
+void launch_conj_c(void* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  MPS_LAUNCH(conj_c_kernel, dim3(g), dim3(256), 0, s, (double2*)p, n);
+  HIPCHK(hipGetLastError());
+}
+
 // d (cols x rows, column major) = the transpose of s (rows x cols, column
 // major), through 32 x 32 LDS tiles so both sides are read and written in
 // contiguous column segments

@@ -95,6 +95,11 @@ void compress_sum_var_apply_H(Plan& p, State* A0, Env* envop, int numv, const st
 void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* psi, State* psi1,
               State* psi2, State* psi3, Env* env1, Env* env2, Env* env3, Env* envop, bool s1_done = false,
               bool next = false);
+// steady_state_search! (steadystate.jl:6-86) on the combined MPO W =
+// applyMPOtoMPO(L, Lconj); p.extra holds steady_state_extra_bytes(max site, kmax)
+size_t steady_state_extra_bytes(i64 site_elems, int kmax);
+double steady_state_search(Plan& p, State* At, MPO* W, Env* envop, int maxsweeps, double converge_rat,
+                           double eigs_tol, int kmax, int* sweeps_run);
 void scal_prod(Plan& p, State* A, State* B, int mode, int slot);
 void scal_prod_ws(Work& w, State* A, State* B, int mode, void* envA, void* envB, void* AL, void* out);
 void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot);

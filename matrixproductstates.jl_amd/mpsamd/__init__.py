@@ -13,7 +13,7 @@ from .core import (  # noqa: F401
     rightorthogonalizeQR_, rightorthonormalizationQR_, normalize_lo_, norm_lo,
     compress_var_apply_H_left_, compress_sum_var_apply_H_left_, compress_sum_var_apply_H_right_,
     compress_var_apply_H_, compress_var_, compress_sum_var, compress_sum_var_apply_H_,
-    compressionSVD_, RK4stp_apply_H_half_, Arnoldi_expm_,
+    compressionSVD_, RK4stp_apply_H_half_, Arnoldi_expm_, steady_state_search_,
     scal_prod, scal_op_prod, scal_prod_no_conj, normMPS, diff_norm,
     measure_excitations_, apply_site_operator_,
 )

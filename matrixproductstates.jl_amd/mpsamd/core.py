@@ -525,6 +525,22 @@ def Arnoldi_expm_(A, rhov, numv, env, envop):
     return coef.view(CDT)
 
 
+def steady_state_search_(At, L, Lconj, maxsweeps, converge_rat, eigs_tol, envop=None):
+    """steadystate.jl:6-86.  ``L`` and ``Lconj`` are host MPO site lists (the
+    reference passes its MPO arrays); their product W = Lconj * L is formed on
+    the host (applyMPOtoMPO) and the search runs on the device state ``At``.
+    Returns (last local eigenvalue, sweeps run)."""
+    from .observables import applyMPOtoMPO
+    W = MPO(applyMPOtoMPO(L, Lconj), At.ctx)
+    if envop is None:
+        envop = build_env(At.dims, At.dims, W.w, ctx=At.ctx)
+    ev = C.c_double()
+    sw = C.c_int()
+    call("mps_steady_state_search", At.h, W.h, envop.h, int(maxsweeps), float(converge_rat), float(eigs_tol),
+         C.byref(ev), C.byref(sw))
+    return float(ev.value), int(sw.value)
+
+
 # --------------------------------------------------------- scalar products
 
 def _c2(name, *h):

@@ -904,3 +904,79 @@ def Arnoldi_expm_(A, rhov, numv, env, envop):
     rhov[0] = compress_sum_var(rhov[0], list(coef), rhov, 1)
     normalize_lo_(rhov[0])
     return coef
+
+
+# --------------------------------------------------------------------------
+# steady-state search -- src/steadystate.jl, with the two-operator helpers of
+# src/MatrixProductStates.jl:263-312 restated as their @tensor index formulas
+# --------------------------------------------------------------------------
+
+def update_lenv_two_op(A0, O1, O2, A, FL):
+    """MatrixProductStates.jl:297-301: FL'[a',w1',w2',b'] =
+    sum FL[a,w1,w2,b] A[a,s,a'] O1[w1,t,w1',s] O2[w2,u,w2',t] conj(A0[b,u,b'])."""
+    T = np.einsum("awxb,asc->wxbsc", FL, A)
+    T = np.einsum("wxbsc,wtys->xbcty", T, O1)
+    T = np.einsum("xbcty,xuzt->bcyuz", T, O2)
+    return np.einsum("bcyuz,bud->cyzd", T, np.conj(A0))
+
+
+def update_renv_two_op(A0, O1, O2, A, FR):
+    """MatrixProductStates.jl:306-310."""
+    T = np.einsum("cyzd,asc->yzdas", FR, A)
+    T = np.einsum("yzdas,wtys->zdawt", T, O1)
+    T = np.einsum("zdawt,xuzt->dawxu", T, O2)
+    return np.einsum("dawxu,bud->awxb", T, np.conj(A0))
+
+
+def local_L_OO_R(FL, O1, O2, FR):
+    """The matrix of A_prod_L_OO_R_eigs (MatrixProductStates.jl:258-263) on
+    vec(A) (column-major): H[(b,u,d), (a,s,c)]."""
+    T = np.einsum("awxb,wtys->axbtys", FL, O1)          # contract w1
+    T = np.einsum("axbtys,xuzt->abysuz", T, O2)          # contract w2 and the middle index
+    H = np.einsum("abysuz,cyzd->budasc", T, FR)          # contract w1', w2'
+    n = H.shape[0] * H.shape[1] * H.shape[2]
+    return H.reshape(n, n, order="F")
+
+
+def _lowest(H):
+    """eigs(HM; nev = 1, which = :SR) of a Hermitian local operator, exactly."""
+    w, v = np.linalg.eigh(0.5 * (H + H.conj().T))
+    return w[0], v[:, 0]
+
+
+def steady_state_search_(At, L, Lconj, maxsweeps, converge_rat, eigs_tol=None):
+    """steadystate.jl:6-86 (in place on the site list At; returns (ae, sweeps)).
+    The local eigenproblems are solved exactly (numpy eigh) where the
+    reference calls ARPACK to eigs_tol."""
+    na = len(At)
+    FL = [None] * na
+    FR = [None] * na
+    FL[0] = np.ones((1, 1, 1, 1), dtype=CDT)
+    for j in range(na - 1):
+        FL[j + 1] = update_lenv_two_op(At[j], L[j], Lconj[j], At[j], FL[j])
+    FR[na - 1] = np.ones((1, 1, 1, 1), dtype=CDT)
+    ae = aestart = 0.0
+    sw = 0
+    for sw in range(1, maxsweeps + 1):
+        for j in range(na - 1, 0, -1):
+            Dl, d, Dr = At[j].shape
+            ae, av = _lowest(local_L_OO_R(FL[j], L[j], Lconj[j], FR[j]))
+            if j == na - 1:
+                aestart = ae
+            Q, R = thin_qr(_r(av, Dl, d * Dr).conj().T)
+            At[j] = _r(Q.conj().T, Dl, d, Dr)
+            FR[j - 1] = update_renv_two_op(At[j], L[j], Lconj[j], At[j], FR[j])
+            At[j - 1] = np.einsum("abl,kl->abk", At[j - 1], R)
+        for j in range(na - 1):
+            Dl, d, Dr = At[j].shape
+            ae, av = _lowest(local_L_OO_R(FL[j], L[j], Lconj[j], FR[j]))
+            Q, R = thin_qr(_r(av, Dl * d, Dr))
+            At[j] = _r(Q, Dl, d, Dr)
+            FL[j + 1] = update_lenv_two_op(At[j], L[j], Lconj[j], At[j], FL[j])
+            At[j + 1] = np.einsum("kl,lsb->ksb", R, At[j + 1])
+        Dl, d, Dr = At[na - 1].shape
+        ae, av = _lowest(local_L_OO_R(FL[na - 1], L[na - 1], Lconj[na - 1], FR[na - 1]))
+        At[na - 1] = _r(av, Dl, d, Dr)
+        if abs(ae) > converge_rat * abs(aestart) or abs(ae) < 1e-10:
+            break
+    return ae, sw

@@ -9,8 +9,9 @@ mpsgroundstate, RK4stp_apply_H_half! (10-argument form) with the QR rank
 cutoff 1e-13, for ``STEPS`` steps.  Per step it records the trace
 tr(rho) = scal_prod_no_conj(IDlmps, rho), the total populations
 scal_prod_no_conj(N{E,S}lmps, rho) / tr, the Hilbert-Schmidt norm <rho|rho>
-and three fingerprints <X_k|rho> against seeded random bond-2 MPS X_k (which
-differ for any two states that differ, while the fixture stays small).
+and three fingerprints <X_k|rho> against seeded vectorised product operators
+X_k (expectation values of random product observables, which differ for any
+two states that differ while the fixture stays small).
 
 These are oracle outputs, not closed forms: the device is compared with them
 to follow the oracle over many steps at the full size, where the oracle
@@ -35,14 +36,17 @@ STEPS = int(os.environ.get("STEPS", "20"))
 
 
 def fingerprint_mps(k, N=N):
-    """Seeded random bond-2 vectorised MPS (complex normal entries / 3)."""
+    """Seeded vectorised product operator, site j = vec(1 + e_k H_j) with H_j a
+    random Hermitian 3 x 3 matrix (e_k = 0.2, 0.5, 1.0): <X_k|rho> is then the
+    expectation value of a product observable, of order tr(rho) and sensitive
+    to every site's content."""
     rng = np.random.default_rng(1000 + k)
+    eps = (0.2, 0.5, 1.0)[k]
     sites = []
-    for j in range(N):
-        dl = 1 if j == 0 else 2
-        dr = 1 if j == N - 1 else 2
-        a = (rng.standard_normal((dl, 9, dr)) + 1j * rng.standard_normal((dl, 9, dr))) / 3.0
-        sites.append(a.astype(np.complex128))
+    for _ in range(N):
+        h = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+        a = np.eye(3) + eps * (h + h.conj().T) / 4
+        sites.append(a.reshape(1, 9, 1, order="F").astype(np.complex128))
     return sites
 
 

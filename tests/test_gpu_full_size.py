@@ -69,3 +69,91 @@ def test_rydberg_density_matrix_N100_D128_one_step():
     assert abs(ne_d - ne_o) < 1e-10 * abs(ne_o)
     assert abs(nr_d - nr_o) < 1e-10 * max(abs(nr_o), 1e-12) + 1e-20
     assert infid < 1e-12
+
+
+def _fixture():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rydberg_N100_D128_trace.npz")
+    return np.load(path)
+
+
+def test_rydberg_density_matrix_N100_D128_trajectory_against_oracle_fixture():
+    # The oracle's per-step trace, populations, <rho|rho> and three fingerprints
+    # <X_k|rho> (tests/golden/make_full_size_trace.py) over its recorded steps,
+    # against the device from the same start, at the cutoff both sides use.
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_full_size_trace import fingerprint_mps
+    f = _fixture()
+    N, D, dt, n_exp, seed = 100, 128, float(f["dt"]), 4, 1
+    p = models.RydbergParams(n_exp=n_exp)
+    assert np.allclose(p.u, f["u"], rtol=1e-12) and np.allclose(p.kappa, f["kappa"], rtol=1e-12)
+    z = models.positions(N, float(N), seed)
+    wop = [1] + [p.w] * (N - 1) + [1]
+    ctx = M.Context(0)
+    ctx.set_qr_rank_tol(float(f["tol"]))
+    rho, dims = M.mpsgroundstate(N, D, 9, ctx)
+    r = [rho.similar() for _ in range(3)]
+    for s in r:
+        s.copy_from(rho)
+    e = [M.build_env(dims, dims, ctx=ctx) for _ in range(3)]
+    eop = M.build_env(dims, dims, wop, ctx=ctx)
+    gL = [M.MPO(h, ctx) for h in models.rydberg_rk4_mpos(p, z, dt)]
+    idm = M.MPS.from_host(models.identity_vec_mps(N), ctx)
+    nem = M.MPS.from_host(models.sum_local_vec_mps(N, models.HEE3), ctx)
+    nrm = M.MPS.from_host(models.sum_local_vec_mps(N, models.HSS3), ctx)
+    X = [M.MPS.from_host(fingerprint_mps(k), ctx) for k in range(3)]
+    worst = {"trace": 0.0, "n_e": 0.0, "n_r": 0.0, "hs": 0.0, "fp": 0.0}
+    for s, _ in enumerate(f["steps"]):
+        M.RK4stp_apply_H_half_(dt, gL, rho, *r, *e, eop)
+        tr = M.scal_prod_no_conj(idm, rho)
+        got = {"trace": tr, "n_e": M.scal_prod_no_conj(nem, rho) / tr, "n_r": M.scal_prod_no_conj(nrm, rho) / tr,
+               "hs": M.scal_prod(rho, rho)}
+        for k in ("n_e", "n_r", "hs"):
+            worst[k] = max(worst[k], abs(got[k] - f[k][s]) / abs(f[k][s]))
+        worst["trace"] = max(worst["trace"], abs(got["trace"] - f["trace"][s]))
+        fp = np.array([M.scal_prod(x, rho) for x in X])
+        worst["fp"] = max(worst["fp"], float(np.max(np.abs(fp - f["fp"][s]) / np.abs(f["fp"][s]))))
+        print("step", s + 1, {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    assert worst["trace"] < 1e-10 and worst["n_e"] < 1e-10 and worst["hs"] < 1e-10
+    assert worst["n_r"] < 1e-9 and worst["fp"] < 1e-9
+
+
+def test_rydberg_density_matrix_N100_D128_200_steps_properties():
+    # cfg1's 200 steps at the full size: no divergence, the trace stays 1 up
+    # to the bond-128 truncation (measured: 1.2e-5 short after 200 steps),
+    # populations stay in [0, 1] per atom and rise monotonically under the
+    # weak probe from the ground state, and purity <rho|rho> stays <= 1.
+    N, D, dt, n_exp, seed = 100, 128, 0.005, 4, 1
+    p = models.RydbergParams(n_exp=n_exp)
+    z = models.positions(N, float(N), seed)
+    wop = [1] + [p.w] * (N - 1) + [1]
+    ctx = M.Context(0)
+    ctx.set_qr_rank_tol(1e-13)
+    rho, dims = M.mpsgroundstate(N, D, 9, ctx)
+    r = [rho.similar() for _ in range(3)]
+    for s in r:
+        s.copy_from(rho)
+    e = [M.build_env(dims, dims, ctx=ctx) for _ in range(3)]
+    eop = M.build_env(dims, dims, wop, ctx=ctx)
+    gL = [M.MPO(h, ctx) for h in models.rydberg_rk4_mpos(p, z, dt)]
+    idm = M.MPS.from_host(models.identity_vec_mps(N), ctx)
+    nem = M.MPS.from_host(models.sum_local_vec_mps(N, models.HEE3), ctx)
+    nrm = M.MPS.from_host(models.sum_local_vec_mps(N, models.HSS3), ctx)
+    prev = 0.0
+    for s in range(1, 201):
+        M.RK4stp_apply_H_half_(dt, gL, rho, *r, *e, eop)
+        if s % 20 == 0:
+            tr = M.scal_prod_no_conj(idm, rho)
+            ne = (M.scal_prod_no_conj(nem, rho) / tr)
+            nr = (M.scal_prod_no_conj(nrm, rho) / tr)
+            hs = M.scal_prod(rho, rho).real
+            print("step", s, "trace", tr, "n_e", ne, "n_r", nr, "purity", hs, flush=True)
+            assert abs(tr.real - 1.0) < 1e-4 and abs(tr.imag) < 1e-6
+            assert 0.0 < ne.real < N and 0.0 < nr.real < N and abs(ne.imag) < 1e-6
+            assert ne.real > prev
+            assert hs <= 1.0 + 1e-10
+            prev = ne.real
+    pops = [M.scal_prod_no_conj(M.MPS.from_host(models.local_population_vec_mps(N, j, models.HEE3), ctx), rho).real
+            for j in range(N)]
+    assert min(pops) >= -1e-8 and max(pops) <= 1.0
