@@ -276,6 +276,25 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
 int mps_arnoldi_expm(mps_mpo_t A, int numv, const mps_state_t* rhov, mps_env_t env,
                      mps_env_t envop, double* coef_out);
 
+/* Two-site variational compression with SVD truncation:
+ * compress_var_two_site (compression.jl:110-164), compress_var_apply_H_two_site
+ * {,_left,_right} (:355-486), compress_sum_var_two_site (:574-647) and
+ * compress_sum_var_apply_H_two_site{,_left,_right} (:1072-1273).  The
+ * reference's versions call an svdecon it never defines; here svdecon is the
+ * thin SVD and the sweeps are built by intent.  Target: sum_l coef[l] A[l]
+ * over the overlap terms (numv of them with H == NULL, numv-1 otherwise) plus
+ * coef[numv-1] H A[numv-1].  mode 0: maxsweeps full sweeps (left, then right)
+ * after left environments from Ain; 1: one left half-sweep (the _left
+ * variants, result right canonical with U S on site 1); 2: one right
+ * half-sweep after right environments from Ain (the _right variants, Ain
+ * right canonical; result left canonical with S V^H on the last site).  Every
+ * bond keeps min(#{k : cumsum(s^2)/sum(s^2) < 1 - svd_tol} + 1, dlim) values;
+ * A0's bonds change accordingly (read them with mps_state_dims) within its
+ * capacity, which must cover Ain's bonds.  env (overlap terms) and envop must
+ * be sized for A0's capacity against each target. */
+int mps_compress_two_site(mps_state_t A0, const mps_env_t* env, mps_env_t envop, mps_state_t Ain, int numv,
+                          const double* coef, const mps_state_t* A, mps_mpo_t H, int mode, int maxsweeps,
+                          double svd_tol, int64_t dlim);
 /* steady_state_search! (steadystate.jl:6-86): sweeps of local
  * smallest-eigenvalue problems of <At| L^dag L |At> over the sites of At
  * (left canonical on entry, not necessarily normalised; overwritten with the

@@ -1405,6 +1405,63 @@ int mps_compression_svd(mps_state_t s, int64_t dmax, double tol, double* err_glo
   });
 }
 
+// two-site variational sweeps (compression.jl:110-164, 355-486, 574-647, 1072-1273)
+int mps_compress_two_site(mps_state_t A0, const mps_env_t* env, mps_env_t envop, mps_state_t Ain, int numv,
+                          const double* coef, const mps_state_t* A, mps_mpo_t H, int mode, int maxsweeps,
+                          double svd_tol, int64_t dlim) {
+  return guard([&] {
+    REQUIRE(A0 && Ain && A && coef && numv >= 1, "invalid arguments");
+    REQUIRE(mode >= 0 && mode <= 2 && maxsweeps >= 0 && dlim >= 1 && svd_tol >= 0.0 && svd_tol < 1.0,
+            "mode 0..2, maxsweeps >= 0, dlim >= 1, svd_tol in [0, 1)");
+    REQUIRE(A0 != Ain, "two-site sweeps write a new state: A0 must not be Ain");
+    State& a0 = A0->s;
+    no_batch(a0.ctx);
+    REQUIRE(a0.N >= 2, "two-site sweeps need at least two sites");
+    const int nov = H ? numv - 1 : numv;
+    REQUIRE(nov >= 0 && (nov == 0 || env), "environments for the overlap terms required");
+    std::vector<State*> As(numv);
+    std::vector<Env*> Es(nov);
+    std::vector<cd> cf(numv);
+    for (int l = 0; l < numv; ++l) {
+      As[l] = &A[l]->s;
+      REQUIRE(A[l] != A0, "A0 must not be one of A");
+      check_states({&a0, As[l]});
+      cf[l] = cd(coef[2 * l], coef[2 * l + 1]);
+    }
+    check_states({&a0, &Ain->s});
+    for (int j = 0; j <= a0.N; ++j)
+      REQUIRE(Ain->s.dims[j] <= (j < a0.N ? a0.cap_l[j] : a0.cap_r[j - 1]), "A0's capacity is smaller than Ain's bonds");
+    for (int l = 0; l < nov; ++l) {
+      Es[l] = &env[l]->e;
+      check_ovl_env(*Es[l], a0);
+    }
+    if (H) {
+      REQUIRE(envop, "operator environment required");
+      check_op_env(envop->e, a0);
+      check_mpo(H->o, a0);
+    }
+    // environments must hold A0's capacity bonds against each target's bonds
+    std::vector<i64> capd(a0.N + 1);
+    for (int j = 0; j <= a0.N; ++j) capd[j] = j < a0.N ? a0.cap_l[j] : a0.cap_r[a0.N - 1];
+    State cap_like;
+    cap_like.N = a0.N;
+    cap_like.dims = capd;
+    for (int l = 0; l < nov; ++l) check_env_fits(*Es[l], cap_like, *As[l], nullptr);
+    if (H) check_env_fits(envop->e, cap_like, *As[nov], &H->o);
+    Ctx* c = a0.ctx;
+    set_device(c);
+    i64 Dm = a0.max_dim(), dm = 1, wm = H ? H->o.max_w() : 1;
+    for (int l = 0; l < numv; ++l) Dm = std::max(Dm, As[l]->max_dim());
+    for (int j = 0; j < a0.N; ++j) dm = std::max(dm, a0.d[j]);
+    size_t before = c->ws_bytes;
+    Plan p(c, Dm, dm, wm, 2 * std::max(nov, 1), two_site_extra_bytes(Dm, dm));
+    track_ws(A0->owner, before);
+    two_site_sweeps(p, &a0, Es.data(), H ? &envop->e : nullptr, &Ain->s, nov, cf.data(), As.data(),
+                    H ? &H->o : nullptr, mode, maxsweeps, svd_tol, dlim);
+    invalidate(A0->owner);
+  });
+}
+
 // steady_state_search! (steadystate.jl:6-86)
 int mps_steady_state_search(mps_state_t At, mps_mpo_t W, mps_env_t envop, int maxsweeps, double converge_rat,
                             double eigs_tol, double* evalue_out, int* sweeps_out) {

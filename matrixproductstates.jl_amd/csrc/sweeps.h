@@ -44,7 +44,9 @@ struct Plan {
   State* rtmp = nullptr;
   Plan(Ctx* c, std::initializer_list<const State*> states, std::initializer_list<const MPO*> mpos,
        int n_ovl, size_t extra_bytes = 0);
-  Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl) : ctx(c), scr(c) { init(Dm, dm, wm, n_ovl, 0); }
+  Plan(Ctx* c, i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes = 0) : ctx(c), scr(c) {
+    init(Dm, dm, wm, n_ovl, extra_bytes);
+  }
   void init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes);
 };
 
@@ -100,6 +102,13 @@ void rk4_half(Plan& p, double dt, MPO* H1, MPO* H2, MPO* H3, State* out, State* 
 size_t steady_state_extra_bytes(i64 site_elems, int kmax);
 double steady_state_search(Plan& p, State* At, MPO* W, Env* envop, int maxsweeps, double converge_rat,
                            double eigs_tol, int kmax, int* sweeps_run);
+// two-site sweeps with SVD truncation (compression.jl:110-164, 355-486,
+// 574-647, 1072-1273, by intent); p.extra holds two_site_extra_bytes(Dm, dm),
+// p.ovl 2 nov site buffers; mode 0 full sweeps, 1 one left, 2 one right
+size_t two_site_extra_bytes(i64 Dm, i64 dm);
+void two_site_sweeps(Plan& p, State* A0, Env** env, Env* envop, State* Ain, int nov,
+                     const std::complex<double>* coef, State** A, MPO* H, int mode, int maxsweeps,
+                     double svd_tol, i64 dlim);
 void scal_prod(Plan& p, State* A, State* B, int mode, int slot);
 void scal_prod_ws(Work& w, State* A, State* B, int mode, void* envA, void* envB, void* AL, void* out);
 void scal_op_prod(Plan& p, State* A, MPO* O, State* B, int slot);

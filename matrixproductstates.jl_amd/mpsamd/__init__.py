@@ -14,6 +14,10 @@ from .core import (  # noqa: F401
     compress_var_apply_H_left_, compress_sum_var_apply_H_left_, compress_sum_var_apply_H_right_,
     compress_var_apply_H_, compress_var_, compress_sum_var, compress_sum_var_apply_H_,
     compressionSVD_, RK4stp_apply_H_half_, Arnoldi_expm_, steady_state_search_,
+    compress_var_two_site, compress_var_apply_H_two_site, compress_var_apply_H_two_site_left,
+    compress_var_apply_H_two_site_right, compress_sum_var_two_site, compress_sum_var_apply_H_two_site,
+    compress_sum_var_apply_H_two_site_left, compress_sum_var_apply_H_two_site_right,
+    RK4stp_apply_H_half_full_two_site,
     scal_prod, scal_op_prod, scal_prod_no_conj, normMPS, diff_norm,
     measure_excitations_, apply_site_operator_,
 )

@@ -93,6 +93,7 @@ SIGNATURES = {
     "mps_compression_svd": [vp, i64, dbl, PD, PD, PD],
     "mps_rk4stp_apply_H_half": [dbl, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     "mps_arnoldi_expm": [vp, i32, PVP, vp, vp, PD],
+    "mps_compress_two_site": [vp, PVP, vp, vp, i32, PD, PVP, vp, i32, i32, dbl, i64],
     "mps_steady_state_search": [vp, vp, vp, i32, dbl, dbl, PD, C.POINTER(C.c_int)],
     "mps_scal_prod": [vp, vp, PD],
     "mps_scal_op_prod": [vp, vp, vp, PD],

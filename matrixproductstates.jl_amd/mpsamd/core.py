@@ -525,6 +525,84 @@ def Arnoldi_expm_(A, rhov, numv, env, envop):
     return coef.view(CDT)
 
 
+# ---------------------------------------------------- two-site sweeps
+def _two_site(Ain, coef, A, H, mode, maxsweeps, svd_tol, dims, dlim):
+    """A new device MPS from mps_compress_two_site (capacity: the dims rule
+    min(d^n, dlim, d^(N-n)) and Ain's bonds); the reference's ``dims`` list
+    is updated in place to the new bonds, as the reference does."""
+    N, d, ctx = Ain.N, Ain.d, Ain.ctx
+    cap = [1] * (N + 1)
+    left = right = 1
+    for n in range(1, N):
+        left *= d[n - 1]
+        cap[n] = min(left, int(dlim))
+    for n in range(N - 1, 0, -1):
+        right *= d[n]
+        cap[n] = min(cap[n], right)
+    cap = [max(c, a) for c, a in zip(cap, Ain.dims)]
+    A0 = MPS(N, d, cap, ctx)
+    nov = len(A) - (1 if H is not None else 0)
+    envs = [build_env(cap, A[l].dims, ctx=ctx) for l in range(nov)]
+    envop = build_env(cap, A[nov].dims, H.w, ctx=ctx) if H is not None else None
+    keep, pc = _coef(coef)
+    call("mps_compress_two_site", A0.h, _handles(envs) if envs else None, envop.h if envop else None, Ain.h,
+         len(A), pc, _handles(A), H.h if H is not None else None, int(mode), int(maxsweeps), float(svd_tol),
+         int(dlim))
+    if dims is not None:
+        dims[:] = A0.dims
+    return A0
+
+
+def compress_var_two_site(Ain, A, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:110-164 (by intent: thin SVD for svdecon)."""
+    return _two_site(Ain, [1.0], [A], None, 0, maxsweeps, svd_tol, dims, dlim)
+
+
+def compress_var_apply_H_two_site(Ain, A, H, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:355-411."""
+    return _two_site(Ain, [1.0], [A], H, 0, maxsweeps, svd_tol, dims, dlim)
+
+
+def compress_var_apply_H_two_site_left(Ain, A, H, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:413-448 (one left half-sweep)."""
+    return _two_site(Ain, [1.0], [A], H, 1, 1, svd_tol, dims, dlim)
+
+
+def compress_var_apply_H_two_site_right(Ain, A, H, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:450-486 (one right half-sweep; Ain right canonical)."""
+    return _two_site(Ain, [1.0], [A], H, 2, 1, svd_tol, dims, dlim)
+
+
+def compress_sum_var_two_site(Ain, coef, A, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:574-647."""
+    return _two_site(Ain, coef, A, None, 0, maxsweeps, svd_tol, dims, dlim)
+
+
+def compress_sum_var_apply_H_two_site(Ain, coef, A, H, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:1072-1157."""
+    return _two_site(Ain, coef, A, H, 0, maxsweeps, svd_tol, dims, dlim)
+
+
+def compress_sum_var_apply_H_two_site_left(Ain, coef, A, H, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:1159-1216."""
+    return _two_site(Ain, coef, A, H, 1, 1, svd_tol, dims, dlim)
+
+
+def compress_sum_var_apply_H_two_site_right(Ain, coef, A, H, maxsweeps, svd_tol, dims, dlim):
+    """compression.jl:1218-1273 (Ain right canonical)."""
+    return _two_site(Ain, coef, A, H, 2, 1, svd_tol, dims, dlim)
+
+
+def RK4stp_apply_H_half_full_two_site(dt, H, psi, svd_tol, dims, dlim):
+    """timeevolution.jl:208-216: the RK4 half step with two-site sweeps in
+    every stage; returns the new state."""
+    psi1 = compress_var_apply_H_two_site_left(psi, psi, H[0], 1, svd_tol, dims, dlim)
+    psi2 = compress_sum_var_apply_H_two_site_right(psi1, [1.0, dt / 2], [psi, psi1], H[1], 1, svd_tol, dims, dlim)
+    psi3 = compress_sum_var_apply_H_two_site_left(psi2, [1.0, dt], [psi, psi2], H[1], 1, svd_tol, dims, dlim)
+    return compress_sum_var_apply_H_two_site_right(psi3, [-1.0 / 3, 1.0 / 3, 2.0 / 3, 1.0 / 3],
+                                                   [psi, psi1, psi2, psi3], H[2], 1, svd_tol, dims, dlim)
+
+
 def steady_state_search_(At, L, Lconj, maxsweeps, converge_rat, eigs_tol, envop=None):
     """steadystate.jl:6-86.  ``L`` and ``Lconj`` are host MPO site lists (the
     reference passes its MPO arrays); their product W = Lconj * L is formed on
