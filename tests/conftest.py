@@ -32,7 +32,7 @@ def _reset_oracle_tol():
 def _torch_hip_first(request):
     """Bring up torch's HIP runtime before the library's when GPU tests run.
 
-    The buffer-level parity helpers (mpsamd.dev) pass torch device memory to
+    The buffer-level parity helpers (tests/devbuf.py) pass torch device memory to
     the library.  torch ships its own HIP runtime next to the one the library
     links; on the GPU hosts torch then sees no device if the library's runtime
     opened it first, so torch goes first in every test process."""

@@ -11,8 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import call
-from .core import default_context, _cbuf
+from mpsamd._lib import call
+from mpsamd.core import default_context, _cbuf
 
 
 def _torch():

@@ -68,7 +68,7 @@ def test_rk4_dense_fixture():
 
 
 def test_qr_fixture():
-    from mpsamd import dev
+    import devbuf as dev
     f = _load("qr_lapack.npz")
     assert np.abs(dev.qrq(f["M"]) - f["Q"]).max() < 1e-12
 

@@ -22,7 +22,7 @@ TOL = 1e-12   # FP64 contractions: relative Frobenius error
 
 @pytest.fixture(scope="module")
 def dev():
-    from mpsamd import dev as D
+    import devbuf as D
     return D
 
 
