@@ -79,6 +79,20 @@ int mps_ctx_create_batch(int device, int nb, int64_t copy_bytes, int64_t ws_byte
 int mps_batch_workspace_bytes(int device, int nb, int64_t Dmax, int64_t d, int64_t wmax, int64_t* out);
 /* A plain context on the batched context's stream, for views. */
 int mps_ctx_create_companion(mps_ctx_t batched, mps_ctx_t* out);
+/* Ragged pool (no counterpart in the reference; it serves ensembles of
+ * trajectories of chains of different lengths, the uneven-N form of
+ * spin_model_quantum_trajectory_rk4.jl:115-209).  Declares that copy b of
+ * the batched context holds a chain of n[b] sites (n sorted longest first,
+ * n[0] = nmax) in containers of nmax slots: its sites 0..n[b]/2-1 in slots
+ * 0..n[b]/2-1 and the rest right-aligned, ending at slot nmax-1.  Build the
+ * states with the nmax-site dims rule (both halves of every chain then see
+ * their own bonds) and set each copy through views (mps_state_view gives copy
+ * b's own n[b]-site chain; mps_state_product refuses the pool).  RK4 and the
+ * H-applying half-sweeps then run every copy in the same launches: per site
+ * slot, the copies present there; each copy's arithmetic is that of its chain
+ * alone in a pool of the same size.  Other sweeps refuse a ragged pool
+ * (MPS_E_STATE).  Call once, before the first sweep. */
+int mps_ctx_set_ragged(mps_ctx_t ctx, int nmax, const int* n);
 /* Use an external HIP stream (hipStream_t passed as void*); NULL = own stream. */
 int mps_ctx_set_stream(mps_ctx_t ctx, void* stream);
 int mps_ctx_synchronize(mps_ctx_t ctx);

@@ -303,6 +303,41 @@ int mps_ctx_create_companion(mps_ctx_t batched, mps_ctx_t* out) {
   });
 }
 
+int mps_ctx_set_ragged(mps_ctx_t ctx, int nmax, const int* n) {
+  return guard([&] {
+    REQUIRE(ctx && n, "invalid arguments");
+    Ctx& c = ctx->c;
+    const int nb = c.bat.nb;
+    REQUIRE(nb > 1, "a ragged pool is a batched context");
+    REQUIRE(!c.rag, "the pool's chain lengths are already set");
+    REQUIRE(n[0] == nmax && nmax >= 2, "copy 0 holds the longest chain, of nmax sites");
+    auto r = std::make_shared<Ctx::Ragged>();
+    r->nmax = nmax;
+    r->n.assign(n, n + nb);
+    for (int b = 0; b < nb; ++b) {
+      REQUIRE(n[b] >= 2 && n[b] <= nmax, "chain lengths in [2, nmax]");
+      REQUIRE(b == 0 || n[b] <= n[b - 1], "copies sorted by chain length, longest first");
+      r->h.push_back(n[b] / 2);
+    }
+    // copy b is present at slot q if q lies in its left part or its right part
+    r->kact.assign(nmax, 0);
+    for (int q = 0; q < nmax; ++q) {
+      int k = 0;
+      for (int b = 0; b < nb; ++b) {
+        const bool on = q < r->h[b] || q >= nmax - (n[b] - r->h[b]);
+        if (on) {
+          REQUIRE(k == b, "ragged pool: copies present at a slot are not a prefix");
+          ++k;
+        }
+      }
+      r->kact[q] = k;
+    }
+    c.rag = r;
+    c.bat.plan_nb = nb;
+    invalidate(ctx);
+  });
+}
+
 int mps_ctx_destroy(mps_ctx_t ctx) {
   return guard([&] {
     if (!ctx) return;
@@ -534,6 +569,7 @@ int mps_state_copy(mps_state_t dst, mps_state_t src) {
 int mps_state_product(mps_state_t s, const double* states) {
   return guard([&] {
     State& st = s->s;
+    if (st.ctx->rag) throw Error(MPS_E_STATE, "not available on a ragged pool (set each copy through a view)");
     set_device(st.ctx);
     // raw zero-padded sites with [0, :, 0] = state (constructors.jl:19-20), then
     // the left QR sweep that mpsproductstate interleaves with their construction
@@ -968,6 +1004,15 @@ static void check_view(const Ctx* batched, const Ctx* comp, int b) {
   REQUIRE(b >= 0 && b < batched->bat.nb, "copy index out of range");
 }
 
+// In a ragged pool, copy b's view is its own chain of n[b] sites: site j and
+// bond i at the slots the pool keeps them in.
+static const Ctx::Ragged* ragged_of(const Ctx* batched, int b, int N) {
+  const Ctx::Ragged* r = batched->rag.get();
+  if (r) REQUIRE(N == r->nmax, "ragged pool: containers span the pool's nmax slots");
+  (void)b;
+  return r;
+}
+
 int mps_state_view(mps_state_t s, int b, mps_ctx_t comp, mps_state_t* out) {
   return guard([&] {
     REQUIRE(s && comp && out, "invalid arguments");
@@ -984,6 +1029,23 @@ int mps_state_view(mps_state_t s, int b, mps_ctx_t comp, mps_state_t* out) {
     v.cap_l = o.cap_l;
     v.cap_r = o.cap_r;
     v.off = o.off;
+    if (const Ctx::Ragged* r = ragged_of(o.ctx, b, o.N)) {
+      const int n = r->n[b];
+      v.N = n;
+      v.d.resize(n);
+      v.cap_l.resize(n);
+      v.cap_r.resize(n);
+      v.off.resize(n);
+      v.dims.resize(n + 1);
+      for (int j = 0; j < n; ++j) {
+        const int q = r->slot_of_site(b, j);
+        v.d[j] = o.d[q];
+        v.cap_l[j] = o.cap_l[q];
+        v.cap_r[j] = o.cap_r[q];
+        v.off[j] = o.off[q];
+      }
+      for (int i = 0; i <= n; ++i) v.dims[i] = o.dims[r->slot_of_bond(b, i)];
+    }
     v.total = o.total;
     v.buf = boff(o.buf, (int64_t)b * o.ctx->bat.bs);
     v.own = false;
@@ -1006,6 +1068,21 @@ int mps_env_view(mps_env_t e, int b, mps_ctx_t comp, mps_env_t* out) {
     v.wd = o.wd;
     v.d2 = o.d2;
     v.off = o.off;
+    if (const Ctx::Ragged* r = ragged_of(o.ctx, b, o.N)) {
+      const int n = r->n[b];
+      v.N = n;
+      v.d1.resize(n + 1);
+      v.wd.resize(n + 1);
+      v.d2.resize(n + 1);
+      v.off.resize(n + 1);
+      for (int i = 0; i <= n; ++i) {
+        const int q = r->slot_of_bond(b, i);
+        v.d1[i] = o.d1[q];
+        v.wd[i] = o.wd[q];
+        v.d2[i] = o.d2[q];
+        v.off[i] = o.off[q];
+      }
+    }
     v.total = o.total;
     v.buf = boff(o.buf, (int64_t)b * o.ctx->bat.bs);
     v.own = false;
@@ -1026,6 +1103,18 @@ int mps_mpo_view(mps_mpo_t m, int b, mps_ctx_t comp, mps_mpo_t* out) {
     v.d = o.d;
     v.w = o.w;
     v.off = o.off;
+    if (const Ctx::Ragged* r = ragged_of(o.ctx, b, o.N)) {
+      const int n = r->n[b];
+      v.N = n;
+      v.d.resize(n);
+      v.off.resize(n);
+      v.w.resize(n + 1);
+      for (int j = 0; j < n; ++j) {
+        v.d[j] = o.d[r->slot_of_site(b, j)];
+        v.off[j] = o.off[r->slot_of_site(b, j)];
+      }
+      for (int i = 0; i <= n; ++i) v.w[i] = o.w[r->slot_of_bond(b, i)];
+    }
     v.total = o.total;
     v.buf = boff(o.buf, (int64_t)b * o.ctx->bat.bs);
     v.own = false;
@@ -1097,6 +1186,42 @@ static void check_env_fits(const Env& e, const State& a, const State& b, const M
   }
 }
 
+// The entry points whose sweeps launch per slot for the copies present there
+// (RK4 and the H-applying half-sweeps) accept a ragged pool's containers once
+// every copy's two halves meet consistently: the states' bonds, the MPOs'
+// bonds and every environment slot across the copy's gap agree, so each
+// copy is a chain of its own with the dims the slots give it.
+struct RaggedCall {
+  Ctx* c;
+  RaggedCall(Ctx* ctx, const std::vector<const State*>& ss, const std::vector<const MPO*>& ms,
+             const std::vector<const Env*>& es)
+      : c(ctx) {
+    if (!c->rag) return;
+    const Ctx::Ragged& r = *c->rag;
+    for (size_t b = 0; b < r.n.size(); ++b) {
+      const int qa = r.h[b], qb = r.nmax - r.n[b] + r.h[b];
+      for (const State* s : ss) {
+        REQUIRE(s->N == r.nmax, "ragged pool: states span the pool's nmax slots");
+        REQUIRE(s->dims[qa] == s->dims[qb], "ragged pool: a chain's two parts meet at bonds of different dimension");
+      }
+      for (const MPO* m : ms) {
+        REQUIRE(m->N == r.nmax, "ragged pool: MPOs span the pool's nmax slots");
+        REQUIRE(m->w[qa] == m->w[qb], "ragged pool: a chain's two parts meet at MPO bonds of different dimension");
+      }
+      for (const Env* e : es) {
+        REQUIRE(e->N == r.nmax, "ragged pool: environments span the pool's nmax slots");
+        for (int q = qa; q <= qb; ++q)
+          REQUIRE(e->d1[q] == e->d1[qa] && e->wd[q] == e->wd[qa] && e->d2[q] == e->d2[qa],
+                  "ragged pool: environment slots differ inside a chain's gap");
+      }
+    }
+    c->rag_ok = true;
+  }
+  ~RaggedCall() { c->rag_ok = false; }
+  RaggedCall(const RaggedCall&) = delete;
+  RaggedCall& operator=(const RaggedCall&) = delete;
+};
+
 int mps_compress_var_apply_H_left(mps_state_t A0, mps_env_t envop, mps_state_t Ain, mps_state_t A,
                                   mps_mpo_t H) {
   return guard([&] {
@@ -1107,6 +1232,7 @@ int mps_compress_var_apply_H_left(mps_state_t A0, mps_env_t envop, mps_state_t A
     check_env_fits(envop->e, A0->s, A0->s, &H->o);
     Ctx* c = A0->s.ctx;
     set_device(c);
+    RaggedCall rc(c, {&A0->s, &Ain->s, &A->s}, {&H->o}, {&envop->e});
     size_t before = c->ws_bytes;
     Plan p(c, {&A0->s, &Ain->s, &A->s}, {&H->o}, 0);
     track_ws(A0->owner, before);
@@ -1140,7 +1266,12 @@ static void sum_var_common(mps_state_t A0, const mps_env_t* env, mps_env_t envop
   check_env_fits(envop->e, A0->s, A0->s, &H->o);
   Ctx* c = A0->s.ctx;
   set_device(c);
-  i64 Dm =std::max(A0->s.max_dim(), Ain->s.max_dim()), dm = 1;
+  std::vector<const State*> rs{&A0->s, &Ain->s};
+  std::vector<const Env*> re{&envop->e};
+  for (int l = 0; l < numv; ++l) rs.push_back(As[l]);
+  for (int l = 0; l + 1 < numv; ++l) re.push_back(Es[l]);
+  RaggedCall rc(c, rs, {&H->o}, re);
+  i64 Dm = std::max(A0->s.max_dim(), Ain->s.max_dim()), dm = 1;
   for (int l = 0; l < numv; ++l) Dm = std::max(Dm, As[l]->max_dim());
   for (int j = 0; j < A0->s.N; ++j) dm = std::max(dm, A0->s.d[j]);
   size_t before = c->ws_bytes;
@@ -1503,6 +1634,8 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     mps_ctx_s* x = psi->owner;
     Ctx* c = &x->c;
     set_device(c);
+    RaggedCall rc(c, {&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s}, {&H1->o, &H2->o, &H3->o},
+                  {&env1->e, &env2->e, &env3->e, &envop->e});
     mps_ctx_s::Key key{{H1, H2, H3, psi0, psi, psi1, psi2, psi3, env1, env2, env3, envop}, dt, c->prec,
                        c->qr_tol, x->ws_gen, c->refine};
     auto run = [&](Plan& p) {

@@ -66,6 +66,10 @@ enum DType { F64 = 0, F32 = 1 };   // storage of complex elements: double2 / flo
 struct Bat {
   int nb = 1;
   int64_t bs = 0;
+  // copies the GEMM plan (tile size, split-K depth) is made for; 0: nb.  A
+  // ragged pool launches a prefix of its copies per site but plans for all of
+  // them, so a copy's summation order never depends on the other copies.
+  int plan_nb = 0;
 };
 template <typename T>
 __host__ __device__ inline T* boff(T* p, int64_t bytes) {

@@ -26,6 +26,23 @@ struct Ctx {
   Bat bat;
   char* slab = nullptr;
   size_t slab_used = 0;
+  // Ragged pool (mps_ctx_set_ragged): the copies hold chains of different
+  // lengths n[b] <= nmax, sorted longest first, in one slot space of nmax
+  // sites.  Copy b keeps its first h[b] = n[b]/2 sites in slots 0..h[b]-1 and
+  // the rest right-aligned in slots nmax-(n[b]-h[b])..nmax-1; the slots in
+  // between are its gap.  With the dims rule's edges at both ends of the slot
+  // space, every copy present at a slot has that slot's dims, and the copies
+  // present at slot q are the prefix 0..kact[q]-1.  The H-applying sweeps
+  // launch each site's work for that prefix only and carry the gap copies'
+  // environments across their gap.
+  struct Ragged {
+    int nmax = 0;
+    std::vector<int> n, h, kact;
+    int slot_of_site(int b, int j) const { return j < h[b] ? j : j + nmax - n[b]; }
+    int slot_of_bond(int b, int i) const { return i <= h[b] ? i : i + nmax - n[b]; }
+  };
+  std::shared_ptr<const Ragged> rag;
+  bool rag_ok = false;   // set by the entry points whose sweeps handle a ragged pool
   // device memory for a container: from the slab (owned = false) or hipMalloc
   void* alloc(size_t bytes, bool& owned);
   // second stream for GEMM chains independent of the main chain (forked

@@ -382,7 +382,8 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   GemmPlan pl;
   // 64 tiles unless they would leave more than half of the CUs idle (every
   // lockstep copy is a block of the grid too)
-  const int64_t t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64) * g.batch * g.bat.nb;
+  const int64_t nbp = g.bat.plan_nb > 0 ? g.bat.plan_nb : g.bat.nb;
+  const int64_t t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64) * g.batch * nbp;
   pl.tm = (2 * t64 < num_cus) ? 32 : 64;
   pl.tiles_m = (int)((g.M + pl.tm - 1) / pl.tm);
   pl.tiles_n = (int)((g.N + pl.tm - 1) / pl.tm);
@@ -392,7 +393,7 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   // over its partials pass); a grid with a tile for every CU is not split.
   // Chunks are at least 32 deep for 64 tiles and 128 deep for 32 tiles (a
   // quarter of the MFMA work per K step).
-  const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch * g.bat.nb;
+  const int64_t blocks = (int64_t)pl.tiles_m * pl.tiles_n * g.batch * nbp;
   const int64_t target = 4LL * num_cus;
   const int64_t min_chunk = (pl.tm == 64) ? 2 * BK : 8 * BK;
   int splits = 1;

@@ -69,6 +69,9 @@ size_t plan_bytes(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes, int num
 
 void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
   Ctx* c = ctx;
+  if (c->rag && !c->rag_ok)
+    throw Error(-4, "not available on a ragged pool (RK4 and the H-applying half-sweeps are; "
+                    "other calls go through a view of one copy)");
   const size_t site = (size_t)Dm * dm * Dm;
   ovl.assign(std::max(n_ovl, 0), nullptr);
   // the side stream pays off once a site's products fill a good part of the
@@ -188,6 +191,44 @@ static void set1(const Work& w, double2* p, cd v) {
   launch_set_scalar(p, v.real(), v.imag(), w.stream ? w.stream : w.c->stream, w.c->bat);
 }
 
+// --------------------------------------------------------- ragged pools
+// Inside a sweep's step at slot j, the context launches for the copies
+// present there (the prefix 0..kact[j]-1); outside it, for all copies.
+struct SiteScope {
+  Ctx* c;
+  int nb;
+  SiteScope(Ctx* ctx, int j) : c(ctx), nb(ctx->bat.nb) {
+    if (c->rag) c->bat.nb = c->rag->kact[j];
+  }
+  ~SiteScope() { c->bat.nb = nb; }
+  SiteScope(const SiteScope&) = delete;
+  SiteScope& operator=(const SiteScope&) = delete;
+};
+
+// The copies in their gap at slot j (kact[j]..nb-1) carry the environments a
+// sweep is updating one slot on: from slot j to j+1 in a sweep to the right,
+// from j+1 to j in a sweep to the left.  Across its whole gap a copy thus
+// hands the environment of its fold bond from one side of the gap to the
+// other, where its next site reads it.
+static void gap_pass(Plan& p, int j, bool to_right, Env* op, Env* const* ovl, int n_ovl) {
+  Ctx* c = p.ctx;
+  if (!c->rag) return;
+  const int k = c->rag->kact[j], nb = (int)c->rag->n.size();
+  if (k >= nb) return;
+  const int64_t o = (int64_t)k * c->bat.bs;
+  Bat g;
+  g.nb = nb - k;
+  g.bs = c->bat.bs;
+  const int src = to_right ? j : j + 1, dst = to_right ? j + 1 : j;
+  auto pass = [&](Env* e) {
+    const i64 n = e->d1[src] * e->wd[src] * e->d2[src];
+    REQUIRE(n == e->d1[dst] * e->wd[dst] * e->d2[dst], "ragged pool: environment slots differ inside a gap");
+    launch_copy_c(boff(e->slot(dst), o), boff(e->slot(src), o), n, c->stream, g);
+  };
+  if (op) pass(op);
+  for (int l = 0; l < n_ovl; ++l) pass(ovl[l]);
+}
+
 static void check_same_dims(const State* a, const State* b, const char* what) {
   REQUIRE(a->N == b->N, std::string(what) + ": chain lengths differ");
   for (int j = 0; j < a->N; ++j) REQUIRE(a->d[j] == b->d[j], std::string(what) + ": local dims differ");
@@ -206,6 +247,8 @@ static void init_left_op(Plan& p, Env* envop, State* Ain, State* A, MPO* H, cd r
   const int N = Ain->N;
   set1(c, envop->slot(0), 1.0);
   for (int j = 0; j + 1 < N; ++j) {
+    SiteScope ss(c, j);
+    gap_pass(p, j, true, envop, nullptr, 0);
     // AOL = prod_AOL(A[j], H[j], envop[j]); envop[j+1] = update_le(Ain[j], AOL)
     prod_AOL(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
              H->wr(j), envop->slot(j), Ain->Dl(j));
@@ -222,6 +265,8 @@ static void init_right_op(Plan& p, Env* envop, State* Ain, State* A, MPO* H, cd 
   const int N = Ain->N;
   set1(c, envop->slot(N), 1.0);
   for (int j = N - 1; j >= 1; --j) {
+    SiteScope ss(c, j);
+    gap_pass(p, j, false, envop, nullptr, 0);
     prod_AOR(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
              H->wr(j), envop->slot(j + 1), Ain->Dr(j));
     update_re(p.work, p.op, envop->slot(j), p.T2, Ain->site(j), A->Dl(j), H->wl(j), A->d[j],
@@ -318,6 +363,8 @@ void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State
     init_left_op(p, envop, Ain, A, H, 1.0);
   nx_right_begin(p, nx, N);
   for (int j = N - 1; j >= 0; --j) {
+    SiteScope ss(p.ctx, j);
+    gap_pass(p, j, false, envop, nullptr, 0);
     // AOR = prod_AOR(A[jj], H[jj], envop[jj+1]); prod_LR!(A0[jj], envop[jj], AOR)
     prod_AOR(p.work, p.op, p.T2, A->site(j), A->Dl(j), A->d[j], A->Dr(j), H->site(j), H->wl(j),
              H->wr(j), envop->slot(j + 1), A0->Dr(j));
@@ -345,6 +392,8 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     Env* e = env[ll];
     set1(ow, e->slot(0), 1.0);
     for (int j = 0; j + 1 < N; ++j) {
+      SiteScope ss(p.ctx, j);
+      gap_pass(p, j, true, nullptr, &e, 1);
       // update_lenv!(env[ll][jj+1], Ain[jj], A[ll][jj], env[ll][jj])
       prod_LR_23(ow, p.AL, e->slot(j), A[ll]->site(j), Ain->Dl(j), A[ll]->Dl(j), A[ll]->d[j],
                  A[ll]->Dr(j), false, false);
@@ -357,6 +406,8 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
   if (!init_done) init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
   nx_right_begin(p, nx, N);
   for (int j = N - 1; j >= 0; --j) {
+    SiteScope ss(p.ctx, j);
+    gap_pass(p, j, false, envop, env, numv - 1);
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
     // overlap terms: AR = prod_LR(A[ll][jj], env[ll][jj+1]); A0 (+)= prod_LR(env[ll][jj], AR)
     for (int ll = 0; ll + 1 < numv; ++ll) {
@@ -395,6 +446,8 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     Env* e = env[ll];
     set1(ow, e->slot(N), 1.0);
     for (int j = N - 1; j >= 1; --j) {
+      SiteScope ss(p.ctx, j);
+      gap_pass(p, j, false, nullptr, &e, 1);
       // update_renv!(env[ll][jj], Ain[jj], A[ll][jj], env[ll][jj+1])
       prod_LR_32(ow, p.AL, A[ll]->site(j), e->slot(j + 1), A[ll]->Dl(j), A[ll]->d[j], A[ll]->Dr(j),
                  Ain->Dr(j), false);
@@ -407,6 +460,8 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
   if (!init_done) init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
   nx_left_begin(p, nx);
   for (int j = 0; j < N; ++j) {
+    SiteScope ss(p.ctx, j);
+    gap_pass(p, j, true, envop, env, numv - 1);
     const i64 Dl0 = A0->Dl(j), Dr0 = A0->Dr(j), d = A0->d[j];
     // overlap terms first: AL[ll] reads A[ll][jj] before A0[jj] is written
     for (int ll = 0; ll + 1 < numv; ++ll)

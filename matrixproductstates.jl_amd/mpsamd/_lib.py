@@ -32,6 +32,7 @@ SIGNATURES = {
     "mps_ctx_create_batch": [i32, i32, i64, i64, PVP],
     "mps_batch_workspace_bytes": [i32, i32, i64, i64, i64, P64],
     "mps_ctx_create_companion": [vp, PVP],
+    "mps_ctx_set_ragged": [vp, i32, C.POINTER(C.c_int)],
     "mps_state_view": [vp, i32, vp, PVP],
     "mps_env_view": [vp, i32, vp, PVP],
     "mps_mpo_view": [vp, i32, vp, PVP],

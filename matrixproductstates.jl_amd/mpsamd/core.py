@@ -64,6 +64,17 @@ class Context:
         obj.h, obj.device, obj.nb = h, self.device, 1
         return obj
 
+    def set_ragged(self, n: Sequence[int]):
+        """Declare a ragged pool: copy b holds a chain of n[b] sites (n sorted
+        longest first) in containers of n[0] slots, its first n[b]//2 sites
+        left-aligned and the rest right-aligned (mps_ctx_set_ragged)."""
+        n = [int(x) for x in n]
+        if len(n) != self.nb:
+            raise _lib.DimensionMismatch(-1, "one chain length per copy")
+        arr = (C.c_int * len(n))(*n)
+        call("mps_ctx_set_ragged", self.h, n[0], arr)
+        self.ragged = n
+
     def close(self):
         if self.h:
             call("mps_ctx_destroy", self.h)
@@ -117,6 +128,12 @@ def _view(cls, name, base, b, ctx):
     call(name, base.h, int(b), ctx.h, C.byref(h))
     obj.h, obj.ctx, obj._base = h, ctx, base   # the view keeps its batched container alive
     return obj
+
+
+def _ragged_len(ctx, b):
+    """Chain length of copy b of a ragged pool (None for other contexts)."""
+    n = getattr(ctx, "ragged", None)
+    return None if n is None else n[b]
 
 
 _default_ctx: Context | None = None
@@ -200,8 +217,13 @@ class MPS:
         return [self.site(j) for j in range(self.N)]
 
     def view(self, b: int, ctx: Context) -> "MPS":
-        """Copy b of a batched state, on the companion context ctx."""
-        return _view(MPS, "mps_state_view", self, b, ctx)
+        """Copy b of a batched state, on the companion context ctx (in a ragged
+        pool: copy b's own chain of n[b] sites)."""
+        v = _view(MPS, "mps_state_view", self, b, ctx)
+        n = _ragged_len(self.ctx, b)
+        if n is not None:
+            v.N, v.d = n, [self.d[0]] * n
+        return v
 
     def copy_from(self, other: "MPS"):
         """A .= copy.(B)"""
@@ -252,7 +274,12 @@ class MPO:
         call("mps_mpo_upload_site", self.h, j, p, s.shape[0], s.shape[1], s.shape[2])
 
     def view(self, b: int, ctx: Context) -> "MPO":
-        return _view(MPO, "mps_mpo_view", self, b, ctx)
+        v = _view(MPO, "mps_mpo_view", self, b, ctx)
+        n = _ragged_len(self.ctx, b)
+        if n is not None:
+            q = lambda i: i if i <= n // 2 else i + self.N - n   # bond slot of copy b's bond i
+            v.N, v.d, v.w = n, [self.d[0]] * n, [self.w[q(i)] for i in range(n + 1)]
+        return v
 
     def set_block(self, j, wl_idx, wr_idx, op):
         """O[j][wl_idx, :, wr_idx, :] = op (update_h! / update_L_Ryd!)."""
@@ -289,7 +316,16 @@ class Env:
         self.mpo_dims = list(mpo_dims) if mpo_dims is not None else None
 
     def view(self, b: int, ctx: Context) -> "Env":
-        return _view(Env, "mps_env_view", self, b, ctx)
+        v = _view(Env, "mps_env_view", self, b, ctx)
+        n = _ragged_len(self.ctx, b)
+        if n is not None:
+            q = lambda i: i if i <= n // 2 else i + self.N - n
+            v.N = n
+            v.dims1 = [self.dims1[q(i)] for i in range(n + 1)]
+            v.dims2 = [self.dims2[q(i)] for i in range(n + 1)]
+            if self.mpo_dims is not None:
+                v.mpo_dims = [self.mpo_dims[q(i)] for i in range(n + 1)]
+        return v
 
     def slot(self, j, shape):
         n = int(np.prod(shape))

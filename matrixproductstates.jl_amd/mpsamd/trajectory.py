@@ -339,3 +339,174 @@ class SpinEnsemble:
             o.__del__()
         self.comp.close()
         self.ctx.close()
+
+
+class RaggedSpinEnsemble:
+    """Trajectories of driven spin chains of different lengths in one ragged pool.
+
+    Trajectory i follows a chain of ``Ns[i]`` sites with positions
+    ``models.positions(Ns[i], Ns[i], pos_seeds[i])`` (unit density, as
+    ``SpinTrajectory`` with L = N) and draws r1, r2 from
+    ``numpy.random.default_rng(seeds[i])``; all share the bond dimension D.
+    The pool keeps the chains longest first (copy b holds trajectory
+    ``order[b]``) in containers of max(Ns) slots (``Context.set_ragged``), so
+    one batched RK4 call advances every chain: each GEMM, QR panel and
+    environment update of a site slot is one launch for all chains present at
+    that slot.  ``normalize_lo!`` of all chains is one call; the jump branch
+    of a trajectory (spin_model_quantum_trajectory_rk4.jl:140-177) runs on a
+    view of its copy, which is its own chain of Ns[i] sites.  A trajectory's
+    arithmetic is that of the same trajectory alone in a pool of the same
+    size, whatever the other chains' lengths.
+    """
+
+    def __init__(self, Ns, D, pos_seeds, seeds, dt=0.01, Omega=0.5, Gamma=1.0, k=2 * np.pi, gam_eg=0.0,
+                 init=(1.0, 0.0), qr_rank_tol=0.0, precision="fp64", device=0):
+        B = len(Ns)
+        if not (len(pos_seeds) == len(seeds) == B and B >= 2):
+            raise ValueError("one chain length, position seed and draw seed per trajectory, at least two")
+        self.B, self.D, self.dt, self.gam_eg = B, D, dt, gam_eg
+        self.Ns = [int(n) for n in Ns]
+        self._model = (Omega, Gamma, k)
+        self.order = sorted(range(B), key=lambda i: -self.Ns[i])
+        self.copy_of = {i: b for b, i in enumerate(self.order)}
+        ns = [self.Ns[i] for i in self.order]
+        self.nmax = nmax = ns[0]
+        dims = M._dims_rule(nmax, D, 2)
+        for n in sorted(set(ns)):
+            # the chain's own bonds through its slots, and one bond all across its gap
+            h = n // 2
+            if (M._dims_rule(n, D, 2) != [dims[i if i <= h else i + nmax - n] for i in range(n + 1)]
+                    or len(set(dims[h:h + nmax - n + 1])) != 1):
+                raise ValueError(f"a chain of {n} sites at D={D} is too short for a pool of {nmax}-site slots "
+                                 "(its two parts would meet at an edge bond)")
+        Hs = MD.spin_rk4_mpos(MD.positions(nmax, float(nmax), 0), 0.0, dt, Omega, Gamma, k)
+        copy_bytes, ws, w = slab_sizes(dims, 2, Hs, 5, B, device)
+        self.hbm_bytes = slab_bytes(copy_bytes, ws, B)
+        self.ctx = M.Context.batched(B, copy_bytes, ws, device)
+        self.ctx.set_ragged(ns)
+        self.ctx.set_qr_rank_tol(qr_rank_tol)
+        self.ctx.set_precision(precision)
+        self.comp = self.ctx.companion()
+        self.comp.set_qr_rank_tol(qr_rank_tol)
+        self.H = [M.MPO(h, self.ctx) for h in Hs]
+        self.A = M.MPS(nmax, 2, dims, self.ctx)
+        self.A_coh, self.A1, self.A2, self.A3 = (self.A.similar() for _ in range(4))
+        self.envop = M.build_env(dims, dims, [1] + [w] * (nmax - 1) + [1], ctx=self.ctx)
+        self.env = [M.build_env(dims, dims, ctx=self.ctx) for _ in range(3)]
+        self.Av = [self.A.view(b, self.comp) for b in range(B)]
+        self.Acv = [self.A_coh.view(b, self.comp) for b in range(B)]
+        self._init_vec = init
+        self._per_len = {}       # chain length -> companion scratch of that length
+        self._jumps = [None] * B  # per copy: (jr, jl) on the companion
+        self.rngs = [None] * B
+        self.p_coh = [[] for _ in range(B)]
+        self.jumps = [[] for _ in range(B)]
+        for b, i in enumerate(self.order):
+            self._set_chain(b, pos_seeds[i])
+            self.reset(i, seeds[i])
+        self.t = 0.0
+
+    def _scratch(self, n):
+        s = self._per_len.get(n)
+        if s is None:
+            init, dims = M.mpsproductstate(n, self.D, 2, np.asarray(self._init_vec, dtype=complex), self.comp)
+            s = {"init": init, "dims": dims,
+                 "A_r": M.MPS(n, 2, dims, self.comp), "A_l": M.MPS(n, 2, dims, self.comp),
+                 "envop_jump": M.build_env(dims, dims, [1] + [2] * (n - 1) + [1], ctx=self.comp)}
+            self._per_len[n] = s
+        return s
+
+    def _set_chain(self, b, pos_seed):
+        """Copy b's RK4 and jump MPOs for the chain with positions from pos_seed."""
+        Omega, Gamma, k = self._model
+        n = self.Ns[self.order[b]]
+        z = MD.positions(n, float(n), pos_seed)
+        for h_dev, h_host in zip(self.H, MD.spin_rk4_mpos(z, 0.0, self.dt, Omega, Gamma, k)):
+            v = h_dev.view(b, self.comp)
+            for j, site in enumerate(h_host):
+                v.upload_site(j, site)
+            v.__del__()
+        old = self._jumps[b]
+        jr, jl = MD.spin_jump_mpos(z, Gamma, k)
+        self._jumps[b] = (M.MPO(jr, self.comp), M.MPO(jl, self.comp))
+        if old is not None:
+            for o in old:
+                o.__del__()
+
+    def reset(self, i, seed):
+        """Restart trajectory i from the initial product state, with draws from default_rng(seed)."""
+        b = self.copy_of[i]
+        self.Av[b].copy_from(self._scratch(self.Ns[i])["init"])
+        self.rngs[b] = np.random.default_rng(seed)
+        self.p_coh[i] = []
+        self.jumps[i] = []
+
+    def launch(self):
+        """Queue the one RK4 call that advances every chain (no host wait)."""
+        M.RK4stp_apply_H_half_(self.dt, self.H, self.A_coh, self.A, self.A1, self.A2, self.A3, *self.env,
+                               self.envop)
+
+    def step(self):
+        self.launch()
+        return self.finish()
+
+    def finish(self):
+        """Batched ``normalize_lo!`` (waits for the stream), then every
+        trajectory's draws and jump branch; returns p_coh by trajectory."""
+        p = M.normalize_lo_batch(self.A_coh)
+        out = np.empty(self.B)
+        for b, i in enumerate(self.order):
+            out[i] = p[b]
+            self.p_coh[i].append(float(p[b]))
+            r1, r2 = self.rngs[b].random(2)
+            step = len(self.p_coh[i]) - 1
+            if r1 < p[b]:
+                continue
+            n = self.Ns[i]
+            s = self._scratch(n)
+            A, Ac = self.Av[b], self.Acv[b]
+            jr, jl = self._jumps[b]
+            M.compress_var_apply_H_(s["A_r"], s["envop_jump"], A, A, jr, 1)
+            p_r = M.normalize_lo_(s["A_r"])
+            M.compress_var_apply_H_(s["A_l"], s["envop_jump"], A, A, jl, 1)
+            p_l = M.normalize_lo_(s["A_l"])
+            epop = M.measure_excitations_(None, A, HEE)
+            acc = jump_table(epop, p_r, p_l, self.dt, self.gam_eg)
+            kind, j = pick_jump(acc, r2, n)
+            if kind == "eg":
+                Ac.copy_from(A)
+                M.apply_site_operator_(Ac, j, HGE)
+                M.leftorthonormalizationQR_(Ac)
+            elif kind == "r":
+                Ac.copy_from(s["A_r"])
+            else:
+                Ac.copy_from(s["A_l"])
+            self.jumps[i].append((step, kind, j))
+        self.A.copy_from(self.A_coh)
+        self.t += self.dt
+        return out
+
+    def state(self, i):
+        """Trajectory i's sites (host copies)."""
+        return self.Av[self.copy_of[i]].to_host()
+
+    def sigma_z(self, i):
+        """<sigma_z^j> of trajectory i."""
+        return 2.0 * M.measure_excitations_(None, self.Av[self.copy_of[i]], HEE) - 1.0
+
+    def close(self):
+        for name in ("Av", "Acv"):
+            for v in getattr(self, name):
+                v.__del__()
+        for s in self._per_len.values():
+            for o in s.values():
+                if hasattr(o, "__del__") and not isinstance(o, list):
+                    o.__del__()
+        for jj in self._jumps:
+            if jj is not None:
+                for o in jj:
+                    o.__del__()
+        for o in (self.A, self.A_coh, self.A1, self.A2, self.A3, self.envop, *self.env, *self.H):
+            o.__del__()
+        self.comp.close()
+        self.ctx.close()
