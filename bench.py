@@ -244,6 +244,113 @@ def uneven_pools_leg(seed, n=512, steps=10, Ns=(40, 60, 80, 100, 120), Ds=(16, 3
     return out
 
 
+def ragged_pools_leg(seed, n=512, steps=5, Ds=(16, 32, 64, 128, 256), N_range=(40, 120),
+                     budget=240 << 30, pool_cap=96 << 30, precision="fp32"):
+    """n quantum-jump trajectories of the driven spin chain with N uniform in
+    N_range and D drawn from Ds: one ragged pool per D (chains of every
+    length advanced by one RK4 call, contractions in complex FP32), split into
+    pools of at most pool_cap bytes; pools run concurrently on their own
+    streams while their slabs fit the HBM budget, the rest wait for one to
+    finish.  The sequential rate is one trajectory after another on a plain
+    context (N = 40 and 120 timed per D, linear in N in between), in FP64 (the
+    plain path) and in FP32."""
+    import mpsamd as M
+    from mpsamd.core import _dims_rule
+    from mpsamd.trajectory import RaggedSpinEnsemble, SpinTrajectory, ragged_ensemble_bytes
+    rng = np.random.default_rng(seed)
+    Ns = rng.integers(N_range[0], N_range[1] + 1, size=n)
+    Dn = rng.choice(np.asarray(Ds), size=n)
+    seeds = rng.integers(1 << 30, size=n)
+    plans = []   # (D, trajectory indices), longest chains first within a D
+    for D in sorted(Ds, reverse=True):
+        idx = sorted(np.flatnonzero(Dn == D).tolist(), key=lambda i: -Ns[i])
+        while idx:
+            k = len(idx)
+            while k > 2 and ragged_ensemble_bytes([Ns[i] for i in idx[:k]], D) > pool_cap:
+                k = (k + 1) // 2
+            plans.append((int(D), idx[:k]))
+            idx = idx[k:]
+    flops = {}
+    for i in range(n):
+        key = (int(Ns[i]), int(Dn[i]))
+        if key not in flops:
+            flops[key] = model_flops(_dims_rule(key[0], key[1], 2), 2, 4)
+    total_flops = steps * sum(flops[(int(Ns[i]), int(Dn[i]))] for i in range(n))
+
+    t0 = time.perf_counter()
+    pending, active, used, peak, done = list(plans), [], 0, 0, 0
+    jumps = 0
+    while pending or active:
+        while pending:
+            D, idx = pending[0]
+            need = ragged_ensemble_bytes([Ns[i] for i in idx], D)
+            if active and used + need > budget:
+                break
+            pending.pop(0)
+            ens = RaggedSpinEnsemble([int(Ns[i]) for i in idx], D, [seed + int(Ns[i]) for i in idx],
+                                     [int(seeds[i]) for i in idx], dt=0.01, qr_rank_tol=1e-13,
+                                     precision=precision)
+            active.append([ens, 0, need])
+            used += need
+            peak = max(peak, used)
+        for a in active:
+            a[0].launch()
+        for a in active:
+            a[0].finish()
+            a[1] += 1
+        for a in [a for a in active if a[1] >= steps]:
+            jumps += sum(len(j) for j in a[0].jumps)
+            done += a[0].B
+            a[0].close()
+            used -= a[2]
+            active.remove(a)
+    wall = time.perf_counter() - t0
+    print(f"ragged pools: {n} trajectories in {len(plans)} pools, {wall:.2f} s", file=sys.stderr, flush=True)
+
+    def seq_rate(prec):
+        ctx = M.Context(0)
+        ctx.set_qr_rank_tol(1e-13)
+        ctx.set_precision(prec)
+        t_seq = 0.0
+        for D in Ds:
+            per = {}
+            for N in N_range:
+                tr = SpinTrajectory(int(N), int(D), seed, dt=0.01, ctx=ctx, pos_seed=seed + int(N))
+                tr.step(), tr.step()       # eager, then graph capture
+                ctx.synchronize()
+                ts = time.perf_counter()
+                for _ in range(2):
+                    tr.step()
+                per[N] = (time.perf_counter() - ts) / 2
+                for v in list(vars(tr).values()):
+                    for o in (v if isinstance(v, (list, tuple)) else [v]):
+                        if isinstance(o, (M.MPS, M.MPO, M.Env)):
+                            o.__del__()
+            a, b = per[N_range[0]], per[N_range[1]]
+            for i in np.flatnonzero(Dn == D):
+                t_seq += steps * (a + (b - a) * (Ns[i] - N_range[0]) / (N_range[1] - N_range[0]))
+        ctx.close()
+        return t_seq
+
+    t64, t32 = seq_rate("fp64"), seq_rate("fp32")
+    out = {"trajectories": n, "steps_each": steps, "N_range": list(N_range), "D_choices": list(Ds),
+           "precision": precision, "pools": len(plans),
+           "pools_by_D": {str(D): sum(1 for p in plans if p[0] == D) for D in Ds},
+           "hbm_budget_bytes": budget, "peak_pool_bytes": peak, "jumps": jumps,
+           "wall_s": round(wall, 3), "trajectories_per_s": round(n / wall, 3),
+           "trajectory_steps_per_s": round(n * steps / wall, 2),
+           "sequential_fp64_trajectories_per_s": round(n / t64, 4),
+           "sequential_fp32_trajectories_per_s": round(n / t32, 4),
+           "speedup_vs_sequential_fp64": round(t64 / wall, 2),
+           "speedup_vs_sequential_fp32": round(t32 / wall, 2),
+           "achieved_gemm_tflops": round(total_flops / wall / 1e12, 2),
+           "mfma_utilisation_fp32": round(total_flops / wall / 1e12 / PEAK["fp32"], 4),
+           "note": "wall time includes building each pool and capturing its step graph; utilisation is "
+                   "the reference's contraction flops over the leg's wall time against the FP32 MFMA peak"}
+    print(f"ragged pools: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_and_serve(seed):
     """Batched quantum-jump trajectories (concurrent per-stream solves vs the same
     solves one after another) and a Poisson request stream through the server."""
@@ -334,6 +441,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 and FP32+refine forms")
     ap.add_argument("--no-serve", action="store_true", help="skip the batched-trajectory and serving legs")
+    ap.add_argument("--ragged-only", action="store_true",
+                    help="run only the uneven-shape ragged-pool leg (512 trajectories, D up to 256)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch kernels eagerly instead of replaying the captured RK4 step")
     ap.add_argument("--timing-pass-only", action="store_true",
@@ -352,6 +461,9 @@ def main():
         sys.exit(2)
 
     import mpsamd as M
+    if args.ragged_only:
+        print(json.dumps({"ragged_pools": ragged_pools_leg(args.seed)}))
+        return
     ctx = M.Context(0)
     if args.no_graphs:
         ctx.set_graphs(False)
@@ -482,7 +594,8 @@ def main():
         uneven, per_solve = batch_and_serve(args.seed)
         batched = {"lockstep_ensemble": ensemble_leg(args.seed), "concurrent_uneven": uneven,
                    "density_matrix_pool": density_matrix_pool_leg(args.seed),
-                   "uneven_pools": uneven_pools_leg(args.seed)}
+                   "uneven_pools": uneven_pools_leg(args.seed),
+                   "ragged_pools": ragged_pools_leg(args.seed)}
         serving = {"lockstep_pools": pooled_serving_leg(args.seed), "per_solve_streams": per_solve}
 
     cpu = None

@@ -195,6 +195,15 @@ def spin_ensemble_bytes(N, D, B, dt=0.01, device=0):
     return slab_bytes(copy_bytes, ws, B)
 
 
+def ragged_ensemble_bytes(Ns, D, dt=0.01, device=0):
+    """HBM bytes of the slab of a ``RaggedSpinEnsemble`` of chains of lengths
+    Ns at bond dimension D, before building it."""
+    nmax = max(Ns)
+    Hs = MD.spin_rk4_mpos(MD.positions(nmax, float(nmax), 0), 0.0, dt)
+    copy_bytes, ws, _ = slab_sizes(M._dims_rule(nmax, D, 2), 2, Hs, 5, len(Ns), device)
+    return slab_bytes(copy_bytes, ws, len(Ns))
+
+
 class SpinEnsemble:
     """A lockstep ensemble of B trajectories of one driven spin chain.
 
