@@ -500,16 +500,23 @@ def main():
     nem = M.MPS.from_host(MD.sum_local_vec_mps(args.N, MD.HEE3), ctx)
     nrm = M.MPS.from_host(MD.sum_local_vec_mps(args.N, MD.HSS3), ctx)
     pem = M.MPS.from_host(MD.pair_sum_vec_mps(args.N, MD.HEE3), ctx)
+    # the reference's photon observables (rydberg_density_matrix_gpu.jl:135-151,
+    # 237-245): I_r = tr(E_R^dag E_R rho), I2_r, and g2 = I2_r / I_r^2
+    from mpsamd import observables as OB
+    ph = {k: M.MPS.from_host(v, ctx) for k, v in OB.photon_observables(z).items()}
 
     def observables(r):
-        # total populations and the pair correlation of the excited level,
-        # g2 = 2 sum_{i<j} <n_i n_j> / (sum_i <n_i>)^2 (undefined, None, on a
+        # total populations, the pair correlation of the excited level
+        # 2 sum_{i<j} <n_i n_j> / (sum_i <n_i>)^2, and the light's g2 (None on a
         # state with no excitation, e.g. the ground state before any step)
         tr_ = M.scal_prod_no_conj(idm, r)
         n_e = M.scal_prod_no_conj(nem, r) / tr_
         pairs = M.scal_prod_no_conj(pem, r) / tr_
+        i_r = M.scal_prod_no_conj(ph["IR"], r) / tr_
+        i2_r = M.scal_prod_no_conj(ph["I2R"], r) / tr_
         return {"trace": tr_, "n_e": n_e, "n_r": M.scal_prod_no_conj(nrm, r) / tr_,
-                "g2_e": (2.0 * pairs / n_e ** 2).real if n_e != 0 else None}
+                "g2_e": (2.0 * pairs / n_e ** 2).real if n_e != 0 else None,
+                "I_r": i_r.real, "I2_r": i2_r.real, "g2_r": OB.g2(i_r.real, i2_r.real)}
 
     tr = M.scal_prod_no_conj(idm, rho)
     ref_obs = observables(rho)
@@ -562,6 +569,8 @@ def main():
                 "err_sigma_z_max_site": float(np.abs(sigma_z(rho) - ref_sz).max()),
                 "err_trace": abs(ob["trace"] - ref_obs["trace"]),
                 "rel_err_pair_corr_e": abs(ob["g2_e"] - ref_obs["g2_e"]) / abs(ref_obs["g2_e"]),
+                "rel_err_I_r": abs(ob["I_r"] - ref_obs["I_r"]) / abs(ref_obs["I_r"]),
+                "rel_err_g2_r": abs(ob["g2_r"] - ref_obs["g2_r"]) / abs(ref_obs["g2_r"]),
                 "infidelity": 1.0 - fidelity(rho, rho64)}
             print(f"{form}: {prec_forms[form]}", file=sys.stderr, flush=True)
         rho.copy_from(rho64)
@@ -621,7 +630,8 @@ def main():
                    "seed": args.seed, "model_gflop_per_step": round(model_flops(dims, 9, p.w) / 1e9, 1),
                    "trace_rho": [round(tr.real, 12), round(tr.imag, 12)],
                    "observables": {"n_e": ref_obs["n_e"].real, "n_r": ref_obs["n_r"].real,
-                                   "pair_corr_e": ref_obs["g2_e"]}},
+                                   "pair_corr_e": ref_obs["g2_e"], "I_r": ref_obs["I_r"],
+                                   "I2_r": ref_obs["I2_r"], "g2_r": ref_obs["g2_r"]}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "profile": prof,
