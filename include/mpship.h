@@ -126,6 +126,24 @@ int mps_ctx_set_graphs(mps_ctx_t ctx, int enable);
  * refused with MPS_E_STATE otherwise). */
 int mps_ctx_set_profiling(mps_ctx_t ctx, int enable);
 int mps_ctx_kernel_stats(mps_ctx_t ctx, double* out);
+/* GEMM plan tuning (no counterpart in the reference; it serves the offline
+ * autotuner mpsamd/autotune.py, whose table csrc/gemm_tuned.inc is compiled
+ * into the library).  Between mps_ctx_gemm_record(ctx, 1) and (ctx, 0) every
+ * contraction the context launches eagerly is recorded (captured graphs are
+ * dropped first).  mps_ctx_gemm_recorded lists the distinct shapes in order of
+ * first launch, 10 int64 per row: M, N, K, batch, copy class (floor(log2 of
+ * the lockstep copies the plan is made for)), arithmetic (MPS_PREC_*), the
+ * tile and split-K depth in use, the copies, the partials workspace bytes;
+ * *rows is the number of shapes (at most max_rows are written).
+ * mps_ctx_gemm_time re-launches the recorded GEMM of that row, on its own
+ * operands and stream, iters times with the given tile (32 or 64) and split-K
+ * depth, and returns the mean microseconds per launch; a depth whose partials
+ * exceed the recorded call's workspace returns MPS_E_ARG before anything is
+ * queued.  The re-launches overwrite the recorded outputs: time on scratch
+ * workloads only. */
+int mps_ctx_gemm_record(mps_ctx_t ctx, int enable);
+int mps_ctx_gemm_recorded(mps_ctx_t ctx, int64_t* out, int max_rows, int* rows);
+int mps_ctx_gemm_time(mps_ctx_t ctx, int row, int tm, int splits, int iters, double* us);
 
 /* ------------------------------------------------------- states (a1, a3) */
 /* Vector{Array{TN,3}} with N sites, local dims d[N], bond dims dims[N+1]

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <algorithm>
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -401,6 +403,88 @@ int mps_ctx_set_profiling(mps_ctx_t ctx, int enable) {
     ctx->c.profiling = enable != 0;
     ctx->c.prof.clear();
     ctx->c.ev_used = 0;
+  });
+}
+
+// ------------------------------------------------------------ GEMM tuning
+// Distinct shapes of the recorded contractions, in order of first launch:
+// key (M, N, K, batch, copy class, arithmetic).
+static std::vector<size_t> distinct_recorded(const Ctx& c) {
+  std::vector<size_t> first;
+  std::vector<std::array<int64_t, 6>> keys;
+  for (size_t i = 0; i < c.rec.size(); ++i) {
+    const Gemm& g = c.rec[i].g;
+    const int64_t nbq = g.bat.plan_nb > 0 ? g.bat.plan_nb : g.bat.nb;
+    const std::array<int64_t, 6> k{g.M, g.N, g.K, g.batch, (int64_t)nb_class(nbq), (int64_t)c.rec[i].cmp};
+    if (std::find(keys.begin(), keys.end(), k) == keys.end()) {
+      keys.push_back(k);
+      first.push_back(i);
+    }
+  }
+  return first;
+}
+
+int mps_ctx_gemm_record(mps_ctx_t ctx, int enable) {
+  return guard_ro([&] {
+    REQUIRE(ctx, "invalid arguments");
+    Ctx& c = ctx->c;
+    if (enable) {
+      c.rec.clear();
+      c.gemm_rec = true;
+      ctx->clear_graphs();   // recorded calls launch eagerly
+    } else {
+      c.gemm_rec = false;
+    }
+  });
+}
+
+int mps_ctx_gemm_recorded(mps_ctx_t ctx, int64_t* out, int max_rows, int* rows) {
+  return guard_ro([&] {
+    REQUIRE(ctx && rows, "invalid arguments");
+    const Ctx& c = ctx->c;
+    const std::vector<size_t> first = distinct_recorded(c);
+    *rows = (int)first.size();
+    for (int r = 0; r < (int)first.size() && r < max_rows && out; ++r) {
+      const Ctx::RecGemm& x = c.rec[first[r]];
+      const int64_t nbq = x.g.bat.plan_nb > 0 ? x.g.bat.plan_nb : x.g.bat.nb;
+      int tm = 0, sp = 0;
+      gemm_plan_info(x.g, x.cmp, c.num_cus, &tm, &sp);
+      int64_t* o = out + 10 * r;
+      o[0] = x.g.M; o[1] = x.g.N; o[2] = x.g.K; o[3] = x.g.batch; o[4] = nb_class(nbq); o[5] = (int64_t)x.cmp;
+      o[6] = tm; o[7] = sp; o[8] = nbq; o[9] = (int64_t)x.ws_bytes;
+    }
+  });
+}
+
+int mps_ctx_gemm_time(mps_ctx_t ctx, int row, int tm, int splits, int iters, double* us) {
+  return guard([&] {
+    REQUIRE(ctx && us && iters >= 1, "invalid arguments");
+    REQUIRE(tm == 32 || tm == 64, "tile 32 or 64");
+    REQUIRE(splits >= 1 && splits <= 64, "split-K depth in [1, 64]");
+    Ctx& c = ctx->c;
+    REQUIRE(!c.capturing && !c.gemm_rec, "stop recording before timing");
+    const std::vector<size_t> first = distinct_recorded(c);
+    REQUIRE(row >= 0 && row < (int)first.size(), "no such recorded GEMM");
+    set_device(&c);
+    Ctx::RecGemm x = c.rec[first[row]];
+    x.g.force_tm = tm;
+    x.g.force_splits = splits;
+    // refused before anything is queued when the partials do not fit the workspace
+    REQUIRE(gemm_workspace_bytes(x.g, x.cmp, c.num_cus) <= (x.ws ? x.ws_bytes : 0),
+            "split-K partials exceed the call's workspace");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    launch_gemm(x.g, x.cmp, x.ws, x.ws_bytes, c.num_cus, x.s);   // warm
+    HIPCHK(hipEventRecord(a, x.s));
+    for (int i = 0; i < iters; ++i) launch_gemm(x.g, x.cmp, x.ws, x.ws_bytes, c.num_cus, x.s);
+    HIPCHK(hipEventRecord(b, x.s));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *us = 1e3 * (double)ms / iters;
   });
 }
 

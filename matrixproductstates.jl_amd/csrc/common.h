@@ -90,6 +90,11 @@ struct Gemm {
   double diag_re = 0;   // added to C[i][i] after alpha/beta (Q = I - V W)
   Bat bat;              // lockstep copies (A, B, C and the split-K partials all offset)
   int max_splits = 0;   // cap on the split-K depth (0: none; 1: no partials buffer needed)
+  // Plan selection: a contraction of the sweeps (tunable) takes its tile and
+  // split-K depth from the autotuned table when its shape is there, else from
+  // the fixed rule; force_tm > 0 overrides both (the tuner's timing runs).
+  bool tunable = false;
+  int force_tm = 0, force_splits = 0;
 };
 
 enum Compute { CMP_F64 = 0, CMP_F32_3M = 1 };
@@ -100,6 +105,9 @@ struct GemmWorkspace {
   size_t bytes = 0;
 };
 size_t gemm_workspace_bytes(const Gemm& g, Compute cmp, int num_cus);
+// the tile and split-K depth launch_gemm uses for g; the lockstep-copy class of the tuned table
+void gemm_plan_info(const Gemm& g, Compute cmp, int num_cus, int* tm, int* splits);
+int nb_class(int64_t nb);
 void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_cus, hipStream_t s);
 
 // Householder QR (zgeqrf + zungqr semantics, optional rank cutoff); in place Q.

@@ -67,6 +67,17 @@ struct Ctx {
     double bytes;      // algorithmic HBM bytes (operands read once, result written once)
   };
   std::vector<ProfRec> prof;
+  // GEMM recording for the autotuner (mps_ctx_gemm_record): every contraction
+  // launched eagerly while on, with its operands and workspace as launched
+  struct RecGemm {
+    Gemm g;
+    Compute cmp;
+    void* ws;
+    size_t ws_bytes;
+    hipStream_t s;
+  };
+  bool gemm_rec = false;
+  std::vector<RecGemm> rec;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   hipEvent_t next_event();

@@ -143,7 +143,9 @@ static double esize(DType t) { return t == F64 ? 16.0 : 8.0; }
 
 void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s) {
   g.bat = c->bat;
+  g.tunable = true;
   if (splitk == nullptr) g.max_splits = 1;   // no partials buffer: K is not split
+  if (c->gemm_rec && !c->capturing) c->rec.push_back(Ctx::RecGemm{g, cmp, splitk, splitk_bytes, s ? s : c->stream});
   if (!c->profiling || c->capturing) {
     launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, s ? s : c->stream);
     return;
@@ -152,12 +154,13 @@ void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, h
   hipEvent_t a;
   c->prof_begin(a);
   launch_gemm(g, cmp, splitk, splitk_bytes, c->num_cus, c->stream);
-  const double flops = 8.0 * (double)g.M * g.N * g.K * g.batch;
-  const double bytes = (double)g.batch * ((double)g.M * g.K * esize(g.ta) * (g.a_bs ? 1 : 0) +
+  // every lockstep copy the launch covers counts
+  const double flops = 8.0 * (double)g.M * g.N * g.K * g.batch * g.bat.nb;
+  const double bytes = (double)g.bat.nb * (double)g.batch * ((double)g.M * g.K * esize(g.ta) * (g.a_bs ? 1 : 0) +
                                           (double)g.K * g.N * esize(g.tb) * (g.b_bs ? 1 : 0) +
                                           (double)g.M * g.N * esize(g.tc)) +
-                       (g.a_bs ? 0.0 : (double)g.M * g.K * esize(g.ta)) +
-                       (g.b_bs ? 0.0 : (double)g.K * g.N * esize(g.tb));
+                       (double)g.bat.nb * ((g.a_bs ? 0.0 : (double)g.M * g.K * esize(g.ta)) +
+                                           (g.b_bs ? 0.0 : (double)g.K * g.N * esize(g.tb)));
   c->prof_end(a, 0, flops, bytes);
 }
 

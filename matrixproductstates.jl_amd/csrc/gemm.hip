@@ -377,9 +377,54 @@ struct GemmPlan {
   int tm, tiles_m, tiles_n, splits, kchunk;
 };
 
+// ------------------------------------------------------- tuned plans
+// Written by mpsamd/autotune.py from timings on the MI355X: per GEMM shape of
+// the sweeps (M, N, K, batch), per power-of-two class of lockstep copies and
+// per arithmetic, the tile and split-K depth that ran fastest.  A table
+// compiled into the library keeps the plan, and so the summation order, a
+// fixed function of the shape.
+struct TunedPlan {
+  int64_t M, N, K, batch;
+  int nb_class, cmp, tm, splits;
+};
+static const TunedPlan kTuned[] = {
+#include "gemm_tuned.inc"
+    {0, 0, 0, 0, -1, -1, 0, 0}};
+
+int nb_class(int64_t nb) {
+  int c = 0;
+  while (nb > 1) {
+    nb >>= 1;
+    ++c;
+  }
+  return c;
+}
+
+static const TunedPlan* tuned_plan(const Gemm& g, int64_t nbp, Compute cmp) {
+  const int cl = nb_class(nbp);
+  for (const TunedPlan& t : kTuned)
+    if (t.M == g.M && t.N == g.N && t.K == g.K && t.batch == g.batch && t.nb_class == cl && t.cmp == (int)cmp)
+      return &t;
+  return nullptr;
+}
+
 static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
-  (void)cmp;   // one plan serves the FP64 and FP32 kernels
   GemmPlan pl;
+  const int64_t nbq = g.bat.plan_nb > 0 ? g.bat.plan_nb : g.bat.nb;
+  const TunedPlan* tp = g.force_tm > 0 ? nullptr : (g.tunable ? tuned_plan(g, nbq, cmp) : nullptr);
+  if (g.force_tm > 0 || tp) {
+    pl.tm = g.force_tm > 0 ? g.force_tm : tp->tm;
+    pl.tiles_m = (int)((g.M + pl.tm - 1) / pl.tm);
+    pl.tiles_n = (int)((g.N + pl.tm - 1) / pl.tm);
+    int splits = std::max(1, g.force_tm > 0 ? g.force_splits : tp->splits);
+    if (g.max_splits > 0 && splits > g.max_splits) splits = g.max_splits;
+    int64_t kchunk = (g.K + splits - 1) / splits;
+    kchunk = std::max<int64_t>(((kchunk + BK - 1) / BK) * BK, BK);
+    splits = (int)std::max<int64_t>((g.K + kchunk - 1) / kchunk, 1);
+    pl.splits = splits;
+    pl.kchunk = (int)kchunk;
+    return pl;
+  }
   // 64 tiles unless they would leave more than half of the CUs idle (every
   // lockstep copy is a block of the grid too)
   const int64_t nbp = g.bat.plan_nb > 0 ? g.bat.plan_nb : g.bat.nb;
@@ -412,6 +457,12 @@ static GemmPlan make_plan(const Gemm& g, int num_cus, Compute cmp) {
   pl.splits = splits;
   pl.kchunk = (int)(kchunk > 0 ? kchunk : BK);
   return pl;
+}
+
+void gemm_plan_info(const Gemm& g, Compute cmp, int num_cus, int* tm, int* splits) {
+  const GemmPlan pl = make_plan(g, num_cus, cmp);
+  *tm = pl.tm;
+  *splits = pl.splits;
 }
 
 // split-K partials of one lockstep copy

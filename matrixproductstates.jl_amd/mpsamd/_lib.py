@@ -46,6 +46,9 @@ SIGNATURES = {
     "mps_ctx_set_refine_sweeps": [vp, i32],
     "mps_ctx_set_profiling": [vp, i32],
     "mps_ctx_kernel_stats": [vp, PD],
+    "mps_ctx_gemm_record": [vp, i32],
+    "mps_ctx_gemm_recorded": [vp, P64, i32, C.POINTER(C.c_int)],
+    "mps_ctx_gemm_time": [vp, i32, i32, i32, i32, PD],
     "mps_state_create": [vp, i32, P64, P64, PVP],
     "mps_state_destroy": [vp],
     "mps_state_dims": [vp, P64],

@@ -114,6 +114,30 @@ class Context:
                 "qr_ms": out[4], "qr_flops": out[5], "qr_launches": int(out[6]), "launches_checked": int(out[7])}
 
 
+
+def gemm_record(ctx: Context, enable: bool):
+    call("mps_ctx_gemm_record", ctx.h, 1 if enable else 0)
+
+
+def gemm_recorded(ctx: Context):
+    """Distinct recorded GEMM shapes: dicts with M, N, K, batch, nb_class, cmp,
+    the plan in use (tm, splits), the copies and the partials workspace."""
+    rows = C.c_int()
+    call("mps_ctx_gemm_recorded", ctx.h, None, 0, C.byref(rows))
+    n = rows.value
+    buf = (C.c_int64 * max(1, 10 * n))()
+    call("mps_ctx_gemm_recorded", ctx.h, buf, n, C.byref(rows))
+    keys = ("M", "N", "K", "batch", "nb_class", "cmp", "tm", "splits", "nb", "ws_bytes")
+    return [dict(zip(keys, [int(x) for x in buf[10 * r:10 * r + 10]])) for r in range(n)]
+
+
+def gemm_time(ctx: Context, row: int, tm: int, splits: int, iters: int = 20) -> float:
+    """Microseconds per launch of recorded GEMM `row` with a forced tile and split-K depth."""
+    us = C.c_double()
+    call("mps_ctx_gemm_time", ctx.h, int(row), int(tm), int(splits), int(iters), C.byref(us))
+    return float(us.value)
+
+
 def batch_workspace_bytes(nb: int, Dmax: int, d: int, wmax: int, device: int = 0) -> int:
     """Workspace per copy of a batched context for RK4 / sweep calls at these sizes."""
     out = C.c_int64()
