@@ -132,12 +132,13 @@ struct OpLoader {
   // output rows (of A) or columns (of B) past the edge, which are never
   // stored, so only k past the chunk needs a mask.
   template <bool FULL, typename S>
-  __device__ __forceinline__ void put(int k0, int kend, int conj, const Frag<TM>& f, S (*Xs)[BK][TM + 1]) const {
+  __device__ __forceinline__ void put(int k0, int kend, int conj, const Frag<TM>& f, S (*Xs)[BK][TM]) const {
 #pragma unroll
     for (int j = 0; j < NF; ++j) {
       const bool ok = FULL || k0 + kk[j] < kend;
-      Xs[0][kk[j]][rr[j]] = (S)(ok ? f.re[j] : 0.0);
-      Xs[1][kk[j]][rr[j]] = (S)(ok ? (conj ? -f.im[j] : f.im[j]) : 0.0);
+      const int c = rr[j] ^ kk[j];   // swizzled row (see zgemm_body)
+      Xs[0][kk[j]][c] = (S)(ok ? f.re[j] : 0.0);
+      Xs[1][kk[j]][c] = (S)(ok ? (conj ? -f.im[j] : f.im[j]) : 0.0);
     }
   }
 };
@@ -205,12 +206,17 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
   constexpr int WT = TM / 32;   // 16x16 MFMA tiles per wave and dimension
   constexpr int WB = TM / 2;    // a wave's output block is WB x WB
   // LDS: the double-buffered operand tiles, then (after the K loop) each
-  // wave's output block, column major with a padded column stride
-  constexpr size_t TILE_BYTES = 2 * sizeof(S[2][2][BK][TM + 1]);
-  constexpr size_t OUT_BYTES = 4 * sizeof(double2) * WB * (WB + 1);
+  // wave's output block, column major.  Rows are not padded but swizzled: element (k, r) of an operand tile
+  // sits at column r ^ k (k < BK = 16 <= TM flips only the low four bits of
+  // r), and element (row, col) of a wave's output block at row ^ (col & 7).
+  // The k-major tile stores of the loaders and the row-major MFMA reads then
+  // meet distinct banks, as the padded layout did, and the largest variant
+  // (FP64, TM = 64) takes exactly 64 KiB of LDS.
+  constexpr size_t TILE_BYTES = 2 * sizeof(S[2][2][BK][TM]);
+  constexpr size_t OUT_BYTES = 4 * sizeof(double2) * WB * WB;
   __shared__ __align__(16) char smem[TILE_BYTES > OUT_BYTES ? TILE_BYTES : OUT_BYTES];
-  auto As = reinterpret_cast<S(*)[2][BK][TM + 1]>(smem);
-  auto Bs = reinterpret_cast<S(*)[2][BK][TM + 1]>(smem + TILE_BYTES / 2);
+  auto As = reinterpret_cast<S(*)[2][BK][TM]>(smem);
+  auto Bs = reinterpret_cast<S(*)[2][BK][TM]>(smem + TILE_BYTES / 2);
   const int tile = blockIdx.x, split = blockIdx.y, b = blockIdx.z % p.batch;
   const int64_t bo = (int64_t)(blockIdx.z / p.batch) * p.bst;   // lockstep copy
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
@@ -264,10 +270,11 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
       S a_r[WT], a_i[WT], a_s[WT], b_r[WT], b_i[WT], b_s[WT];
 #pragma unroll
       for (int i = 0; i < WT; ++i) {
-        a_r[i] = As[cur][0][k][wm + i * 16 + lr];
-        a_i[i] = As[cur][1][k][wm + i * 16 + lr];
-        b_r[i] = Bs[cur][0][k][wn + i * 16 + lr];
-        b_i[i] = Bs[cur][1][k][wn + i * 16 + lr];
+        const int ra = (wm + i * 16 + lr) ^ k, rb = (wn + i * 16 + lr) ^ k;
+        a_r[i] = As[cur][0][k][ra];
+        a_i[i] = As[cur][1][k][ra];
+        b_r[i] = Bs[cur][0][k][rb];
+        b_i[i] = Bs[cur][1][k][rb];
         a_s[i] = a_r[i] + a_i[i];
         b_s[i] = b_r[i] + b_i[i];
       }
@@ -288,7 +295,7 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
   //   v_mfma_f32_16x16x4_f32: lane l, element r -> row 4(l/16) + r, col l%16
   // go to the wave's block, then each store instruction writes whole column
   // segments of WB rows (64 / WB columns per instruction).
-  double2* blk = reinterpret_cast<double2*>(smem) + wave * WB * (WB + 1);
+  double2* blk = reinterpret_cast<double2*>(smem) + wave * WB * WB;
 #pragma unroll
   for (int i = 0; i < WT; ++i)
 #pragma unroll
@@ -298,7 +305,7 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
         const S x1 = p1[i][j][r], x2 = p2[i][j][r], x3 = p3[i][j][r];
         const int row = i * 16 + (F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
         const int col = j * 16 + (lane & 15);
-        blk[col * (WB + 1) + row] = make_double2((double)(x1 - x2), (double)(x3 - x1 - x2));
+        blk[col * WB + (row ^ (col & 7))] = make_double2((double)(x1 - x2), (double)(x3 - x1 - x2));
       }
   __syncthreads();
   constexpr int CPI = 64 / WB;   // columns per store instruction
@@ -306,7 +313,7 @@ __device__ __forceinline__ void zgemm_body(const GemmParams& p) {
   for (int q = 0; q < WB / CPI; ++q) {
     const int rl = lane % WB, cl = q * CPI + lane / WB;
     const int row = m0 + wm + rl, col = n0 + wn + cl;
-    const double2 v = blk[cl * (WB + 1) + rl];
+    const double2 v = blk[cl * WB + (rl ^ (cl & 7))];
     if (row < p.M && col < p.N) store_elem<TC>(p, b, bo, split, row, col, v.x, v.y);
   }
 }
