@@ -55,6 +55,15 @@ def model_flops(dims, d, w):
 
 QR_RANK_TOL = 1e-13   # the cutoff the full-size parity tests use on both sides
 
+# Counter measurements of the dominant GEMM launch at this workload's shape, the
+# 1792x1152x128 product of prod_AOL (complex FP64, 64-tiles), from rocprofv3
+# over `bench.py --gemm-counters fp64` (FETCH_SIZE x 2.00 and WRITE_SIZE x 1.00,
+# the factors a 256 MiB copy gives; SQ_VALU_MFMA_BUSY_CYCLES over the kernel's
+# cycles at 2.4 GHz): profiles/zgemm_prodAOL_N100_D128_d9_w14_counters.md
+DOMINANT_LAUNCH = {"kernel": "zgemm_f64_kernel<64>", "shape": "1792x1152x128 (prod_AOL, D=128, d=9, w=14)",
+                   "hbm_read_bytes": 26.55e6, "hbm_write_bytes": 33.03e6, "algorithmic_bytes": 39.06e6,
+                   "kernel_us": 42.4, "mfma_busy": 0.48}
+
 
 def setup(N, D, n_exp, dt, seed, ctx):
     import mpsamd as M
@@ -242,6 +251,43 @@ def uneven_pools_leg(seed, n=512, steps=10, Ns=(40, 60, 80, 100, 120), Ds=(16, 3
            "note": "wall time includes building each pool and capturing its step graph"}
     print(f"uneven pools: {out}", file=sys.stderr, flush=True)
     return out
+
+
+def gemm_counters(prec, D=128, d=9, w=14, reps=5):
+    """For counter runs of the dominant GEMMs (run under rocprofv3): a 256 MiB
+    device copy (calibration of the byte counters), then `reps` calls of
+    prod_AOL (mps_base.jl:26-39) at the benchmark shape through the C-ABI:
+    its 1792x1152x128 product and its batched MPO product.  torch provides
+    the device buffers; the library computes."""
+    import ctypes as C
+    import torch
+    torch.cuda.init()
+    n = 256 << 20
+    src = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    import mpsamd as M
+    from mpsamd._lib import call
+    ctx = M.default_context()
+    ctx.set_precision(prec)
+    rng = np.random.default_rng(0)
+
+    def dev(*shape):
+        a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        t = torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to("cuda")
+        torch.cuda.synchronize()
+        return t
+
+    A, O, L = dev(D, d, D), dev(w, d, w, d), dev(D, w, D)
+    out = torch.zeros(D * d * D * w, dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    p = lambda t: C.c_void_p(t.data_ptr())   # noqa: E731
+    for _ in range(reps):
+        call("mps_dev_prod_AOL", ctx.h, p(out), p(A), D, d, D, p(O), w, w, p(L), D)
+    ctx.synchronize()
+    return {"calibration_copy_bytes": n, "prod_AOL_calls": reps, "shape": {"D": D, "d": d, "w": w},
+            "precision": prec}
 
 
 def ragged_pools_leg(seed, n=512, steps=5, Ds=(16, 32, 64, 128, 256), N_range=(40, 120),
@@ -441,6 +487,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 and FP32+refine forms")
     ap.add_argument("--no-serve", action="store_true", help="skip the batched-trajectory and serving legs")
+    ap.add_argument("--gemm-counters", choices=["fp64", "fp32"], default=None,
+                    help="only the calibration copy and the dominant GEMM launches (for rocprofv3 --pmc runs)")
     ap.add_argument("--ragged-only", action="store_true",
                     help="run only the uneven-shape ragged-pool leg (512 trajectories, D up to 256)")
     ap.add_argument("--no-graphs", action="store_true",
@@ -460,6 +508,9 @@ def main():
                                    "(trajectories/requests would shard across GPUs with no collectives)"}))
         sys.exit(2)
 
+    if args.gemm_counters:
+        print(json.dumps({"gemm_counters": gemm_counters(args.gemm_counters)}))
+        return
     import mpsamd as M
     if args.ragged_only:
         print(json.dumps({"ragged_pools": ragged_pools_leg(args.seed)}))
@@ -588,8 +639,13 @@ def main():
         if args.timing_pass_only:
             sweeps = args.steps / (time.perf_counter() - tp)
         achieved = st["gemm_flops"] / (st["gemm_ms"] * 1e-3) / 1e12
+        dl = DOMINANT_LAUNCH
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                "frac": round(achieved / PEAK["fp64"], 4),
+                "traffic": dl["hbm_read_bytes"] + dl["hbm_write_bytes"],
+                "traffic_scope": f"HBM bytes of one {dl['kernel']} launch of shape {dl['shape']} "
+                                 f"(rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated); algorithmic "
+                                 f"{dl['algorithmic_bytes']:.4g} B; MFMA busy {dl['mfma_busy']}",
                 "kernel": "zgemm_f64_kernel (+ split-K reduce)",
                 "launches": st["gemm_launches"], "avg_launch_us": round(st["gemm_ms"] * 1e3 / max(st["gemm_launches"], 1), 2)}
         prof = {"gemm_ms_per_step": st["gemm_ms"] / args.steps, "qr_ms_per_step": st["qr_ms"] / args.steps,
