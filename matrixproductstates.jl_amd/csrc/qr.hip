@@ -379,6 +379,39 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
   // reduced together with the current column's dot products (uniform)
   bool have_next = false;
   double nx_xs = 0.0, nx_ar = 0.0, nx_ai = 0.0;
+  // Identity prefix (8-column panels): the zlarfg inputs of every column,
+  // reduced together up front.  While the panel's reflectors are identities
+  // no column changes, so column k's inputs are these values, with the same
+  // bits as its own reduction would give (the same per-thread sums, and the
+  // butterfly pairs lanes alike for every value); a panel of identity
+  // reflectors then costs one reduction instead of one per column.
+  bool up_valid = false;
+  double up_xs = 0.0, up_ar = 0.0, up_ai = 0.0;   // lanes 8k..8k+7 hold column k's value
+  if constexpr (NB == 8) {
+    double vx[8], vr[8], vi[8];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const double2 x = a[i][k];
+        const double n2 = fma(x.x, x.x, x.y * x.y);
+        s += (i > 0 || tid > k) ? n2 : 0.0;
+      }
+      vx[k] = s;
+      vr[k] = (tid == k) ? a[0][k].x : 0.0;
+      vi[k] = (tid == k) ? a[0][k].y : 0.0;
+    }
+    red[ph][(wave * GMAX + 0) * 64 + lane] = wave_rs8(vx);
+    red[ph][(wave * GMAX + 1) * 64 + lane] = wave_rs8(vr);
+    red[ph][(wave * GMAX + 2) * 64 + lane] = wave_rs8(vi);
+    __syncthreads();
+    up_xs = wave_sum_slots(red[ph], 0);
+    up_ar = wave_sum_slots(red[ph], 1);
+    up_ai = wave_sum_slots(red[ph], 2);
+    ph ^= 1;
+    up_valid = true;
+  }
   // One column step: column k sits in slot O of a[.][], the columns trailing
   // it in slots O+1 .. W-1 (zero past the panel) and are updated in place.
   // Steps go in pairs (O = 0, then 1) and the panel then shifts left by two
@@ -397,6 +430,10 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       xs = nx_xs;
       ar = nx_ar;
       ai = nx_ai;
+    } else if (up_valid) {
+      xs = bcast(up_xs, 8 * k);
+      ar = bcast(up_ar, 8 * k);
+      ai = bcast(up_ai, 8 * k);
     } else {
       double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -431,6 +468,7 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
         if (rr < M) Vk[(int64_t)k * m + NT * i] = make_double2(rr == k ? 1.0 : 0.0, 0.0);
       }
     } else {
+      up_valid = false;   // this reflector changes the columns after k
       // beta = -sign(Re alpha) ||(alpha, x)||, tau = (beta - alpha) / beta,
       // v = x / (alpha - beta); alpha - beta != 0 here.  One reciprocal
       // square root gives both beta and 1/beta (the norm is far from the
@@ -861,6 +899,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     Vs = boff(Vs, bo);
     Tp = boff(Tp, bo);
     tau = boff(tau, bo);
+  }
+  // a panel of identity reflectors leaves every trailing column as it is
+  {
+    bool ident = true;
+    for (int q = 0; q < nbp; ++q) {
+      const double2 t = tau[c0 + q];
+      ident = ident && t.x == 0.0 && t.y == 0.0;
+    }
+    if (ident) return;
   }
   constexpr int NW = NT / 64;
   constexpr int NPAIR = OWN_G ? NB * (NB - 1) / 2 : 0;
