@@ -110,8 +110,9 @@ int mps_ctx_set_precision(mps_ctx_t ctx, int prec);
  * in complex FP64 that refit stage 4's target -1/3 psi + 1/3 psi1 +
  * 2/3 psi2 + 1/3 H3 psi3, each started from the previous result; the result
  * stays left canonical.  The context keeps one or two states of psi's shape
- * for it (the 10-argument form's copy of psi and the refit's intermediate).
- * Default 0.  MPS_E_STATE on a batched context. */
+ * for it (the 10-argument form's copy of psi and the refit's intermediate);
+ * on a batched context they come from the slab, so copy_bytes must hold two
+ * more states of psi's shape.  Default 0. */
 int mps_ctx_set_refine_sweeps(mps_ctx_t ctx, int n);
 /* Enable / disable hipGraph capture of whole RK4 steps (default on). */
 int mps_ctx_set_graphs(mps_ctx_t ctx, int enable);

@@ -383,8 +383,6 @@ int mps_ctx_set_precision(mps_ctx_t ctx, int prec) {
 int mps_ctx_set_refine_sweeps(mps_ctx_t ctx, int n) {
   return guard([&] {
     REQUIRE(n >= 0 && n <= 64 && n % 2 == 0, "refinement half-sweeps: an even number in [0, 64]");
-    if (n > 0 && ctx->c.bat.nb != 1)
-      throw Error(MPS_E_STATE, "RK4 refinement is not available on a batched context");
     ctx->c.refine = n;
     invalidate(ctx);
   });
@@ -1730,7 +1728,6 @@ int mps_rk4stp_apply_H_half(double dt, mps_mpo_t H1, mps_mpo_t H2, mps_mpo_t H3,
     Plan p(c, {&psi->s, &psi1->s, &psi2->s, &psi3->s, &out->s}, {&H1->o, &H2->o, &H3->o}, 4);
     track_ws(x, before);
     if (c->refine > 0) {
-      if (c->bat.nb != 1) throw Error(MPS_E_STATE, "RK4 refinement is not available on a batched context");
       p.rtmp = internal_like(x, x->rtmp, psi->s);
       if (out == psi) p.rkeep = internal_like(x, x->rkeep, psi->s);
     }

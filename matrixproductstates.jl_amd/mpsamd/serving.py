@@ -322,7 +322,9 @@ class TrajRequest:
     left: int = 0
     # "spin": quantum-jump trajectory (d = 2), result [<sigma_z^j>..., p_coh];
     # "rydberg": density-matrix evolution (d = 9, no draws: seed unused),
-    # result [trace, Rydberg-level population / trace]
+    # result [trace, Rydberg-level population / trace]; "rydberg_p": the same
+    # evolution in the precision form (operator contractions in complex FP32,
+    # two FP64 refinement half-sweeps of every step's last stage)
     kind: str = "spin"
 
 
@@ -401,16 +403,19 @@ class DensityMatrixPool:
     a view.  ``RydbergEvolution`` is the same evolution on a plain context.
     """
 
-    def __init__(self, N, D, slots=32, pos_seed=0, dt=0.005, n_exp=2, device=0):
-        self.key = ("rydberg", N, D)
+    def __init__(self, N, D, slots=32, pos_seed=0, dt=0.005, n_exp=2, device=0, precision="fp64", refine=0):
+        self.key = ("rydberg" if (precision, refine) == ("fp64", 0) else "rydberg_p", N, D)
         self.N, self.D, self.dt = N, D, dt
         self.p = MD.RydbergParams(n_exp=n_exp)
         Ls = MD.rydberg_rk4_mpos(self.p, MD.positions(N, float(N), pos_seed), dt)
         dims = M._dims_rule(N, D, 9)
-        copy_bytes, ws, w = slab_sizes(dims, 9, Ls, 4, slots, device)
+        # the refinement keeps two more states of rho's shape per copy
+        copy_bytes, ws, w = slab_sizes(dims, 9, Ls, 4 + (2 if refine else 0), slots, device)
         self.hbm_bytes = slab_bytes(copy_bytes, ws, slots)
         self.ctx = M.Context.batched(slots, copy_bytes, ws, device)
         self.ctx.set_qr_rank_tol(QR_RANK_TOL)
+        self.ctx.set_precision(precision)
+        self.ctx.set_refine_sweeps(refine)
         self.comp = self.ctx.companion()
         self.comp.set_qr_rank_tol(QR_RANK_TOL)
         self.L = [M.MPO(h, self.ctx) for h in Ls]
@@ -493,7 +498,10 @@ def solve_alone(r: TrajRequest, **system):
     """The same request on its own plain context (the sequential baseline)."""
     ctx = M.Context(0)
     ctx.set_qr_rank_tol(QR_RANK_TOL)
-    if r.kind == "rydberg":
+    if r.kind == "rydberg_p":
+        ctx.set_precision("fp32")
+        ctx.set_refine_sweeps(2)
+    if r.kind in ("rydberg", "rydberg_p"):
         tr = RydbergEvolution(r.N, r.D, r.pos_seed, ctx=ctx)
         for _ in range(r.steps):
             tr.launch_coherent()
@@ -512,11 +520,11 @@ def solve_alone(r: TrajRequest, **system):
     return r
 
 
-def density_pool_bytes(N, D, slots, n_exp=2, dt=0.005, device=0):
+def density_pool_bytes(N, D, slots, n_exp=2, dt=0.005, device=0, refine=0):
     """HBM bytes of the slab of a ``DensityMatrixPool``, before building it
     (the MPO shapes do not depend on the positions)."""
     Ls = MD.rydberg_rk4_mpos(MD.RydbergParams(n_exp=n_exp), MD.positions(N, float(N), 0), dt)
-    copy_bytes, ws, _ = slab_sizes(M._dims_rule(N, D, 9), 9, Ls, 4, slots, device)
+    copy_bytes, ws, _ = slab_sizes(M._dims_rule(N, D, 9), 9, Ls, 4 + (2 if refine else 0), slots, device)
     return slab_bytes(copy_bytes, ws, slots)
 
 
@@ -550,8 +558,8 @@ class PooledServer:
         """HBM bytes of the slab of the pool for (kind, N, D)."""
         if key not in self._bytes:
             kind, N, D = key
-            if kind == "rydberg":
-                self._bytes[key] = density_pool_bytes(N, D, self.dm_slots)
+            if kind in ("rydberg", "rydberg_p"):
+                self._bytes[key] = density_pool_bytes(N, D, self.dm_slots, refine=2 if kind == "rydberg_p" else 0)
             else:
                 self._bytes[key] = spin_ensemble_bytes(N, D, self.slots, self.system.get("dt", 0.01))
         return self._bytes[key]
@@ -560,6 +568,8 @@ class PooledServer:
         kind, N, D = key
         if kind == "rydberg":
             return DensityMatrixPool(N, D, self.dm_slots)
+        if kind == "rydberg_p":
+            return DensityMatrixPool(N, D, self.dm_slots, precision="fp32", refine=2)
         return TrajectoryPool(N, D, self.slots, **self.system)
 
     def run(self, requests, warm=()):

@@ -162,6 +162,34 @@ def test_density_matrix_pool_same_alone_and_in_the_mix():
     assert np.abs(single.result - alone.result).max() < 1e-10
 
 
+def test_precision_form_pool_same_alone_and_in_the_mix():
+    # density matrices in the precision form (FP32 operator contractions, two
+    # FP64 refinement half-sweeps per step) served from a lockstep pool: the
+    # same request alone and in a mix gives the same bits; against the same
+    # form on a plain context (other GEMM plans) and against the FP64 path it
+    # agrees to the precision form's bar, and it differs from FP64 (FP32 ran)
+    N, D, ps, steps = 6, 9, 3, 6
+    print("case precision-form pool", N, D, steps, flush=True)
+    alone = S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg_p")
+    pool = S.DensityMatrixPool(N, D, slots=4, precision="fp32", refine=2)
+    _pool_run(pool, {0: [(alone, 0)]}, steps)
+    pool.close()
+    rs = [S.TrajRequest(i, N, D, 10 + i, seed=0, steps=3 + i, kind="rydberg_p") for i in range(4)]
+    mix = S.TrajRequest(9, N, D, ps, seed=0, steps=steps, kind="rydberg_p")
+    pool = S.DensityMatrixPool(N, D, slots=4, precision="fp32", refine=2)
+    done = _pool_run(pool, {0: [(rs[0], 0), (rs[1], 1)], 2: [(mix, 2), (rs[2], 3)], 4: [(rs[3], 0)]}, 12)
+    pool.close()
+    assert mix in done and all(r in done for r in rs)
+    assert np.abs(mix.result - alone.result).max() <= SAME
+    plain_p = S.solve_alone(S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg_p"))
+    fp64 = S.solve_alone(S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg"))
+    d_plain = np.abs(plain_p.result - alone.result).max()
+    d64 = np.abs(fp64.result - alone.result)   # trace and Rydberg population: the form's bar is absolute
+    print("precision pool vs plain precision form", d_plain, "vs FP64", d64, "FP64", fp64.result, flush=True)
+    assert d_plain < 1e-6
+    assert d64.max() < 1e-6 and d64.max() > 1e-14
+
+
 def test_density_matrix_pool_matches_oracle():
     N, D, ps, steps, dt = 5, 9, 4, 4, 0.005
     r = S.TrajRequest(0, N, D, ps, seed=0, steps=steps, kind="rydberg")
