@@ -470,6 +470,52 @@ def pooled_serving_leg(seed, n_burst=480, n=320, rate=30.0, systems=4, N=60, D=3
         "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4),
         "hbm_budget_bytes": st["hbm_budget_bytes"], "peak_pool_bytes": st["peak_pool_bytes"],
         "by_kind": kinds}
+    out["three_forms"] = three_forms_leg(seed)
+    return out
+
+
+def three_forms_leg(seed, n=240, rate=50.0, budget=96 << 30, sample_per_kind=4):
+    """A Poisson stream drawing from the three evolution forms: quantum-jump
+    spin trajectories (d = 2, FP64), Rydberg density matrices (d = 9, FP64) and
+    the same density matrices in the precision form (FP32 contractions + two
+    FP64 refinement half-sweeps), over several (N, D) shapes, each (form,
+    shape) continuously batched in its own pool, pools admitted against an
+    HBM budget.  A seeded sample of each form's responses is solved again
+    alone on a plain context in FP64 (the path the parity tests pin to the
+    oracle) and compared."""
+    from mpsamd import serving as S
+    systems = ([("spin", Nn, Dd, seed + 20 + 2 * Nn + Dd) for Nn in (20, 60, 100) for Dd in (32, 64)]
+               + [("rydberg", Nn, 32, seed + 40 + Nn) for Nn in (40, 60)]
+               + [("rydberg_p", 40, 32, seed + 80)])
+    reqs = S.poisson_trajectory_requests(n, rate=rate, seed=seed + 3, systems=systems, steps_range=(10, 30))
+    st = S.PooledServer(slots=64, dm_slots=32, max_pools=len(systems), hbm_budget_bytes=budget, dt=0.01).run(reqs)
+    print(f"three-form serving: {st}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed + 4)
+    kinds, acc = {}, {}
+    for k in ("spin", "rydberg", "rydberg_p"):
+        mine = [r for r in reqs if r.kind == k]
+        lat = np.array([r.done - r.arrival for r in mine])
+        kinds[k] = {"requests": len(mine), "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                    "p99_latency_s": round(float(np.percentile(lat, 99)), 4)}
+        errs = []
+        for i in rng.choice(len(mine), size=min(sample_per_kind, len(mine)), replace=False):
+            r = mine[int(i)]
+            ref = S.TrajRequest(r.rid, r.N, r.D, r.pos_seed, r.seed, r.steps,
+                                kind="rydberg" if k == "rydberg_p" else k)
+            S.solve_alone(ref, dt=0.01) if k == "spin" else S.solve_alone(ref)
+            errs.append(float(np.abs(r.result - ref.result).max()))
+        acc[k] = {"sampled": len(errs), "max_abs_err_vs_fp64_alone": max(errs),
+                  "median_abs_err_vs_fp64_alone": float(np.median(errs))}
+    out = {"mix": "spin trajectories N in {20, 60, 100} x D in {32, 64} (64 slots each); Rydberg density "
+                  "matrices N in {40, 60}, D = 32 (32 slots), FP64 and (N = 40) precision form; 10-30 steps "
+                  "each, chains drawn uniformly",
+           "requests": st["requests"], "arrival_rate_req_s": rate, "throughput_req_s": round(st["throughput_req_s"], 3),
+           "p50_latency_s": round(st["p50_latency_s"], 4), "p99_latency_s": round(st["p99_latency_s"], 4),
+           "pools_opened": st["pools_opened"], "hbm_budget_bytes": st["hbm_budget_bytes"],
+           "peak_pool_bytes": st["peak_pool_bytes"], "by_kind": kinds, "accuracy": acc,
+           "accuracy_note": "results: spin [<sigma_z^j>..., last p_coh], density matrix [trace, Rydberg population];"
+                            " each sampled response against the same request alone in FP64"}
+    print(f"three-form serving: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -489,6 +535,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the batched-trajectory and serving legs")
     ap.add_argument("--gemm-counters", choices=["fp64", "fp32"], default=None,
                     help="only the calibration copy and the dominant GEMM launches (for rocprofv3 --pmc runs)")
+    ap.add_argument("--three-forms-only", action="store_true",
+                    help="run only the serving leg over the three evolution forms")
     ap.add_argument("--ragged-only", action="store_true",
                     help="run only the uneven-shape ragged-pool leg (512 trajectories, D up to 256)")
     ap.add_argument("--no-graphs", action="store_true",
@@ -512,6 +560,9 @@ def main():
         print(json.dumps({"gemm_counters": gemm_counters(args.gemm_counters)}))
         return
     import mpsamd as M
+    if args.three_forms_only:
+        print(json.dumps({"three_forms": three_forms_leg(args.seed)}))
+        return
     if args.ragged_only:
         print(json.dumps({"ragged_pools": ragged_pools_leg(args.seed)}))
         return
