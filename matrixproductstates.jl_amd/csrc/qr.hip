@@ -207,18 +207,35 @@ __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {   // conj(a)
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
-__global__ __launch_bounds__(64) void qr_colnorm_kernel(const double2* __restrict__ A, int m, int64_t rs,
-                                                        int64_t cs, double* __restrict__ colnrm, int64_t bs) {
+// ||A(:, c)||^2 per column: four loads per thread in flight at a time, then
+// a fixed-order sum (per thread, wave butterfly, then the four waves)
+constexpr int CN_NT = 256;
+__global__ __launch_bounds__(CN_NT) void qr_colnorm_kernel(const double2* __restrict__ A, int m, int64_t rs,
+                                                           int64_t cs, double* __restrict__ colnrm, int64_t bs) {
   A = boff(A, blockIdx.y * bs);
   colnrm = boff(colnrm, blockIdx.y * bs);
-  const int c = blockIdx.x, lane = threadIdx.x;
+  const int c = blockIdx.x, tid = threadIdx.x;
+  __shared__ double sh[CN_NT / 64];
   double s = 0.0;
-  for (int r = lane; r < m; r += 64) {
-    const double2 v = A[(int64_t)r * rs + (int64_t)c * cs];
-    s += v.x * v.x + v.y * v.y;
+  for (int r0 = 0; r0 < m; r0 += 4 * CN_NT) {
+    double2 v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + tid + CN_NT * i;
+      v[i] = r < m ? A[(int64_t)r * rs + (int64_t)c * cs] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s += v[i].x * v[i].x + v[i].y * v[i].y;
   }
   s = wave_sum(s);
-  if (lane == 0) colnrm[c] = s;
+  if ((tid & 63) == 0) sh[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < CN_NT / 64; ++w) t += sh[w];
+    colnrm[c] = t;
+  }
 }
 
 // Cross-half sums with the gfx950 lane swaps (VALU, no LDS round trip):
@@ -336,19 +353,19 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     Tp = boff(Tp, bo);
   }
   constexpr int NW = NT / 64;
-  constexpr int GMAX = (2 * (NB - 1) + 7) / 8 + 1;        // 8-value groups per column step (+1 look-ahead)
-  static_assert(GMAX <= 8, "panel too wide");
+  constexpr int GSTEP = (2 * (NB - 1) + 7) / 8 + 1;       // 8-value groups per column step (+1 look-ahead)
+  static_assert(GSTEP <= 8, "panel too wide");
+  constexpr int GMAX = GSTEP > 4 ? GSTEP : 4;              // the up-front reduction uses 4
   __shared__ double red[2][NW * GMAX * 64];
   __shared__ double2 s_tau[NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int M = m - c0;
-  double thr = 0.0;
-  if (tol > 0.0) {
-    double t = 0.0;
-    for (int c = 0; c < n; ++c) t += colnrm[c];
-    thr = tol * sqrt(t);
-  }
-  const double thr2 = thr * thr;
+  // rank cutoff tol * ||A||_F from the column norms, summed in a fixed order
+  // (per thread, then the wave butterfly, then over waves); the loads are
+  // issued with the panel's own
+  double tsum = 0.0;
+  if (tol > 0.0)
+    for (int c = tid; c < n; c += NT) tsum += colnrm[c];
   double2 a[R][NB];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -405,13 +422,29 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     red[ph][(wave * GMAX + 0) * 64 + lane] = wave_rs8(vx);
     red[ph][(wave * GMAX + 1) * 64 + lane] = wave_rs8(vr);
     red[ph][(wave * GMAX + 2) * 64 + lane] = wave_rs8(vi);
+    if (tol > 0.0) {
+      const double vt[8] = {tsum, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      red[ph][(wave * GMAX + 3) * 64 + lane] = wave_rs8(vt);
+    }
     __syncthreads();
     up_xs = wave_sum_slots(red[ph], 0);
     up_ar = wave_sum_slots(red[ph], 1);
     up_ai = wave_sum_slots(red[ph], 2);
+    if (tol > 0.0) tsum = bcast(wave_sum_slots(red[ph], 3), 0);
     ph ^= 1;
     up_valid = true;
+  } else if (tol > 0.0) {
+    // red[1] is first written by the second column step, after the first
+    // step's barrier, so every thread has read these partials by then
+    const double w = wave_sum(tsum);
+    if (lane == 0) red[1][wave] = w;
+    __syncthreads();
+    tsum = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) tsum += red[1][q];
   }
+  const double thr = tol > 0.0 ? tol * sqrt(tsum) : 0.0;
+  const double thr2 = thr * thr;
   // One column step: column k sits in slot O of a[.][], the columns trailing
   // it in slots O+1 .. W-1 (zero past the panel) and are updated in place.
   // Steps go in pairs (O = 0, then 1) and the panel then shifts left by two
@@ -900,15 +933,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     Tp = boff(Tp, bo);
     tau = boff(tau, bo);
   }
-  // a panel of identity reflectors leaves every trailing column as it is
-  {
-    bool ident = true;
-    for (int q = 0; q < nbp; ++q) {
-      const double2 t = tau[c0 + q];
-      ident = ident && t.x == 0.0 && t.y == 0.0;
-    }
-    if (ident) return;
-  }
   constexpr int NW = NT / 64;
   constexpr int NPAIR = OWN_G ? NB * (NB - 1) / 2 : 0;
   constexpr int NVAL = 2 * NB + 2 * NPAIR;               // w (NB complex), then the Gram pairs
@@ -933,8 +957,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   const int j = c0 + nbp + col;
   const int M = m - c0;
   const double2* V = Vs + (int64_t)c0 * m + c0;   // reflector q (panel-local), row c0 + r: V[r + q m]
-  double2 ct = make_double2(0.0, 0.0);
-  if (OWN_G && tid < nbp) ct = tau[c0 + tid];
+  // lane q < nbp of every wave reads tau_q (loaded with the operands)
+  const int lane_q = (threadIdx.x & 63) < nbp ? (int)(threadIdx.x & 63) : nbp - 1;
+  const double2 tq = tau[c0 + lane_q];
   double2 x[RA], v[RA][NB];
 #pragma unroll
   for (int i = 0; i < RA; ++i) {
@@ -949,6 +974,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
       v[i][q] = (ok && q < nbp) ? va : make_double2(0.0, 0.0);
     }
   }
+  // a panel of identity reflectors leaves every trailing column as it is
+  // (the ballot is the same in every wave: the whole workgroup returns)
+  if (__ballot((threadIdx.x & 63) < nbp && (tq.x != 0.0 || tq.y != 0.0)) == 0) return;
+  const double2 ct = (OWN_G && tid < nbp) ? tq : make_double2(0.0, 0.0);
   // value e: e < 2 NB: Re/Im of w_{e/2} = v_{e/2}^H x; else Re/Im of pair
   // p = (e - 2 NB) / 2, G[pair_j(p)][pair_l(p)].  Rows outermost: the 8
   // sums of a group advance together, so consecutive fmas are independent
@@ -1256,7 +1285,7 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
   const unsigned nb = (unsigned)bat.nb;
   const int64_t bs = bat.bs;
   if (tol > 0.0)
-    MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(64), 0, s, A, m, rs, cs, colnrm, bs);
+    MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, colnrm, bs);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
