@@ -95,6 +95,9 @@ struct Gemm {
   // the fixed rule; force_tm > 0 overrides both (the tuner's timing runs).
   bool tunable = false;
   int force_tm = 0, force_splits = 0;
+  // cap on the workgroups of the launch (0: none): the output blocks are then
+  // walked by fewer workgroups, leaving CUs to concurrent work on other streams
+  int max_wg = 0;
 };
 
 enum Compute { CMP_F64 = 0, CMP_F32_3M = 1 };

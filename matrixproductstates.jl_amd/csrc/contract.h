@@ -54,6 +54,9 @@ struct Ctx {
   // site behind the current sweep (default priority)
   hipStream_t aux = nullptr;
   hipEvent_t ev_site = nullptr, ev_aux = nullptr;
+  // workgroup cap of the aux stream's GEMMs: they run while the main stream
+  // factors a site (a QR on a few CUs at a time) and must leave CUs to it
+  int aux_max_wg = 0;
   void ensure_aux();
   double* dscal = nullptr;      // 256 doubles of device scalars
   double* hscal = nullptr;      // pinned host mirror

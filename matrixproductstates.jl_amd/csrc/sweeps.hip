@@ -353,6 +353,31 @@ static void nx_left_end(Plan& p, const NextInit* nx, int N) {
   for (int k = 0; k < nx->n_ovl; ++k) set1(p.aux_w, nx->oe[k]->slot(N), nx->ocoef[k]);
 }
 
+// The next sweep's environment step for a site is queued one site late:
+// behind the products of the following site, so the aux stream's GEMMs run
+// while the main stream factors that site (a latency-bound QR on a few CUs)
+// instead of competing with the main stream's products.
+struct NxQueue {
+  Plan& p;
+  const NextInit* nx;
+  State* A0;
+  bool left;
+  int pend = -1;
+  void push(int j) {
+    flush();
+    pend = j;
+  }
+  void flush() {
+    if (pend < 0 || !nx) return;
+    SiteScope ss(p.ctx, pend);
+    if (left)
+      nx_left_step(p, nx, A0, pend);
+    else
+      nx_right_step(p, nx, A0, pend);
+    pend = -1;
+  }
+};
+
 // compress_var_apply_H_left! (compression.jl:321-347)
 void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State* A, MPO* H,
                                const NextInit* nx, bool init_done) {
@@ -362,6 +387,7 @@ void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State
   else
     init_left_op(p, envop, Ain, A, H, 1.0);
   nx_right_begin(p, nx, N);
+  NxQueue nq{p, nx, A0, false};
   for (int j = N - 1; j >= 0; --j) {
     SiteScope ss(p.ctx, j);
     gap_pass(p, j, false, envop, nullptr, 0);
@@ -370,12 +396,14 @@ void compress_var_apply_H_left(Plan& p, State* A0, Env* envop, State* Ain, State
              H->wr(j), envop->slot(j + 1), A0->Dr(j));
     prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, A->Dl(j), H->wl(j), A0->Dl(j), A->d[j],
                A0->Dr(j), false);
+    nq.flush();
     if (j == 0) break;
     qr_right(p, A0, j, nullptr);
-    nx_right_step(p, nx, A0, j);
+    nq.push(j);
     update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), A->Dl(j), H->wl(j), A->d[j], A0->Dr(j),
               A0->Dl(j));
   }
+  nq.flush();
   nx_right_end(p, nx);
 }
 
@@ -405,6 +433,7 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
   State* Ah = A[numv - 1];
   if (!init_done) init_left_op(p, envop, Ain, Ah, H, coef[numv - 1]);
   nx_right_begin(p, nx, N);
+  NxQueue nq{p, nx, A0, false};
   for (int j = N - 1; j >= 0; --j) {
     SiteScope ss(p.ctx, j);
     gap_pass(p, j, false, envop, env, numv - 1);
@@ -422,14 +451,16 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     p.join();
     prod_LR_34(p.work, p.op, A0->site(j), envop->slot(j), p.T2, Ah->Dl(j), H->wl(j), Dl0, d, Dr0,
                numv > 1);
+    nq.flush();
     if (j == 0) break;
     qr_right(p, A0, j, nullptr);
     p.fork();
     for (int ll = 0; ll + 1 < numv; ++ll)
       update_renv(ow, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
-    nx_right_step(p, nx, A0, j);
+    nq.push(j);
     update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
   }
+  nq.flush();
   nx_right_end(p, nx);
   p.join();
 }
@@ -459,6 +490,7 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
   State* Ah = A[numv - 1];
   if (!init_done) init_right_op(p, envop, Ain, Ah, H, coef[numv - 1]);
   nx_left_begin(p, nx);
+  NxQueue nq{p, nx, A0, true};
   for (int j = 0; j < N; ++j) {
     SiteScope ss(p.ctx, j);
     gap_pass(p, j, true, envop, env, numv - 1);
@@ -475,14 +507,16 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     p.join();
     prod_LR_43(p.work, p.op, A0->site(j), p.T2, envop->slot(j + 1), Dl0, d, Ah->Dr(j), H->wr(j), Dr0,
                numv > 1);
+    nq.flush();
     if (j == N - 1) break;
     qr_left(p, A0, j, nullptr);
     p.fork();
     for (int ll = 0; ll + 1 < numv; ++ll)
       update_lenv(ow, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
-    nx_left_step(p, nx, A0, j);
+    nq.push(j);
     update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
   }
+  nq.flush();
   nx_left_end(p, nx, N);
   p.join();
 }

@@ -13,7 +13,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPO = os.path.dirname(PKG)
-SOURCES = ["gemm.hip", "qr.hip", "svd.hip", "misc.hip", "contract.hip", "sweeps.hip", "steady.hip", "twosite.hip", "capi.hip"]
+SOURCES = ["gemm.hip", "gemm_capped.hip", "qr.hip", "svd.hip", "misc.hip", "contract.hip", "sweeps.hip", "steady.hip", "twosite.hip", "capi.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -202,3 +202,34 @@ def test_arnoldi_two_streams(rng):
     got = grhov[0].to_host()
     for j in range(N):
         assert rel(got[j], rhov[0][j]) < 1e-9, j
+
+
+def test_rk4_capped_environment_stream_bitwise(monkeypatch):
+    # The stream that builds the next sweep's environments runs its GEMMs on
+    # a capped grid (workgroups walk several output blocks); the per-block
+    # arithmetic is unchanged, so a hard cap (8 workgroups) and no cap give
+    # the same bits after two steps.
+    N, D, dt = 6, 64, 0.005
+    p = models.RydbergParams(n_exp=2)
+    z = models.positions(N, 6.0, 3)
+    L = models.rydberg_rk4_mpos(p, z, dt)
+    rho, dims = O.mpsrandom(N, D, 9, np.random.default_rng(31))
+    wop = [1] + [p.w] * (N - 1) + [1]
+    out = []
+    for cap in ("0", "8"):
+        print("case aux cap", cap, flush=True)
+        monkeypatch.setenv("MPS_AUX_MAX_WG", cap)   # read when the context first needs the stream
+        ctx = M.Context(0)
+        gL = [M.MPO(h, ctx) for h in L]
+        g = M.MPS.from_host(rho, ctx)
+        g1, g2, g3 = (M.MPS.from_host(rho, ctx) for _ in range(3))
+        ge = [M.build_env(dims, dims, ctx=ctx) for _ in range(3)]
+        geop = M.build_env(dims, dims, wop, ctx=ctx)
+        for _ in range(2):
+            M.RK4stp_apply_H_half_(dt, gL, g, g1, g2, g3, *ge, geop)
+        out.append(g.to_host())
+        for o in (g, g1, g2, g3, *ge, geop, *gL):
+            o.__del__()
+        ctx.close()
+    for j in range(N):
+        assert np.array_equal(out[0][j], out[1][j]), j
