@@ -455,10 +455,11 @@ void compress_sum_var_apply_H_left(Plan& p, State* A0, Env** env, Env* envop, St
     if (j == 0) break;
     qr_right(p, A0, j, nullptr);
     p.fork();
+    // the operator environment first: the main stream's next products wait on it
+    update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
     for (int ll = 0; ll + 1 < numv; ++ll)
       update_renv(ow, env[ll]->slot(j), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dl(j));
     nq.push(j);
-    update_re(p.work, p.op, envop->slot(j), p.T2, A0->site(j), Ah->Dl(j), H->wl(j), d, Dr0, Dl0);
   }
   nq.flush();
   nx_right_end(p, nx);
@@ -511,10 +512,11 @@ void compress_sum_var_apply_H_right(Plan& p, State* A0, Env** env, Env* envop, S
     if (j == N - 1) break;
     qr_left(p, A0, j, nullptr);
     p.fork();
+    // the operator environment first: the main stream's next products wait on it
+    update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
     for (int ll = 0; ll + 1 < numv; ++ll)
       update_lenv(ow, env[ll]->slot(j + 1), A0->site(j), p.ovl[ll], Dl0, d, Dr0, A[ll]->Dr(j), 0);
     nq.push(j);
-    update_le(p.work, p.op, envop->slot(j + 1), p.T2, A0->site(j), Dl0, d, Ah->Dr(j), H->wr(j), Dr0);
   }
   nq.flush();
   nx_left_end(p, nx, N);
