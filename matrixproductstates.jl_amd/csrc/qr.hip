@@ -21,6 +21,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 namespace mpship {
@@ -210,11 +211,20 @@ __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {   // conj(a)
 // ||A(:, c)||^2 per column: four loads per thread in flight at a time, then
 // a fixed-order sum (per thread, wave butterfly, then the four waves)
 constexpr int CN_NT = 256;
+// (norms = false: only the progress words).  Workgroup 0 also zeroes the
+// chain kernel's progress words prog[0 .. nprog).
 __global__ __launch_bounds__(CN_NT) void qr_colnorm_kernel(const double2* __restrict__ A, int m, int64_t rs,
-                                                           int64_t cs, double* __restrict__ colnrm, int64_t bs) {
+                                                           int64_t cs, double* __restrict__ colnrm,
+                                                           unsigned* __restrict__ prog, int nprog, bool norms,
+                                                           int64_t bs) {
   A = boff(A, blockIdx.y * bs);
   colnrm = boff(colnrm, blockIdx.y * bs);
   const int c = blockIdx.x, tid = threadIdx.x;
+  if (prog != nullptr && c == 0) {
+    prog = boff(prog, blockIdx.y * bs);
+    for (int i = tid; i < nprog; i += CN_NT) prog[i] = 0u;
+  }
+  if (!norms) return;
   __shared__ double sh[CN_NT / 64];
   double s = 0.0;
   for (int r0 = 0; r0 < m; r0 += 4 * CN_NT) {
@@ -337,48 +347,57 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // zlarft) from the Gram entries G[j][l] = v_j^H v_l; otherwise the apply
 // kernel works from the Gram entries itself and T is formed after the
 // factorization from the Gram GEMM.
-template <int R, int NB, int NT, bool PT>
-__global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, int m, int n, int64_t rs,
-                                                      int64_t cs, int c0, int nbp, double tol,
-                                                      const double* __restrict__ colnrm,
-                                                      double2* __restrict__ Vs, double2* __restrict__ tau,
-                                                      double2* __restrict__ Tp, int64_t bs) {
-  static_assert(NT > NB, "the first row slot must cover the panel's diagonal");
-  {
-    const int64_t bo = blockIdx.y * bs;   // lockstep copy
-    A = boff(A, bo);
-    colnrm = boff(colnrm, bo);
-    Vs = boff(Vs, bo);
-    tau = boff(tau, bo);
-    Tp = boff(Tp, bo);
+// Stores of a reflector and its tau: plain, or (CHAIN) agent-scope relaxed
+// atomic stores, which reach the level where other workgroups' agent-scope
+// loads read them, so a progress counter stored after them (once the stores
+// are complete) publishes them to the chain kernel's other workgroups.
+template <bool CHAIN>
+__device__ __forceinline__ void st_c(double2* p, double2 v) {
+  if constexpr (CHAIN) {
+    double* q = reinterpret_cast<double*>(p);
+    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    *p = v;
   }
-  constexpr int NW = NT / 64;
-  constexpr int GSTEP = (2 * (NB - 1) + 7) / 8 + 1;       // 8-value groups per column step (+1 look-ahead)
+}
+__device__ __forceinline__ double2 ld_agent(const double2* p) {
+  const double* q = reinterpret_cast<const double*>(p);
+  return make_double2(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                      __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+// wait until every store this thread issued is complete (vmcnt(0); expcnt and
+// lgkmcnt left alone)
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// LDS reduction slots of the panel factorization
+template <int NB, int NT>
+struct PanelShape {
+  static constexpr int NW = NT / 64;
+  static constexpr int GSTEP = (2 * (NB - 1) + 7) / 8 + 1;   // 8-value groups per column step (+1 look-ahead)
   static_assert(GSTEP <= 8, "panel too wide");
-  constexpr int GMAX = GSTEP > 4 ? GSTEP : 4;              // the up-front reduction uses 4
-  __shared__ double red[2][NW * GMAX * 64];
-  __shared__ double2 s_tau[NB];
+  static constexpr int GMAX = GSTEP > 4 ? GSTEP : 4;         // the up-front reduction uses 4
+  static constexpr int RED = NW * GMAX * 64;
+};
+
+// Factors the NB-column panel held in a[R][NB] (columns c0 .. c0+nbp-1).
+// Thread t's slot i holds row base + t + NT i, where base = c0 for the panel
+// kernel (rows from the panel's diagonal down) and 0 for the chain kernel
+// (all m rows); tk = t + base - c0 is slot 0's row relative to the panel's
+// first diagonal element, so the diagonal of column k is tk == k, rows with
+// tk < k are R entries and slots past M rows are zero.  Ak / Vk point at this
+// thread's slot-0 row of column c0 in A and in the reflector store Vs.
+// CHAIN: reflectors and tau are published (agent-scope stores) and *prog is
+// advanced to k+1 once column k's reflector is complete, then to nbp.
+template <int R, int NB, int NT, bool CHAIN>
+__device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, const int M, const int m,
+                                             const int64_t cs, const int c0, const int nbp, const double tol,
+                                             double tsum, double2* const Ak, double2* const Vk,
+                                             double2* __restrict__ tau, double2* s_tau,
+                                             double (&red)[2][PanelShape<NB, NT>::RED], unsigned* prog) {
+  constexpr int NW = PanelShape<NB, NT>::NW;
+  constexpr int GMAX = PanelShape<NB, NT>::GMAX;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int M = m - c0;
-  // rank cutoff tol * ||A||_F from the column norms, summed in a fixed order
-  // (per thread, then the wave butterfly, then over waves); the loads are
-  // issued with the panel's own
-  double tsum = 0.0;
-  if (tol > 0.0)
-    for (int c = tid; c < n; c += NT) tsum += colnrm[c];
-  double2 a[R][NB];
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    const int rr = tid + NT * i;
-    const int64_t ro = (int64_t)(c0 + min(rr, M - 1)) * rs;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const double2 x = A[ro + (int64_t)(c0 + min(j, nbp - 1)) * cs];
-      a[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
-    }
-  }
-  double2* const Ak = A + (int64_t)(c0 + tid) * rs + (int64_t)c0 * cs;   // row c0 + tid (slot 0), column c0
-  double2* const Vk = Vs + (int64_t)c0 * m + c0 + tid;                    // column c0, row c0 + tid
   // sum over waves of the butterfly results of group g; lane l gets value l >> 3
   auto wave_sum_slots = [&](const double* r, int g) {
     double s = 0.0;
@@ -413,11 +432,11 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       for (int i = 0; i < R; ++i) {
         const double2 x = a[i][k];
         const double n2 = fma(x.x, x.x, x.y * x.y);
-        s += (i > 0 || tid > k) ? n2 : 0.0;
+        s += (i > 0 || tk > k) ? n2 : 0.0;
       }
       vx[k] = s;
-      vr[k] = (tid == k) ? a[0][k].x : 0.0;
-      vi[k] = (tid == k) ? a[0][k].y : 0.0;
+      vr[k] = (tk == k) ? a[0][k].x : 0.0;
+      vi[k] = (tk == k) ? a[0][k].y : 0.0;
     }
     red[ph][(wave * GMAX + 0) * 64 + lane] = wave_rs8(vx);
     red[ph][(wave * GMAX + 1) * 64 + lane] = wave_rs8(vr);
@@ -473,10 +492,10 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       for (int i = 0; i < R; ++i) {
         const double2 x = a[i][O];
         const double n2 = fma(x.x, x.x, x.y * x.y);
-        v8[0] += (i > 0 || tid > k) ? n2 : 0.0;
+        v8[0] += (i > 0 || tk > k) ? n2 : 0.0;
       }
-      v8[1] = (tid == k) ? a[0][O].x : 0.0;
-      v8[2] = (tid == k) ? a[0][O].y : 0.0;
+      v8[1] = (tk == k) ? a[0][O].x : 0.0;
+      v8[2] = (tk == k) ? a[0][O].y : 0.0;
       red[ph][wave * GMAX * 64 + lane] = wave_rs8(v8);
       __syncthreads();
       const double s0 = wave_sum_slots(red[ph], 0);
@@ -490,15 +509,15 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     if (ident) {
       if (tid == 0) {
         s_tau[k] = make_double2(0.0, 0.0);
-        tau[c0 + k] = make_double2(0.0, 0.0);
+        st_c<CHAIN>(tau + c0 + k, make_double2(0.0, 0.0));
       }
       // column k out unchanged (R part to A; Vs: 1 at row k, zero elsewhere);
       // the trailing columns stay as they are
-      if (tid <= k) Ak[(int64_t)k * cs] = a[0][O];
+      if (tk <= k) Ak[(int64_t)k * cs] = a[0][O];
 #pragma unroll
       for (int i = 0; i < R; ++i) {
         const int rr = tid + NT * i;
-        if (rr < M) Vk[(int64_t)k * m + NT * i] = make_double2(rr == k ? 1.0 : 0.0, 0.0);
+        if (rr < M) st_c<CHAIN>(Vk + (int64_t)k * m + NT * i, make_double2((i == 0 && tk == k) ? 1.0 : 0.0, 0.0));
       }
     } else {
       up_valid = false;   // this reflector changes the columns after k
@@ -514,9 +533,9 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       const double iden = rcp_nr(fma(dr, dr, ai * ai));
       const double2 t = make_double2((beta - ar) * ib, -ai * ib);
       const double2 sc = make_double2(dr * iden, -ai * iden);
-      if (tid == 0) {
-        s_tau[k] = t;
-        tau[c0 + k] = t;
+      if (tid == 0) s_tau[k] = t;
+      if constexpr (!CHAIN) {
+        if (tid == 0) tau[c0 + k] = t;
       }
       // vm = v_k on rows k..M-1 and 0 elsewhere (masks the dots and the update)
       double2 vm[R];
@@ -524,14 +543,20 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
       for (int i = 0; i < R; ++i) {
         const double2 y = cmul(a[i][O], sc);
         if (i == 0)
-          vm[0] = (tid == k) ? make_double2(1.0, 0.0) : ((tid > k) ? y : make_double2(0.0, 0.0));
+          vm[0] = (tk == k) ? make_double2(1.0, 0.0) : ((tk > k) ? y : make_double2(0.0, 0.0));
         else
           vm[i] = y;
       }
-      if (tid <= k) Ak[(int64_t)k * cs] = (tid == k) ? make_double2(beta, 0.0) : a[0][O];
+      // column k out: R part to A, the reflector to Vs (CHAIN: after the
+      // barrier below, published at the next one)
+      auto put_column = [&]() {
+        if (tk <= k) Ak[(int64_t)k * cs] = (tk == k) ? make_double2(beta, 0.0) : a[0][O];
 #pragma unroll
-      for (int i = 0; i < R; ++i)
-        if (tid + NT * i < M) Vk[(int64_t)k * m + NT * i] = vm[i];
+        for (int i = 0; i < R; ++i)
+          if (tid + NT * i < M) st_c<CHAIN>(Vk + (int64_t)k * m + NT * i, vm[i]);
+        if (CHAIN && tid == 0) st_c<CHAIN>(tau + c0 + k, t);
+      };
+      if constexpr (!CHAIN) put_column();
       // s_j = v_k^H a_j for the T trailing slots, four columns per butterfly
 #pragma unroll
       for (int g = 0; g < GW; ++g) {
@@ -560,21 +585,29 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
 #pragma unroll
         for (int i = 0; i < R; ++i) {
           const double2 x = a[i][O + 1], v = vm[i];
-          const double2 xb = (i > 0 || tid > k + 1) ? x : make_double2(0.0, 0.0);
-          const double2 vb = (i > 0 || tid > k + 1) ? v : make_double2(0.0, 0.0);
+          const double2 xb = (i > 0 || tk > k + 1) ? x : make_double2(0.0, 0.0);
+          const double2 vb = (i > 0 || tk > k + 1) ? v : make_double2(0.0, 0.0);
           v8[0] = fma(xb.x, xb.x, fma(xb.y, xb.y, v8[0]));
           v8[1] = fma(vb.x, xb.x, fma(vb.y, xb.y, v8[1]));
           v8[2] = fma(vb.x, xb.y, fma(-vb.y, xb.x, v8[2]));
           v8[3] = fma(vb.x, vb.x, fma(vb.y, vb.y, v8[3]));
         }
-        const bool at = (tid == k + 1);
+        const bool at = (tk == k + 1);
         v8[4] = at ? a[0][O + 1].x : 0.0;
         v8[5] = at ? a[0][O + 1].y : 0.0;
         v8[6] = at ? vm[0].x : 0.0;
         v8[7] = at ? vm[0].y : 0.0;
         red[ph][(wave * GMAX + GW) * 64 + lane] = wave_rs8(v8);
       }
+      // CHAIN: the columns before k were stored before this point; once
+      // every thread's stores are complete, they are published
+      if constexpr (CHAIN) wait_stores();
       __syncthreads();
+      if constexpr (CHAIN) {
+        if (tid == 0 && k > 0)
+          __hip_atomic_store(prog, (unsigned)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        put_column();
+      }
       double sg[GW];
 #pragma unroll
       for (int g = 0; g < GW; ++g) sg[g] = wave_sum_slots(red[ph], g);
@@ -637,6 +670,55 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
     segment(std::integral_constant<int, 8>{}, 0, min(nbp, 4));
     segment(std::integral_constant<int, 4>{}, 4, nbp);
   }
+  if constexpr (CHAIN) {
+    // the last column (stored after its step's barrier) and trailing
+    // identity columns
+    wait_stores();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(prog, (unsigned)nbp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int R, int NB, int NT, bool PT>
+__global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, int m, int n, int64_t rs,
+                                                      int64_t cs, int c0, int nbp, double tol,
+                                                      const double* __restrict__ colnrm,
+                                                      double2* __restrict__ Vs, double2* __restrict__ tau,
+                                                      double2* __restrict__ Tp, int64_t bs) {
+  static_assert(NT > NB, "the first row slot must cover the panel's diagonal");
+  {
+    const int64_t bo = blockIdx.y * bs;   // lockstep copy
+    A = boff(A, bo);
+    colnrm = boff(colnrm, bo);
+    Vs = boff(Vs, bo);
+    tau = boff(tau, bo);
+    Tp = boff(Tp, bo);
+  }
+  constexpr int NW = NT / 64;
+  __shared__ double red[2][PanelShape<NB, NT>::RED];
+  __shared__ double2 s_tau[NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int M = m - c0;
+  // rank cutoff tol * ||A||_F from the column norms, summed in a fixed order
+  // (per thread, then the wave butterfly, then over waves); the loads are
+  // issued with the panel's own
+  double tsum = 0.0;
+  if (tol > 0.0)
+    for (int c = tid; c < n; c += NT) tsum += colnrm[c];
+  double2 a[R][NB];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int rr = tid + NT * i;
+    const int64_t ro = (int64_t)(c0 + min(rr, M - 1)) * rs;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double2 x = A[ro + (int64_t)(c0 + min(j, nbp - 1)) * cs];
+      a[i][j] = (rr < M && j < nbp) ? x : make_double2(0.0, 0.0);
+    }
+  }
+  double2* const Ak = A + (int64_t)(c0 + tid) * rs + (int64_t)c0 * cs;   // row c0 + tid (slot 0), column c0
+  double2* const Vk = Vs + (int64_t)c0 * m + c0 + tid;                    // column c0, row c0 + tid
+  panel_factor<R, NB, NT, false>(a, tid, M, m, cs, c0, nbp, tol, tsum, Ak, Vk, tau, s_tau, red, nullptr);
   // rows above the panel of its reflectors are zero (the GEMMs of the Q
   // formation run over all m rows)
   for (int r = tid; r < c0; r += NT)
@@ -779,6 +861,162 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restric
     for (int c = 0; c < NB; ++c)
       Tall[(int64_t)p * NB * NB + lane + c * NB] = (lane < nbp && c < nbp) ? trow[c] : make_double2(0.0, 0.0);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Chain QR: the whole factorization (zgeqr2's reflectors) in one launch of
+// P = ceil(n/8) workgroups per copy.  Workgroup p holds columns 8p .. 8p+7
+// over all m rows in registers (thread t, slot i: row t + NT i).  It applies
+// the reflectors of panels 0 .. p-1 one at a time, as their owners publish
+// them,
+//   a_j <- a_j - conj(tau_c) v_c (v_c^H a_j)        (H(c)^H, as zgeqr2),
+// then factors its own panel with panel_factor, which publishes each
+// reflector (its Vs column and tau, agent-scope stores) by advancing
+// prog[p].  The owner of the next panel applies the current panel's
+// reflectors while they are being produced, so no panel waits for a
+// separate trailing-update launch, and the factorization is one dispatch
+// instead of one panel and one update launch per 8 columns.
+// prog[0 .. P) are zero at launch (qr_colnorm_kernel clears them); prog[k] = c
+// means reflectors 0 .. c-1 of panel k are complete, PROG_ABORT that panel's
+// owner gave up.  A workgroup waits only on panels of lower index, owned by
+// workgroups of lower index in the same copy, which dispatch (in order of
+// workgroup index) has placed on the GPU before it.  Every wait is bounded by
+// CHAIN_WAIT_TICKS of s_memrealtime (100 MHz): past it the workgroup marks
+// its own panel aborted and stores NaN as its tau, so a failure reaches Q as
+// NaN instead of hanging the queue.
+constexpr unsigned PROG_ABORT = 0x100u;
+constexpr uint64_t CHAIN_WAIT_TICKS = 100000000ull;   // 1 s
+constexpr int CH_NB = 8;
+
+template <int R, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void qr_chain_kernel(
+    double2* __restrict__ A, int m, int n, int64_t rs, int64_t cs, double tol, const double* __restrict__ colnrm,
+    double2* __restrict__ Vs, double2* __restrict__ tau, unsigned* __restrict__ prog, int64_t bs) {
+  constexpr int NB = CH_NB;
+  static_assert(NT >= 256, "the first row slot must cover every diagonal row (n <= NT)");
+  {
+    const int64_t bo = blockIdx.y * bs;   // lockstep copy
+    A = boff(A, bo);
+    colnrm = boff(colnrm, bo);
+    Vs = boff(Vs, bo);
+    tau = boff(tau, bo);
+    prog = boff(prog, bo);
+  }
+  constexpr int NW = NT / 64;
+  constexpr int GMAX = PanelShape<NB, NT>::GMAX;
+  __shared__ double red[2][PanelShape<NB, NT>::RED];
+  __shared__ double2 s_tau[NB];
+  __shared__ unsigned s_poll[2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p = blockIdx.x, c0 = NB * p, nbp = min(NB, n - c0);
+  double tsum = 0.0;
+  if (tol > 0.0)
+    for (int c = tid; c < n; c += NT) tsum += colnrm[c];
+  double2 a[R][NB];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int rr = tid + NT * i;
+    const int64_t ro = (int64_t)min(rr, m - 1) * rs;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double2 x = A[ro + (int64_t)(c0 + min(j, nbp - 1)) * cs];
+      a[i][j] = (rr < m && j < nbp) ? x : make_double2(0.0, 0.0);
+    }
+  }
+  auto bcast = [&](double v, int srclane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), srclane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), srclane);
+    return __hiloint2double(hi, lo);
+  };
+  auto load_refl = [&](int c, double2 (&v)[R], double2& t) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int rr = tid + NT * i;
+      v[i] = (rr < m) ? ld_agent(Vs + (int64_t)c * m + rr) : make_double2(0.0, 0.0);
+    }
+    t = ld_agent(tau + c);
+  };
+  int ph = 0, pp = 0;
+  bool fail = false;
+  for (int k = 0; k < p && !fail; ++k) {
+    unsigned done = 0;
+    while (done < (unsigned)NB) {
+      if (tid == 0) {
+        unsigned v = __hip_atomic_load(prog + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v <= done) {
+          const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+          do {
+            __builtin_amdgcn_s_sleep(1);
+            v = __hip_atomic_load(prog + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          } while (v <= done && __builtin_amdgcn_s_memrealtime() - t0 < CHAIN_WAIT_TICKS);
+        }
+        s_poll[pp] = v;
+      }
+      __syncthreads();
+      const unsigned avail = s_poll[pp];
+      pp ^= 1;
+      if (avail <= done || avail > (unsigned)NB) {
+        fail = true;
+        break;
+      }
+      // reflectors done .. avail-1, the next one's loads in flight while the
+      // current one is applied
+      double2 vc[R], tc;
+      load_refl(NB * k + (int)done, vc, tc);
+      for (unsigned c = done; c < avail; ++c) {
+        double2 vn[R], tn = make_double2(0.0, 0.0);
+        if (c + 1 < avail) load_refl(NB * k + (int)c + 1, vn, tn);
+        if (tc.x != 0.0 || tc.y != 0.0) {   // the same value in every thread
+          // s_j = v^H a_j, columns 0-3 and 4-7 in one butterfly each
+          double v8[2][8];
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+              const double2 x = a[i][j];
+              sr = fma(vc[i].x, x.x, fma(vc[i].y, x.y, sr));
+              si = fma(vc[i].x, x.y, fma(-vc[i].y, x.x, si));
+            }
+            v8[j / 4][2 * (j % 4)] = sr;
+            v8[j / 4][2 * (j % 4) + 1] = si;
+          }
+          red[ph][(wave * GMAX + 0) * 64 + lane] = wave_rs8(v8[0]);
+          red[ph][(wave * GMAX + 1) * 64 + lane] = wave_rs8(v8[1]);
+          __syncthreads();
+          double sg[2] = {0.0, 0.0};
+#pragma unroll
+          for (int w = 0; w < NW; ++w) {
+            sg[0] += red[ph][(w * GMAX + 0) * 64 + lane];
+            sg[1] += red[ph][(w * GMAX + 1) * 64 + lane];
+          }
+          ph ^= 1;
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            const int q = j % 4;
+            const double sr = bcast(sg[j / 4], 16 * q), si = bcast(sg[j / 4], 16 * q + 8);
+            const double fr = tc.x * sr + tc.y * si, fi = tc.x * si - tc.y * sr;
+#pragma unroll
+            for (int i = 0; i < R; ++i) a[i][j] = cupd(a[i][j], vc[i], fr, fi);
+          }
+        }
+        if (c + 1 < avail) {
+#pragma unroll
+          for (int i = 0; i < R; ++i) vc[i] = vn[i];
+          tc = tn;
+        }
+      }
+      done = avail;
+    }
+  }
+  if (fail) {
+    if (tid == 0) __hip_atomic_store(prog + p, PROG_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < nbp) tau[c0 + tid] = make_double2(__builtin_nan(""), __builtin_nan(""));
+    return;
+  }
+  __syncthreads();   // the last apply's reads of red[] before panel_factor writes it
+  panel_factor<R, NB, NT, true>(a, tid - c0, m, m, cs, c0, nbp, tol, tsum, A + (int64_t)tid * rs + (int64_t)c0 * cs,
+                                Vs + (int64_t)c0 * m + tid, tau, s_tau, red, prog + p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1267,10 +1505,57 @@ static QrGemms qr_gemms(double2* A, int64_t m, int64_t n, int64_t rs, int64_t cs
 struct QrWork {
   double2 *Vs, *tau, *Tall, *G, *W;
   double* colnrm;
+  unsigned* prog;
   void* gws;
   size_t gws_bytes;
   int num_cus;
 };
+
+// Q = I(:, 0:n) - V W over the panel from the reflectors in q.Vs / q.tau (n <= FW_NMAX)
+static void qr_form_q(double2* A, int m, int n, int64_t rs, int64_t cs, const QrWork& q, double2* R_out,
+                      hipStream_t s, Bat bat) {
+  const unsigned nb = (unsigned)bat.nb;
+  const int64_t bs = bat.bs;
+  QrGemms g = qr_gemms(A, m, n, rs, cs, q.Vs, q.G, q.W);
+  g.gram.bat = bat;
+  g.q.bat = bat;
+  launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+  const dim3 grid((unsigned)((n + 3) / 4), nb);
+  if (n <= 64)
+    MPS_LAUNCH((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, q.Vs, q.G, q.tau, q.W, R_out, bs);
+  else if (n <= 128)
+    MPS_LAUNCH((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, q.Vs, q.G, q.tau, q.W, R_out, bs);
+  else
+    MPS_LAUNCH((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, q.Vs, q.G, q.tau, q.W, R_out, bs);
+  launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+}
+
+// The chain factorization (qr_chain_kernel) then the Q formation; needs
+// n <= min(NT, FW_NMAX) and m within R * NT rows.
+template <int R, int NT>
+static void qr_chain(double2* A, int m, int n, int64_t rs, int64_t cs, double tol, const QrWork& q, double2* R_out,
+                     hipStream_t s, Bat bat) {
+  const unsigned nb = (unsigned)bat.nb;
+  const int64_t bs = bat.bs;
+  const int P = (n + CH_NB - 1) / CH_NB;
+  MPS_LAUNCH(qr_colnorm_kernel, dim3(tol > 0.0 ? (unsigned)n : 1u, nb), dim3(CN_NT), 0, s, A, m, rs, cs, q.colnrm,
+             q.prog, P, tol > 0.0, bs);
+  MPS_LAUNCH((qr_chain_kernel<R, NT>), dim3((unsigned)P, nb), dim3(NT), 0, s, A, m, n, rs, cs, tol, q.colnrm, q.Vs,
+             q.tau, q.prog, bs);
+  qr_form_q(A, m, n, rs, cs, q, R_out, s, bat);
+}
+
+// MPS_QR_CHAIN=0 in the environment selects the launch-per-panel factorization
+static bool chain_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("MPS_QR_CHAIN");
+    return e == nullptr || atoi(e) != 0;
+  }();
+  return on;
+}
+static bool chain_fits(int64_t n, int nt, const Bat& bat) {
+  return chain_enabled() && bat.nb == 1 && n <= nt && n <= FW_NMAX;
+}
 
 // PT: the panel kernel forms each panel's T (16-column panels, where the
 // apply kernel's own Gram reduction would cost more than the panel's);
@@ -1285,7 +1570,8 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
   const unsigned nb = (unsigned)bat.nb;
   const int64_t bs = bat.bs;
   if (tol > 0.0)
-    MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, colnrm, bs);
+    MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, colnrm,
+               (unsigned*)nullptr, 0, true, bs);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
@@ -1296,23 +1582,11 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
       MPS_LAUNCH((qr_apply_col_kernel<RA, NB, NTA, !PT>), dim3((unsigned)trail, nb), dim3(NTA), 0, s, A,
                          m, rs, cs, c0, nbp, Vs, Tp, tau, bs);
   }
-  QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
-  g.gram.bat = bat;
-  g.q.bat = bat;
   if (n <= FW_NMAX) {
-    launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
-    const dim3 grid((unsigned)((n + 3) / 4), nb);
-    if (n <= 64)
-      MPS_LAUNCH((qr_wsolve_kernel<1>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
-                         bs);
-    else if (n <= 128)
-      MPS_LAUNCH((qr_wsolve_kernel<2>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
-                         bs);
-    else
-      MPS_LAUNCH((qr_wsolve_kernel<4>), grid, dim3(256), 0, s, A, m, n, rs, cs, Vs, q.G, tau, q.W, R_out,
-                         bs);
-    launch_gemm(g.q, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
+    qr_form_q(A, m, n, rs, cs, q, R_out, s, bat);
   } else {
+    QrGemms g = qr_gemms(A, m, n, rs, cs, Vs, q.G, q.W);
+    g.gram.bat = bat;
     if (!PT) {
       launch_gemm(g.gram, CMP_F64, q.gws, q.gws_bytes, q.num_cus, s);
       MPS_LAUNCH((qr_tfactor_kernel<NB>), dim3((unsigned)((n + NB - 1) / NB), nb), dim3(64), 0, s, q.G,
@@ -1326,7 +1600,7 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
 // V (m x n) | tau (n) | column norms (n) | T blocks (ceil(n/NB) x NB x NB,
 // NB <= 16) | G (n x n) | W (n x n) | split-K partials of the two GEMMs
 struct QrLayout {
-  size_t vs, tau, cn, tall, g, w, gws, total, gws_bytes;
+  size_t vs, tau, cn, prog, tall, g, w, gws, total, gws_bytes;
 };
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
@@ -1335,6 +1609,7 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
   L.vs = o; o = al256(o + (size_t)m * n * sizeof(double2));
   L.tau = o; o = al256(o + (size_t)n * sizeof(double2));
   L.cn = o; o = al256(o + (size_t)n * sizeof(double));
+  L.prog = o; o = al256(o + (size_t)((n + CH_NB - 1) / CH_NB) * sizeof(unsigned));
   L.tall = o; o = al256(o + (size_t)(n + 16) * 16 * sizeof(double2));
   L.g = o; o = al256(o + (size_t)n * n * sizeof(double2));
   L.w = o; o = al256(o + (size_t)n * n * sizeof(double2));
@@ -1364,6 +1639,7 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
   q.Vs = (double2*)(w + L.vs);
   q.tau = (double2*)(w + L.tau);
   q.colnrm = (double*)(w + L.cn);
+  q.prog = (unsigned*)(w + L.prog);
   q.Tall = (double2*)(w + L.tall);
   q.G = (double2*)(w + L.g);
   q.W = (double2*)(w + L.w);
@@ -1386,6 +1662,8 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
     qr_blocked<2, 8, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 768)
     qr_blocked<3, 8, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
+  else if (m <= 1280 && chain_fits(n, 256, bat))
+    qr_chain<5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 1280)
     qr_blocked<5, 8, 256, 20, 5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 2560)

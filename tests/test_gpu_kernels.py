@@ -82,7 +82,7 @@ def test_overlap_contractions(dev, shape, rng):
 
 
 @pytest.mark.parametrize("mn", [(8, 4), (64, 32), (256, 128), (1152, 128), (300, 77), (1000, 50),
-                                (2304, 256), (2600, 40)])
+                                (2304, 256), (2600, 40), (1152, 8), (1280, 256), (900, 3)])
 @pytest.mark.parametrize("view", [False, True])
 def test_qrq_full_rank(dev, mn, view, rng):
     m, n = mn
@@ -106,7 +106,8 @@ def test_qrq_zero_padded_closed_form(dev):
 
 
 @pytest.mark.parametrize("tol", [1e-13, 1e-8])
-@pytest.mark.parametrize("mnr", [(64, 16, 3), (600, 70, 9), (1152, 128, 20), (2304, 40, 5)])
+@pytest.mark.parametrize("mnr", [(64, 16, 3), (600, 70, 9), (1152, 128, 20), (2304, 40, 5), (1100, 256, 60),
+                                 (1152, 128, 125)])
 def test_qrq_rank_cutoff(dev, tol, mnr, rng):
     # low-rank panel plus rounding-level noise: with the cutoff both sides take
     # the identity reflector on the noise columns, so the completion agrees.
