@@ -347,28 +347,42 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // zlarft) from the Gram entries G[j][l] = v_j^H v_l; otherwise the apply
 // kernel works from the Gram entries itself and T is formed after the
 // factorization from the Gram GEMM.
-// Stores of a reflector and its tau: plain, or (CHAIN) agent-scope relaxed
-// atomic stores, which reach the level where other workgroups' agent-scope
-// loads read them, so a progress counter stored after them (once the stores
-// are complete) publishes them to the chain kernel's other workgroups.
-template <bool CHAIN>
-__device__ __forceinline__ void st_c(double2* p, double2 v) {
-  if constexpr (CHAIN) {
-    double* q = reinterpret_cast<double*>(p);
-    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    *p = v;
-  }
+// Exchange between the chain kernel's workgroups: raw-buffer loads and
+// stores with sc1 set (device scope), the accesses agent-scope atomic loads
+// and stores compile to: such a store is written through to where the other
+// workgroups' sc1 loads read it, and it is complete (vmcnt) once it is there.
+constexpr int AUX_SC1 = 16;                 // cache-policy operand: sc1
+constexpr uint32_t OFF_NONE = 0xFFFFFFF0u;  // past every buffer: the hardware drops the access
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ double2 ld_agent(const double2* p) {
-  const double* q = reinterpret_cast<const double*>(p);
-  return make_double2(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                      __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ double2 ld_dev(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, AUX_SC1);
+  return make_double2(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
 }
-// wait until every store this thread issued is complete (vmcnt(0); expcnt and
-// lgkmcnt left alone)
-__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+template <int AUX>
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, double2 x) {
+  typedef int v4i __attribute__((ext_vector_type(4)));
+  const v4i v = {__double2loint(x.x), __double2hiint(x.x), __double2loint(x.y), __double2hiint(x.y)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)byte_off, 0, AUX);
+}
+// where the chain kernel's panel_factor writes: R entries to A (plain), the
+// reflectors to Vs and tau (sc1); byte offsets of this thread's slot-0 row of
+// column c0 and the steps per column and per row slot
+struct ChainIO {
+  __amdgpu_buffer_rsrc_t rA, rV, rT;
+  uint32_t abase, acol, vbase, vcol, vslot;
+  unsigned* prog;
+};
+// Every column a chain workgroup finishes costs each thread exactly
+// CHAIN_OPS stores (R entry, R reflector slots, tau; a store the thread does
+// not make goes to OFF_NONE), so vmcnt(CHAIN_OPS) means "all but the latest
+// column's stores are complete".
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
 
 // LDS reduction slots of the panel factorization
 template <int NB, int NT>
@@ -387,14 +401,17 @@ struct PanelShape {
 // first diagonal element, so the diagonal of column k is tk == k, rows with
 // tk < k are R entries and slots past M rows are zero.  Ak / Vk point at this
 // thread's slot-0 row of column c0 in A and in the reflector store Vs.
-// CHAIN: reflectors and tau are published (agent-scope stores) and *prog is
-// advanced to k+1 once column k's reflector is complete, then to nbp.
+// CHAIN: the column stores go through io (see ChainIO), and the reflectors
+// are published through io->prog, which holds the number c of complete
+// reflectors in bits 0-7 and, in bits 8-15, which of them are identities
+// (tau = 0).  A column's stores are issued after its step's barrier; at each
+// later barrier every column but the latest one is complete and published.
 template <int R, int NB, int NT, bool CHAIN>
 __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, const int M, const int m,
                                              const int64_t cs, const int c0, const int nbp, const double tol,
                                              double tsum, double2* const Ak, double2* const Vk,
-                                             double2* __restrict__ tau, double2* s_tau,
-                                             double (&red)[2][PanelShape<NB, NT>::RED], unsigned* prog) {
+                                             double2* tau, double2* s_tau,
+                                             double (&red)[2][PanelShape<NB, NT>::RED], const ChainIO* io) {
   constexpr int NW = PanelShape<NB, NT>::NW;
   constexpr int GMAX = PanelShape<NB, NT>::GMAX;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -411,6 +428,21 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
     return __hiloint2double(hi, lo);
   };
   int ph = 0;
+  unsigned idmask = 0;   // CHAIN: identity reflectors of this panel so far (bit k)
+  int stored = 0;        // CHAIN: columns whose stores have been issued
+  // CHAIN: column k's stores, R + 2 per thread (R entry rows <= k, reflector
+  // slots, tau from thread 0)
+  auto chain_store = [&](int k, double2 rval, const double2 (&vv)[R], double2 tval) {
+    if constexpr (CHAIN) {
+      st_buf<0>(io->rA, tk <= k ? io->abase + (uint32_t)k * io->acol : OFF_NONE, rval);
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        st_buf<AUX_SC1>(io->rV, tid + NT * i < M ? io->vbase + (uint32_t)k * io->vcol + (uint32_t)i * io->vslot : OFF_NONE,
+                        vv[i]);
+      st_buf<AUX_SC1>(io->rT, tid == 0 ? (uint32_t)(c0 + k) * 16u : OFF_NONE, tval);
+      stored = k + 1;
+    }
+  };
   // look-ahead: the zlarfg inputs of the next column, downdated from sums
   // reduced together with the current column's dot products (uniform)
   bool have_next = false;
@@ -507,17 +539,23 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
     have_next = false;
     const bool ident = (thr > 0.0) ? (xs <= thr2 && fabs(ai) <= thr) : (xs == 0.0 && ai == 0.0);
     if (ident) {
-      if (tid == 0) {
-        s_tau[k] = make_double2(0.0, 0.0);
-        st_c<CHAIN>(tau + c0 + k, make_double2(0.0, 0.0));
-      }
+      idmask |= 1u << k;
+      if (tid == 0) s_tau[k] = make_double2(0.0, 0.0);
       // column k out unchanged (R part to A; Vs: 1 at row k, zero elsewhere);
       // the trailing columns stay as they are
-      if (tk <= k) Ak[(int64_t)k * cs] = a[0][O];
+      if constexpr (CHAIN) {
+        double2 e[R];
 #pragma unroll
-      for (int i = 0; i < R; ++i) {
-        const int rr = tid + NT * i;
-        if (rr < M) st_c<CHAIN>(Vk + (int64_t)k * m + NT * i, make_double2((i == 0 && tk == k) ? 1.0 : 0.0, 0.0));
+        for (int i = 0; i < R; ++i) e[i] = make_double2((i == 0 && tk == k) ? 1.0 : 0.0, 0.0);
+        chain_store(k, a[0][O], e, make_double2(0.0, 0.0));
+      } else {
+        if (tid == 0) tau[c0 + k] = make_double2(0.0, 0.0);
+        if (tk <= k) Ak[(int64_t)k * cs] = a[0][O];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int rr = tid + NT * i;
+          if (rr < M) Vk[(int64_t)k * m + NT * i] = make_double2((i == 0 && tk == k) ? 1.0 : 0.0, 0.0);
+        }
       }
     } else {
       up_valid = false;   // this reflector changes the columns after k
@@ -548,15 +586,13 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
           vm[i] = y;
       }
       // column k out: R part to A, the reflector to Vs (CHAIN: after the
-      // barrier below, published at the next one)
-      auto put_column = [&]() {
+      // barrier below)
+      if constexpr (!CHAIN) {
         if (tk <= k) Ak[(int64_t)k * cs] = (tk == k) ? make_double2(beta, 0.0) : a[0][O];
 #pragma unroll
         for (int i = 0; i < R; ++i)
-          if (tid + NT * i < M) st_c<CHAIN>(Vk + (int64_t)k * m + NT * i, vm[i]);
-        if (CHAIN && tid == 0) st_c<CHAIN>(tau + c0 + k, t);
-      };
-      if constexpr (!CHAIN) put_column();
+          if (tid + NT * i < M) Vk[(int64_t)k * m + NT * i] = vm[i];
+      }
       // s_j = v_k^H a_j for the T trailing slots, four columns per butterfly
 #pragma unroll
       for (int g = 0; g < GW; ++g) {
@@ -599,14 +635,15 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
         v8[7] = at ? vm[0].y : 0.0;
         red[ph][(wave * GMAX + GW) * 64 + lane] = wave_rs8(v8);
       }
-      // CHAIN: the columns before k were stored before this point; once
-      // every thread's stores are complete, they are published
-      if constexpr (CHAIN) wait_stores();
+      // CHAIN: every column stored before the latest one is complete in
+      // every thread past the barrier: publish them, then store column k
+      if constexpr (CHAIN) wait_vm<R + 2>();
       __syncthreads();
       if constexpr (CHAIN) {
-        if (tid == 0 && k > 0)
-          __hip_atomic_store(prog, (unsigned)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        put_column();
+        if (tid == 0 && stored > 1)
+          __hip_atomic_store(io->prog, (unsigned)(stored - 1) | (idmask << 8), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        chain_store(k, (tk == k) ? make_double2(beta, 0.0) : a[0][O], vm, t);
       }
       double sg[GW];
 #pragma unroll
@@ -671,11 +708,11 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
     segment(std::integral_constant<int, 4>{}, 4, nbp);
   }
   if constexpr (CHAIN) {
-    // the last column (stored after its step's barrier) and trailing
-    // identity columns
-    wait_stores();
+    // the rest, once complete
+    wait_vm<0>();
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(prog, (unsigned)nbp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0)
+      __hip_atomic_store(io->prog, (unsigned)nbp | (idmask << 8), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -718,7 +755,8 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(double2* __restrict__ A, i
   }
   double2* const Ak = A + (int64_t)(c0 + tid) * rs + (int64_t)c0 * cs;   // row c0 + tid (slot 0), column c0
   double2* const Vk = Vs + (int64_t)c0 * m + c0 + tid;                    // column c0, row c0 + tid
-  panel_factor<R, NB, NT, false>(a, tid, M, m, cs, c0, nbp, tol, tsum, Ak, Vk, tau, s_tau, red, nullptr);
+  panel_factor<R, NB, NT, false>(a, tid, M, m, cs, c0, nbp, tol, tsum, Ak, Vk, tau, s_tau, red,
+                                 (const ChainIO*)nullptr);
   // rows above the panel of its reflectors are zero (the GEMMs of the Q
   // formation run over all m rows)
   for (int r = tid; r < c0; r += NT)
@@ -864,34 +902,41 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Chain QR: the whole factorization (zgeqr2's reflectors) in one launch of
-// P = ceil(n/8) workgroups per copy.  Workgroup p holds columns 8p .. 8p+7
-// over all m rows in registers (thread t, slot i: row t + NT i).  It applies
-// the reflectors of panels 0 .. p-1 one at a time, as their owners publish
-// them,
+// Chain QR: the whole factorization (zgeqr2's reflectors) in one launch.
+// Each of the P = ceil(n/8) panels has a workgroup that holds its columns
+// 8p .. 8p+7 over all m rows in registers (thread t, slot i: row t + NT i).
+// It applies the reflectors of panels 0 .. p-1 one at a time, as their owners
+// publish them,
 //   a_j <- a_j - conj(tau_c) v_c (v_c^H a_j)        (H(c)^H, as zgeqr2),
 // then factors its own panel with panel_factor, which publishes each
-// reflector (its Vs column and tau, agent-scope stores) by advancing
-// prog[p].  The owner of the next panel applies the current panel's
-// reflectors while they are being produced, so no panel waits for a
-// separate trailing-update launch, and the factorization is one dispatch
-// instead of one panel and one update launch per 8 columns.
-// prog[0 .. P) are zero at launch (qr_colnorm_kernel clears them); prog[k] = c
-// means reflectors 0 .. c-1 of panel k are complete, PROG_ABORT that panel's
-// owner gave up.  A workgroup waits only on panels of lower index, owned by
-// workgroups of lower index in the same copy, which dispatch (in order of
-// workgroup index) has placed on the GPU before it.  Every wait is bounded by
-// CHAIN_WAIT_TICKS of s_memrealtime (100 MHz): past it the workgroup marks
-// its own panel aborted and stores NaN as its tau, so a failure reaches Q as
-// NaN instead of hanging the queue.
-constexpr unsigned PROG_ABORT = 0x100u;
+// reflector (its Vs column and tau) by advancing prog[p].  The owner of the
+// next panel applies the current panel's reflectors while they are being
+// produced, so no panel waits for a separate trailing-update launch, and the
+// factorization is one dispatch instead of a panel and an update launch per
+// 8 columns.
+// The published reflectors travel through device-scope (sc1) stores and
+// loads (ChainIO), which reach across the XCDs' separate L2 caches.
+// prog[0 .. P) are zero at launch (qr_colnorm_kernel clears them); bits 0-7
+// of prog[k] count the complete reflectors of panel k, bits 8-15 mark which
+// of them are identities (those are skipped without loading them), and
+// PROG_ABORT says that panel's owner gave up.  Workgroup p is panel p's
+// owner and waits only on panels of lower index, owned by workgroups of lower
+// index, which dispatch (in order of workgroup index) has placed on the GPU
+// before it.  Every wait
+// is bounded by CHAIN_WAIT_TICKS of s_memrealtime (100 MHz): past it the
+// workgroup marks its own panel aborted and stores NaN as its tau, so a
+// failure reaches Q as NaN instead of hanging the queue.
+constexpr unsigned PROG_ABORT = 0x80000000u;
 constexpr uint64_t CHAIN_WAIT_TICKS = 100000000ull;   // 1 s
 constexpr int CH_NB = 8;
 
 template <int R, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void qr_chain_kernel(
     double2* __restrict__ A, int m, int n, int64_t rs, int64_t cs, double tol, const double* __restrict__ colnrm,
-    double2* __restrict__ Vs, double2* __restrict__ tau, unsigned* __restrict__ prog, int64_t bs) {
+    double2* Vs, double2* tau, unsigned* prog, int64_t bs) {
+  // Vs, tau and prog are written by other workgroups while this one reads
+  // them: no __restrict__ (with it the compiler takes a polled word as
+  // invariant and hoists its load out of the wait loop)
   constexpr int NB = CH_NB;
   static_assert(NT >= 256, "the first row slot must cover every diagonal row (n <= NT)");
   {
@@ -909,6 +954,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   __shared__ unsigned s_poll[2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p = blockIdx.x, c0 = NB * p, nbp = min(NB, n - c0);
+  ChainIO io;
+  io.rA = buf_rsrc(A, (uint32_t)(((int64_t)(m - 1) * rs + (int64_t)(n - 1) * cs + 1) * sizeof(double2)));
+  io.rV = buf_rsrc(Vs, (uint32_t)((size_t)m * n * sizeof(double2)));
+  io.rT = buf_rsrc(tau, (uint32_t)(n * sizeof(double2)));
+  io.abase = (uint32_t)(((int64_t)tid * rs + (int64_t)c0 * cs) * sizeof(double2));
+  io.acol = (uint32_t)(cs * sizeof(double2));
+  io.vbase = (uint32_t)(((int64_t)c0 * m + tid) * sizeof(double2));
+  io.vcol = (uint32_t)(m * sizeof(double2));
+  io.vslot = (uint32_t)(NT * sizeof(double2));
+  io.prog = prog + p;
   double tsum = 0.0;
   if (tol > 0.0)
     for (int c = tid; c < n; c += NT) tsum += colnrm[c];
@@ -932,79 +987,94 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int rr = tid + NT * i;
-      v[i] = (rr < m) ? ld_agent(Vs + (int64_t)c * m + rr) : make_double2(0.0, 0.0);
+      v[i] = ld_dev(io.rV, rr < m ? (uint32_t)(((size_t)c * m + rr) * sizeof(double2)) : OFF_NONE);
     }
-    t = ld_agent(tau + c);
+    t = ld_dev(io.rT, (uint32_t)(c * sizeof(double2)));
   };
   int ph = 0, pp = 0;
   bool fail = false;
+  // applies reflector v (tau t, the same in every thread) to the 8 columns
+  auto apply = [&](const double2 (&v)[R], double2 t) {
+    // s_j = v^H a_j, columns 0-3 and 4-7 in one butterfly each
+    double v8[2][8];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const double2 x = a[i][j];
+        sr = fma(v[i].x, x.x, fma(v[i].y, x.y, sr));
+        si = fma(v[i].x, x.y, fma(-v[i].y, x.x, si));
+      }
+      v8[j / 4][2 * (j % 4)] = sr;
+      v8[j / 4][2 * (j % 4) + 1] = si;
+    }
+    red[ph][(wave * GMAX + 0) * 64 + lane] = wave_rs8(v8[0]);
+    red[ph][(wave * GMAX + 1) * 64 + lane] = wave_rs8(v8[1]);
+    __syncthreads();
+    double sg[2] = {0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      sg[0] += red[ph][(w * GMAX + 0) * 64 + lane];
+      sg[1] += red[ph][(w * GMAX + 1) * 64 + lane];
+    }
+    ph ^= 1;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int q = j % 4;
+      const double sr = bcast(sg[j / 4], 16 * q), si = bcast(sg[j / 4], 16 * q + 8);
+      const double fr = t.x * sr + t.y * si, fi = t.x * si - t.y * sr;
+#pragma unroll
+      for (int i = 0; i < R; ++i) a[i][j] = cupd(a[i][j], v[i], fr, fi);
+    }
+  };
   for (int k = 0; k < p && !fail; ++k) {
     unsigned done = 0;
     while (done < (unsigned)NB) {
       if (tid == 0) {
         unsigned v = __hip_atomic_load(prog + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v <= done) {
+        if ((v & 0xFFu) <= done) {
           const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
           do {
             __builtin_amdgcn_s_sleep(1);
             v = __hip_atomic_load(prog + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          } while (v <= done && __builtin_amdgcn_s_memrealtime() - t0 < CHAIN_WAIT_TICKS);
+          } while ((v & 0xFFu) <= done && __builtin_amdgcn_s_memrealtime() - t0 < CHAIN_WAIT_TICKS);
         }
         s_poll[pp] = v;
       }
       __syncthreads();
-      const unsigned avail = s_poll[pp];
+      const unsigned pv = s_poll[pp];
       pp ^= 1;
-      if (avail <= done || avail > (unsigned)NB) {
+      const unsigned avail = pv & 0xFFu;
+      if ((pv & PROG_ABORT) != 0u || avail <= done || avail > (unsigned)NB) {
         fail = true;
         break;
       }
-      // reflectors done .. avail-1, the next one's loads in flight while the
-      // current one is applied
-      double2 vc[R], tc;
-      load_refl(NB * k + (int)done, vc, tc);
-      for (unsigned c = done; c < avail; ++c) {
-        double2 vn[R], tn = make_double2(0.0, 0.0);
-        if (c + 1 < avail) load_refl(NB * k + (int)c + 1, vn, tn);
-        if (tc.x != 0.0 || tc.y != 0.0) {   // the same value in every thread
-          // s_j = v^H a_j, columns 0-3 and 4-7 in one butterfly each
-          double v8[2][8];
-#pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            double sr = 0.0, si = 0.0;
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-              const double2 x = a[i][j];
-              sr = fma(vc[i].x, x.x, fma(vc[i].y, x.y, sr));
-              si = fma(vc[i].x, x.y, fma(-vc[i].y, x.x, si));
-            }
-            v8[j / 4][2 * (j % 4)] = sr;
-            v8[j / 4][2 * (j % 4) + 1] = si;
-          }
-          red[ph][(wave * GMAX + 0) * 64 + lane] = wave_rs8(v8[0]);
-          red[ph][(wave * GMAX + 1) * 64 + lane] = wave_rs8(v8[1]);
-          __syncthreads();
-          double sg[2] = {0.0, 0.0};
-#pragma unroll
-          for (int w = 0; w < NW; ++w) {
-            sg[0] += red[ph][(w * GMAX + 0) * 64 + lane];
-            sg[1] += red[ph][(w * GMAX + 1) * 64 + lane];
-          }
-          ph ^= 1;
-#pragma unroll
-          for (int j = 0; j < NB; ++j) {
-            const int q = j % 4;
-            const double sr = bcast(sg[j / 4], 16 * q), si = bcast(sg[j / 4], 16 * q + 8);
-            const double fr = tc.x * sr + tc.y * si, fi = tc.x * si - tc.y * sr;
-#pragma unroll
-            for (int i = 0; i < R; ++i) a[i][j] = cupd(a[i][j], vc[i], fr, fi);
-          }
-        }
-        if (c + 1 < avail) {
-#pragma unroll
-          for (int i = 0; i < R; ++i) vc[i] = vn[i];
-          tc = tn;
-        }
+      // the non-identity reflectors among done .. avail-1, applied in order
+      // through two register buffers: the next one's loads are issued as
+      // soon as a buffer is free, so they overlap the other buffer's apply.
+      // A dynamic loop (two copies of the apply, not eight) keeps the code
+      // within the instruction cache; more buffers spill (the 8 columns
+      // already take 160 registers).
+      unsigned rem = ((1u << avail) - (1u << done)) & ~(pv >> 8);
+      auto pop = [&]() {
+        if (rem == 0u) return -1;
+        const int q = __builtin_ctz(rem);
+        rem &= rem - 1u;
+        return q;
+      };
+      double2 vb0[R], vb1[R], tb0 = make_double2(0.0, 0.0), tb1 = make_double2(0.0, 0.0);
+      int q0 = pop(), q1 = pop();
+      if (q0 >= 0) load_refl(NB * k + q0, vb0, tb0);
+      if (q1 >= 0) load_refl(NB * k + q1, vb1, tb1);
+      while (q0 >= 0) {
+        apply(vb0, tb0);
+        q0 = pop();
+        if (q0 >= 0) load_refl(NB * k + q0, vb0, tb0);
+        if (q1 < 0) break;
+        apply(vb1, tb1);
+        q1 = pop();
+        if (q1 >= 0) load_refl(NB * k + q1, vb1, tb1);
       }
       done = avail;
     }
@@ -1016,7 +1086,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   }
   __syncthreads();   // the last apply's reads of red[] before panel_factor writes it
   panel_factor<R, NB, NT, true>(a, tid - c0, m, m, cs, c0, nbp, tol, tsum, A + (int64_t)tid * rs + (int64_t)c0 * cs,
-                                Vs + (int64_t)c0 * m + tid, tau, s_tau, red, prog + p);
+                                Vs + (int64_t)c0 * m + tid, tau, s_tau, red, &io);
 }
 
 // ---------------------------------------------------------------------------
@@ -1553,6 +1623,7 @@ static bool chain_enabled() {
   }();
   return on;
 }
+
 static bool chain_fits(int64_t n, int nt, const Bat& bat) {
   return chain_enabled() && bat.nb == 1 && n <= nt && n <= FW_NMAX;
 }
