@@ -225,6 +225,7 @@ int mps_ctx_create(int device, mps_ctx_t* out) {
       hipDeviceProp_t prop;
       HIPCHK(hipGetDeviceProperties(&prop, device));
       x->c.num_cus = prop.multiProcessorCount;
+      qr_chain_probe(device);
       HIPCHK(hipStreamCreateWithFlags(&x->c.stream, hipStreamNonBlocking));
       x->c.own_stream = true;
       HIPCHK(hipMalloc((void**)&x->c.dscal, 256 * sizeof(double)));
@@ -249,6 +250,7 @@ int mps_ctx_create_batch(int device, int nb, int64_t copy_bytes, int64_t ws_byte
       hipDeviceProp_t prop;
       HIPCHK(hipGetDeviceProperties(&prop, device));
       c.num_cus = prop.multiProcessorCount;
+      qr_chain_probe(device);
       HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
       c.own_stream = true;
       // copy stride: scalars + workspace + the caller's containers, 256-byte aligned

@@ -119,6 +119,9 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus);
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
                void* R_out, hipStream_t s, Bat bat = Bat());
+// once per device, outside graph capture: checks the workgroup-to-XCD layout
+// the single-launch (chain) QR relies on; launch_qr uses that path only after it passed
+void qr_chain_probe(int device);
 
 // ------------------------------------------------------- device helpers
 // DPP move of a double (two 32-bit halves), bound_ctrl: out-of-row sources read 0

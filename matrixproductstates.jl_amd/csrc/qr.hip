@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 
 namespace mpship {
@@ -211,18 +212,23 @@ __device__ __forceinline__ double2 cconjmul(double2 a, double2 b) {   // conj(a)
 // ||A(:, c)||^2 per column: four loads per thread in flight at a time, then
 // a fixed-order sum (per thread, wave butterfly, then the four waves)
 constexpr int CN_NT = 256;
-// (norms = false: only the progress words).  Workgroup 0 also zeroes the
-// chain kernel's progress words prog[0 .. nprog).
+// (norms = false: none).  For the chain kernel (prog != nullptr) workgroup c
+// also zeroes column c of the reflector store Vs (m rows; identity
+// reflectors are not stored there), and workgroup 0 zeroes the progress
+// words prog[0 .. nprog).
 __global__ __launch_bounds__(CN_NT) void qr_colnorm_kernel(const double2* __restrict__ A, int m, int64_t rs,
                                                            int64_t cs, double* __restrict__ colnrm,
-                                                           unsigned* __restrict__ prog, int nprog, bool norms,
-                                                           int64_t bs) {
+                                                           unsigned* __restrict__ prog, int nprog,
+                                                           double2* __restrict__ Vs, bool norms, int64_t bs) {
   A = boff(A, blockIdx.y * bs);
   colnrm = boff(colnrm, blockIdx.y * bs);
   const int c = blockIdx.x, tid = threadIdx.x;
-  if (prog != nullptr && c == 0) {
+  if (prog != nullptr) {
     prog = boff(prog, blockIdx.y * bs);
-    for (int i = tid; i < nprog; i += CN_NT) prog[i] = 0u;
+    Vs = boff(Vs, blockIdx.y * bs);
+    if (c == 0)
+      for (int i = tid; i < nprog; i += CN_NT) prog[i] = 0u;
+    for (int r = tid; r < m; r += CN_NT) Vs[(int64_t)c * m + r] = make_double2(0.0, 0.0);
   }
   if (!norms) return;
   __shared__ double sh[CN_NT / 64];
@@ -347,10 +353,10 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // zlarft) from the Gram entries G[j][l] = v_j^H v_l; otherwise the apply
 // kernel works from the Gram entries itself and T is formed after the
 // factorization from the Gram GEMM.
-// Exchange between the chain kernel's workgroups: raw-buffer loads and
-// stores with sc1 set (device scope), the accesses agent-scope atomic loads
-// and stores compile to: such a store is written through to where the other
-// workgroups' sc1 loads read it, and it is complete (vmcnt) once it is there.
+// Exchange between the chain kernel's workgroups, which all run on one XCD:
+// plain stores (the L1 writes through; a store is complete once that XCD's
+// L2 holds it) and raw-buffer loads with sc1 set (device scope), which miss
+// the reading CU's L1 and read the L2 the stores went to.
 constexpr int AUX_SC1 = 16;                 // cache-policy operand: sc1
 constexpr uint32_t OFF_NONE = 0xFFFFFFF0u;  // past every buffer: the hardware drops the access
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
@@ -366,18 +372,14 @@ __device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, uint32_t byte_o
   const v4i v = {__double2loint(x.x), __double2hiint(x.x), __double2loint(x.y), __double2hiint(x.y)};
   __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)byte_off, 0, AUX);
 }
-// where the chain kernel's panel_factor writes: R entries to A (plain), the
-// reflectors to Vs and tau (sc1); byte offsets of this thread's slot-0 row of
+// where the chain kernel's panel_factor writes: R entries to A, the
+// reflectors to Vs and tau; byte offsets of this thread's slot-0 row of
 // column c0 and the steps per column and per row slot
 struct ChainIO {
   __amdgpu_buffer_rsrc_t rA, rV, rT;
   uint32_t abase, acol, vbase, vcol, vslot;
   unsigned* prog;
 };
-// Every column a chain workgroup finishes costs each thread exactly
-// CHAIN_OPS stores (R entry, R reflector slots, tau; a store the thread does
-// not make goes to OFF_NONE), so vmcnt(CHAIN_OPS) means "all but the latest
-// column's stores are complete".
 template <int N>
 __device__ __forceinline__ void wait_vm() {
   static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
@@ -404,8 +406,8 @@ struct PanelShape {
 // CHAIN: the column stores go through io (see ChainIO), and the reflectors
 // are published through io->prog, which holds the number c of complete
 // reflectors in bits 0-7 and, in bits 8-15, which of them are identities
-// (tau = 0).  A column's stores are issued after its step's barrier; at each
-// later barrier every column but the latest one is complete and published.
+// (tau = 0).  A column's stores are issued after its step's barrier and
+// published, once complete, at the next barrier.
 template <int R, int NB, int NT, bool CHAIN>
 __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, const int M, const int m,
                                              const int64_t cs, const int c0, const int nbp, const double tol,
@@ -430,16 +432,21 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
   int ph = 0;
   unsigned idmask = 0;   // CHAIN: identity reflectors of this panel so far (bit k)
   int stored = 0;        // CHAIN: columns whose stores have been issued
-  // CHAIN: column k's stores, R + 2 per thread (R entry rows <= k, reflector
-  // slots, tau from thread 0)
-  auto chain_store = [&](int k, double2 rval, const double2 (&vv)[R], double2 tval) {
+  // CHAIN: column k's stores: R entries (rows <= k) to A, tau from thread 0,
+  // and the reflector to Vs unless it is an identity (Vs was zeroed before
+  // the launch; an identity's column of V never reaches Q, as its row of W
+  // is zero)
+  auto chain_store = [&](int k, double2 rval, const double2* vv, double2 tval) {
     if constexpr (CHAIN) {
       st_buf<0>(io->rA, tk <= k ? io->abase + (uint32_t)k * io->acol : OFF_NONE, rval);
+      if (vv != nullptr) {
 #pragma unroll
-      for (int i = 0; i < R; ++i)
-        st_buf<AUX_SC1>(io->rV, tid + NT * i < M ? io->vbase + (uint32_t)k * io->vcol + (uint32_t)i * io->vslot : OFF_NONE,
-                        vv[i]);
-      st_buf<AUX_SC1>(io->rT, tid == 0 ? (uint32_t)(c0 + k) * 16u : OFF_NONE, tval);
+        for (int i = 0; i < R; ++i)
+          st_buf<0>(io->rV,
+                    tid + NT * i < M ? io->vbase + (uint32_t)k * io->vcol + (uint32_t)i * io->vslot : OFF_NONE,
+                    vv[i]);
+      }
+      st_buf<0>(io->rT, tid == 0 ? (uint32_t)(c0 + k) * 16u : OFF_NONE, tval);
       stored = k + 1;
     }
   };
@@ -496,6 +503,21 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
   }
   const double thr = tol > 0.0 ? tol * sqrt(tsum) : 0.0;
   const double thr2 = thr * thr;
+  if constexpr (CHAIN && NB == 8) {
+    // a panel of identity reflectors only (each column's zlarfg test on the
+    // up-front values, the same values its step uses): nothing of it is read
+    // by the other workgroups, so it is published at once
+    const bool idl = (thr > 0.0) ? (up_xs <= thr2 && fabs(up_ai) <= thr) : (up_xs == 0.0 && up_ai == 0.0);
+    const uint64_t ib = __ballot((lane & 7) == 0 && (lane >> 3) < nbp && idl);
+    const uint64_t all = [&] {
+      uint64_t b = 0;
+      for (int k = 0; k < nbp; ++k) b |= 1ull << (8 * k);
+      return b;
+    }();
+    if (ib == all && tid == 0)
+      __hip_atomic_store(io->prog, (unsigned)nbp | (((1u << nbp) - 1u) << 8), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   // One column step: column k sits in slot O of a[.][], the columns trailing
   // it in slots O+1 .. W-1 (zero past the panel) and are updated in place.
   // Steps go in pairs (O = 0, then 1) and the panel then shifts left by two
@@ -544,10 +566,7 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
       // column k out unchanged (R part to A; Vs: 1 at row k, zero elsewhere);
       // the trailing columns stay as they are
       if constexpr (CHAIN) {
-        double2 e[R];
-#pragma unroll
-        for (int i = 0; i < R; ++i) e[i] = make_double2((i == 0 && tk == k) ? 1.0 : 0.0, 0.0);
-        chain_store(k, a[0][O], e, make_double2(0.0, 0.0));
+        chain_store(k, a[0][O], nullptr, make_double2(0.0, 0.0));
       } else {
         if (tid == 0) tau[c0 + k] = make_double2(0.0, 0.0);
         if (tk <= k) Ak[(int64_t)k * cs] = a[0][O];
@@ -635,14 +654,14 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
         v8[7] = at ? vm[0].y : 0.0;
         red[ph][(wave * GMAX + GW) * 64 + lane] = wave_rs8(v8);
       }
-      // CHAIN: every column stored before the latest one is complete in
+      // CHAIN: every column stored so far (issued a step ago) is complete in
       // every thread past the barrier: publish them, then store column k
-      if constexpr (CHAIN) wait_vm<R + 2>();
+      if constexpr (CHAIN) wait_vm<0>();
       __syncthreads();
       if constexpr (CHAIN) {
-        if (tid == 0 && stored > 1)
-          __hip_atomic_store(io->prog, (unsigned)(stored - 1) | (idmask << 8), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0 && stored > 0)
+          __hip_atomic_store(io->prog, (unsigned)stored | (idmask << 8), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
         chain_store(k, (tk == k) ? make_double2(beta, 0.0) : a[0][O], vm, t);
       }
       double sg[GW];
@@ -712,7 +731,7 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
     wait_vm<0>();
     __syncthreads();
     if (tid == 0)
-      __hip_atomic_store(io->prog, (unsigned)nbp | (idmask << 8), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(io->prog, (unsigned)nbp | (idmask << 8), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
@@ -914,21 +933,24 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restric
 // produced, so no panel waits for a separate trailing-update launch, and the
 // factorization is one dispatch instead of a panel and an update launch per
 // 8 columns.
-// The published reflectors travel through device-scope (sc1) stores and
-// loads (ChainIO), which reach across the XCDs' separate L2 caches.
-// prog[0 .. P) are zero at launch (qr_colnorm_kernel clears them); bits 0-7
+// One XCD: the grid has 8 workgroups per panel, dealt to the 8 XCDs round
+// robin (qr_chain_probe checks that once per device), and only those on XCD 0
+// (read from the XCC_ID hardware register) work, so every exchange goes
+// through that XCD's L2.  They take panel tickets from prog[P] in arrival
+// order (an atomic in that same L2); the others exit at once.
+// prog[0 .. P+1) are zero at launch (qr_colnorm_kernel clears them); bits 0-7
 // of prog[k] count the complete reflectors of panel k, bits 8-15 mark which
 // of them are identities (those are skipped without loading them), and
-// PROG_ABORT says that panel's owner gave up.  Workgroup p is panel p's
-// owner and waits only on panels of lower index, owned by workgroups of lower
-// index, which dispatch (in order of workgroup index) has placed on the GPU
-// before it.  Every wait
+// PROG_ABORT says that panel's owner gave up.  A workgroup waits only on
+// panels with lower tickets, taken by workgroups already running.  Every wait
 // is bounded by CHAIN_WAIT_TICKS of s_memrealtime (100 MHz): past it the
 // workgroup marks its own panel aborted and stores NaN as its tau, so a
 // failure reaches Q as NaN instead of hanging the queue.
 constexpr unsigned PROG_ABORT = 0x80000000u;
 constexpr uint64_t CHAIN_WAIT_TICKS = 100000000ull;   // 1 s
 constexpr int CH_NB = 8;
+constexpr int CH_WG_PER_PANEL = 8;   // one per XCD
+__device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg(6164) & 15u; }   // hwreg(HW_REG_XCC_ID, 0, 4)
 
 template <int R, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void qr_chain_kernel(
@@ -952,8 +974,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   __shared__ double red[2][PanelShape<NB, NT>::RED];
   __shared__ double2 s_tau[NB];
   __shared__ unsigned s_poll[2];
+  __shared__ int s_ticket;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int p = blockIdx.x, c0 = NB * p, nbp = min(NB, n - c0);
+  const int P = (n + NB - 1) / NB;
+  if (tid == 0) {
+    int t = P;
+    if (xcc_id() == 0u) t = (int)__hip_atomic_fetch_add(prog + P, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_ticket = t;
+  }
+  __syncthreads();
+  const int p = s_ticket;
+  if (p >= P) return;   // another XCD, or every panel taken
+  const int c0 = NB * p, nbp = min(NB, n - c0);
   ChainIO io;
   io.rA = buf_rsrc(A, (uint32_t)(((int64_t)(m - 1) * rs + (int64_t)(n - 1) * cs + 1) * sizeof(double2)));
   io.rV = buf_rsrc(Vs, (uint32_t)((size_t)m * n * sizeof(double2)));
@@ -1080,7 +1112,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     }
   }
   if (fail) {
-    if (tid == 0) __hip_atomic_store(prog + p, PROG_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(prog + p, PROG_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (tid < nbp) tau[c0 + tid] = make_double2(__builtin_nan(""), __builtin_nan(""));
     return;
   }
@@ -1608,10 +1640,10 @@ static void qr_chain(double2* A, int m, int n, int64_t rs, int64_t cs, double to
   const unsigned nb = (unsigned)bat.nb;
   const int64_t bs = bat.bs;
   const int P = (n + CH_NB - 1) / CH_NB;
-  MPS_LAUNCH(qr_colnorm_kernel, dim3(tol > 0.0 ? (unsigned)n : 1u, nb), dim3(CN_NT), 0, s, A, m, rs, cs, q.colnrm,
-             q.prog, P, tol > 0.0, bs);
-  MPS_LAUNCH((qr_chain_kernel<R, NT>), dim3((unsigned)P, nb), dim3(NT), 0, s, A, m, n, rs, cs, tol, q.colnrm, q.Vs,
-             q.tau, q.prog, bs);
+  MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, q.colnrm, q.prog, P + 1,
+             q.Vs, tol > 0.0, bs);
+  MPS_LAUNCH((qr_chain_kernel<R, NT>), dim3((unsigned)(CH_WG_PER_PANEL * P), nb), dim3(NT), 0, s, A, m, n, rs, cs,
+             tol, q.colnrm, q.Vs, q.tau, q.prog, bs);
   qr_form_q(A, m, n, rs, cs, q, R_out, s, bat);
 }
 
@@ -1624,8 +1656,37 @@ static bool chain_enabled() {
   return on;
 }
 
+// The chain kernel needs workgroup x of a launch on XCD x % 8 (so its grid of
+// 8 workgroups per panel puts one per panel on XCD 0).  qr_chain_probe runs
+// once per device, from context creation (never under graph capture), and
+// records whether 64 workgroups land that way; the chain path is used only
+// where they did.
+constexpr int PROBE_WG = 64;
+__global__ void xcc_probe_kernel(unsigned* out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
+}
+static int g_chain_ok[64] = {};   // per device: 0 not probed, 1 round robin from XCD 0, -1 not
+static std::mutex g_chain_mu;
+void qr_chain_probe(int device) {
+  if (device < 0 || device >= 64) return;
+  std::lock_guard<std::mutex> lk(g_chain_mu);
+  if (g_chain_ok[device] != 0) return;
+  unsigned* d = nullptr;
+  unsigned h[PROBE_WG];
+  HIPCHK(hipMalloc((void**)&d, PROBE_WG * sizeof(unsigned)));
+  MPS_LAUNCH(xcc_probe_kernel, dim3(PROBE_WG), dim3(64), 0, (hipStream_t)0, d);
+  const hipError_t e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+  HIPCHK(hipFree(d));
+  HIPCHK(e);
+  bool ok = true;
+  for (int i = 0; i < PROBE_WG; ++i) ok = ok && h[i] == (unsigned)(i % CH_WG_PER_PANEL);
+  g_chain_ok[device] = ok ? 1 : -1;
+}
 static bool chain_fits(int64_t n, int nt, const Bat& bat) {
-  return chain_enabled() && bat.nb == 1 && n <= nt && n <= FW_NMAX;
+  if (!chain_enabled() || bat.nb != 1 || n > nt || n > FW_NMAX) return false;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
+  return g_chain_ok[dev] == 1;
 }
 
 // PT: the panel kernel forms each panel's T (16-column panels, where the
@@ -1642,7 +1703,7 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
   const int64_t bs = bat.bs;
   if (tol > 0.0)
     MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, colnrm,
-               (unsigned*)nullptr, 0, true, bs);
+               (unsigned*)nullptr, 0, (double2*)nullptr, true, bs);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
@@ -1680,7 +1741,7 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
   L.vs = o; o = al256(o + (size_t)m * n * sizeof(double2));
   L.tau = o; o = al256(o + (size_t)n * sizeof(double2));
   L.cn = o; o = al256(o + (size_t)n * sizeof(double));
-  L.prog = o; o = al256(o + (size_t)((n + CH_NB - 1) / CH_NB) * sizeof(unsigned));
+  L.prog = o; o = al256(o + (size_t)((n + CH_NB - 1) / CH_NB + 1) * sizeof(unsigned));
   L.tall = o; o = al256(o + (size_t)(n + 16) * 16 * sizeof(double2));
   L.g = o; o = al256(o + (size_t)n * n * sizeof(double2));
   L.w = o; o = al256(o + (size_t)n * n * sizeof(double2));
