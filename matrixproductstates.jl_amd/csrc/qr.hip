@@ -1545,7 +1545,9 @@ __global__ __launch_bounds__(256) void qr_wsolve_kernel(const double2* __restric
       if (sl == S) {
 #pragma unroll
         for (int cc = CH - 1; cc >= 0; --cc) {
-          if (q * CH + cc < n) {
+          // an identity reflector (tau = 0) has w_i = 0 and changes no other
+          // row: its step is skipped (rank-deficient panels skip most steps)
+          if (q * CH + cc < n && (tv[cc].x != 0.0 || tv[cc].y != 0.0)) {
             const int owner = ob + cc;
             const double2 wl = cmul(tv[cc], acc[S]);
             const int lo_r = __builtin_amdgcn_readlane(__double2loint(wl.x), owner);
@@ -1577,7 +1579,10 @@ __global__ __launch_bounds__(256) void qr_wsolve_kernel(const double2* __restric
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int r = lane + 64 * s;
-      if (r < n) W[(int64_t)c * n + r] = acc[s];
+      if (r < n) {
+        const double2 tr = sTau[r];   // rows of skipped steps: w_r = 0
+        W[(int64_t)c * n + r] = (tr.x != 0.0 || tr.y != 0.0) ? acc[s] : make_double2(0.0, 0.0);
+      }
     }
   }
 }
