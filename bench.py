@@ -519,6 +519,36 @@ def three_forms_leg(seed, n=240, rate=50.0, budget=96 << 30, sample_per_kind=4):
     return out
 
 
+def qr_panels_leg(seed, m=1152, n=128, reps=60, ranks=(128, 60, 20), tol=1e-13):
+    """leftorth! (orthonormalisation.jl:164-171) on one site whose panel is the
+    headline workload's bulk shape (D*d x D = 1152 x 128), full rank and of
+    rank r plus rounding-level noise under the 1e-13 cutoff: the QR time that
+    the RK4 step pays about 400 times.  Each factorization is timed from the
+    host around the call and a synchronize (launch and sync included)."""
+    import mpsamd as M
+    ctx = M.Context(0)
+    rng = np.random.default_rng(seed)
+    st = M.MPS(1, [1], [m, n], ctx)
+    out = {"shape": [m, n], "reps": reps, "chain_path": os.environ.get("MPS_QR_CHAIN", "1") != "0"}
+    for r in ranks:
+        a = (rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))) @ \
+            (rng.standard_normal((r, n)) + 1j * rng.standard_normal((r, n)))
+        if r < n:
+            a = a + 1e-17 * (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n)))
+        ctx.set_qr_rank_tol(tol if r < n else 0.0)
+        ts = []
+        for i in range(reps):
+            st.upload([a.reshape(m, 1, n, order="F")])
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            M.leftorth_(st, 0)
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        out[f"rank{r}_us"] = round(float(np.median(ts[5:])) * 1e6, 1)
+    ctx.set_qr_rank_tol(0.0)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -544,6 +574,9 @@ def main():
     ap.add_argument("--timing-pass-only", action="store_true",
                     help="run only the event-timed single-stream pass (for a kernel trace whose per-launch "
                          "durations compare with the roofline figures); value is that pass's rate")
+    ap.add_argument("--qr-panels", action="store_true",
+                    help="only time leftorth! on 1152 x 128 panels of full and low rank (MPS_QR_CHAIN=0 in the "
+                         "environment: the launch-per-panel factorization)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -558,6 +591,9 @@ def main():
 
     if args.gemm_counters:
         print(json.dumps({"gemm_counters": gemm_counters(args.gemm_counters)}))
+        return
+    if args.qr_panels:
+        print(json.dumps({"qr_panels": qr_panels_leg(args.seed)}))
         return
     import mpsamd as M
     if args.three_forms_only:
