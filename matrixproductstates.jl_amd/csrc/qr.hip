@@ -527,7 +527,7 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
     constexpr int W = decltype(Wc)::value;
     constexpr int O = decltype(Oc)::value;
     constexpr int T = W - 1 - O;                 // trailing slots
-    static_assert(T >= 1, "a step needs a trailing slot");
+    static_assert(T >= 0, "slot O must be within the register panel");
     constexpr int GW = (2 * T + 7) / 8;
     if (k >= NB) __builtin_unreachable();
     // zlarfg on column k (= a[.][O]): ||x||^2 below row k and alpha = a(k, k)
@@ -612,6 +612,12 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
         for (int i = 0; i < R; ++i)
           if (tid + NT * i < M) Vk[(int64_t)k * m + NT * i] = vm[i];
       }
+      if constexpr (T == 0) {
+        // the panel's last column: nothing trails it, so no dots, no update
+        // and no barrier (CHAIN: the end of the panel publishes it)
+        if constexpr (CHAIN) chain_store(k, (tk == k) ? make_double2(beta, 0.0) : a[0][O], vm, t);
+        return;
+      }
       // s_j = v_k^H a_j for the T trailing slots, four columns per butterfly
 #pragma unroll
       for (int g = 0; g < GW; ++g) {
@@ -635,7 +641,7 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
       }
       // look-ahead sums for column k+1 (a[.][O+1] before its update), rows > k+1:
       // ||a||^2, v_k^H a, ||v_k||^2, and a(k+1), v_k(k+1) from their owner
-      {
+      if constexpr (T >= 1) {
         double v8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < R; ++i) {
@@ -664,7 +670,7 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
                              __HIP_MEMORY_SCOPE_WORKGROUP);
         chain_store(k, (tk == k) ? make_double2(beta, 0.0) : a[0][O], vm, t);
       }
-      double sg[GW];
+      double sg[GW > 0 ? GW : 1];   // (GW = 0 only in the last-column step, which returned above)
 #pragma unroll
       for (int g = 0; g < GW; ++g) sg[g] = wave_sum_slots(red[ph], g);
       const double la = wave_sum_slots(red[ph], GW);
@@ -723,8 +729,11 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
     segment(std::integral_constant<int, 4>{}, 12, nbp);
   } else {
     static_assert(NB == 8, "panel width 8 or 16");
+    // columns 6 and 7 in a 2-slot panel: column 6's step works on column 7
+    // alone, and column 7's step has nothing trailing (no reduction, no barrier)
     segment(std::integral_constant<int, 8>{}, 0, min(nbp, 4));
-    segment(std::integral_constant<int, 4>{}, 4, nbp);
+    segment(std::integral_constant<int, 4>{}, 4, min(nbp, 6));
+    segment(std::integral_constant<int, 2>{}, 6, nbp);
   }
   if constexpr (CHAIN) {
     // the rest, once complete
@@ -949,6 +958,7 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restric
 constexpr unsigned PROG_ABORT = 0x80000000u;
 constexpr uint64_t CHAIN_WAIT_TICKS = 100000000ull;   // 1 s
 constexpr int CH_NB = 8;
+constexpr int CH_SLOTS = 5;   // reflectors staged per batch (5 x 20 KiB of LDS at m <= 1280)
 constexpr int CH_WG_PER_PANEL = 8;   // one per XCD
 __device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg(6164) & 15u; }   // hwreg(HW_REG_XCC_ID, 0, 4)
 
@@ -975,6 +985,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   __shared__ double2 s_tau[NB];
   __shared__ unsigned s_poll[2];
   __shared__ int s_ticket;
+  // published reflectors staged in LDS (rows t + NT i, as the columns) with
+  // their tau (one copy per wave), CH_SLOTS of them per batch
+  __shared__ double2 ring[CH_SLOTS][R * NT];
+  __shared__ double2 ring_tau[CH_SLOTS][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int P = (n + NB - 1) / NB;
   if (tid == 0) {
@@ -1015,13 +1029,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), srclane);
     return __hiloint2double(hi, lo);
   };
-  auto load_refl = [&](int c, double2 (&v)[R], double2& t) {
+  // reflector c into ring slot sl: each wave's rows straight into LDS (16-byte
+  // device-scope buffer loads with the LDS flag; rows past m read as zero),
+  // and tau by the wave's lane 0 into its own copy
+  auto stage_refl = [&](int c, int sl) {
+    typedef __attribute__((address_space(3))) void* lds_ptr;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const int rr = tid + NT * i;
-      v[i] = ld_dev(io.rV, rr < m ? (uint32_t)(((size_t)c * m + rr) * sizeof(double2)) : OFF_NONE);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(io.rV, (lds_ptr)&ring[sl][wave * 64 + NT * i], 16,
+                                               rr < m ? (int)(((size_t)c * m + rr) * sizeof(double2)) : (int)OFF_NONE,
+                                               0, 0, AUX_SC1);
     }
-    t = ld_dev(io.rT, (uint32_t)(c * sizeof(double2)));
+    if (lane == 0)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(io.rT, (lds_ptr)&ring_tau[sl][wave], 16, (int)(c * sizeof(double2)), 0, 0,
+                                               AUX_SC1);
   };
   int ph = 0, pp = 0;
   bool fail = false;
@@ -1082,33 +1104,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         fail = true;
         break;
       }
-      // the non-identity reflectors among done .. avail-1, applied in order
-      // through two register buffers: the next one's loads are issued as
-      // soon as a buffer is free, so they overlap the other buffer's apply.
-      // A dynamic loop (two copies of the apply, not eight) keeps the code
-      // within the instruction cache; more buffers spill (the 8 columns
-      // already take 160 registers).
+      // the non-identity reflectors among done .. avail-1 (at most CH_SLOTS
+      // per batch): all staged at once, one wait, then applied in order from
+      // LDS.  (Staging the next batch while this one is applied measured no
+      // faster: when the owner publishes one reflector at a time, each still
+      // costs its discovery, load and apply in turn.)
       unsigned rem = ((1u << avail) - (1u << done)) & ~(pv >> 8);
-      auto pop = [&]() {
-        if (rem == 0u) return -1;
+      int ns = 0;
+      unsigned upto = avail;
+      while (rem != 0u && ns < CH_SLOTS) {
         const int q = __builtin_ctz(rem);
         rem &= rem - 1u;
-        return q;
-      };
-      double2 vb0[R], vb1[R], tb0 = make_double2(0.0, 0.0), tb1 = make_double2(0.0, 0.0);
-      int q0 = pop(), q1 = pop();
-      if (q0 >= 0) load_refl(NB * k + q0, vb0, tb0);
-      if (q1 >= 0) load_refl(NB * k + q1, vb1, tb1);
-      while (q0 >= 0) {
-        apply(vb0, tb0);
-        q0 = pop();
-        if (q0 >= 0) load_refl(NB * k + q0, vb0, tb0);
-        if (q1 < 0) break;
-        apply(vb1, tb1);
-        q1 = pop();
-        if (q1 >= 0) load_refl(NB * k + q1, vb1, tb1);
+        stage_refl(NB * k + q, ns++);
       }
-      done = avail;
+      if (rem != 0u) upto = (unsigned)__builtin_ctz(rem);   // the rest in the next batch
+      if (ns > 0) {
+        wait_vm<0>();
+        __syncthreads();
+        for (int sl = 0; sl < ns; ++sl) {
+          double2 v[R];
+#pragma unroll
+          for (int i = 0; i < R; ++i) v[i] = ring[sl][tid + NT * i];
+          apply(v, ring_tau[sl][wave]);
+        }
+      }
+      done = upto;
     }
   }
   if (fail) {
