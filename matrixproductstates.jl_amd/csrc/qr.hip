@@ -958,7 +958,10 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restric
 constexpr unsigned PROG_ABORT = 0x80000000u;
 constexpr uint64_t CHAIN_WAIT_TICKS = 100000000ull;   // 1 s
 constexpr int CH_NB = 8;
-constexpr int CH_SLOTS = 5;   // reflectors staged per batch (5 x 20 KiB of LDS at m <= 1280)
+// reflectors staged per batch: 5 x 20 KiB of LDS at m <= 1280 (256 threads),
+// 2 x 40 KiB at m <= 2560 (512 threads)
+template <int R, int NT>
+constexpr int chain_slots() { return (R * NT * 16) <= 20480 ? 5 : 2; }
 constexpr int CH_WG_PER_PANEL = 8;   // one per XCD
 __device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg(6164) & 15u; }   // hwreg(HW_REG_XCC_ID, 0, 4)
 
@@ -987,6 +990,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   __shared__ int s_ticket;
   // published reflectors staged in LDS (rows t + NT i, as the columns) with
   // their tau (one copy per wave), CH_SLOTS of them per batch
+  constexpr int CH_SLOTS = chain_slots<R, NT>();
   __shared__ double2 ring[CH_SLOTS][R * NT];
   __shared__ double2 ring_tau[CH_SLOTS][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1823,6 +1827,8 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
     qr_chain<5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 1280)
     qr_blocked<5, 8, 256, 20, 5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
+  else if (m <= 2560 && chain_fits(n, 512, bat))
+    qr_chain<5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 2560)
     qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else
