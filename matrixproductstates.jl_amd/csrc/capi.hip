@@ -375,6 +375,13 @@ int mps_ctx_set_qr_rank_tol(mps_ctx_t ctx, double tol) {
   });
 }
 
+int mps_ctx_set_qr_single_launch(mps_ctx_t ctx, int on) {
+  return guard([&] {
+    REQUIRE(on >= -1 && on <= 1, "qr form: 1 single launch, 0 launch per panel, -1 process default");
+    ctx->c.qr_chain = on;
+  });
+}
+
 int mps_ctx_set_precision(mps_ctx_t ctx, int prec) {
   return guard([&] {
     REQUIRE(prec == MPS_PREC_FP64 || prec == MPS_PREC_FP32, "unknown precision");
@@ -982,7 +989,7 @@ int mps_dev_qrq(mps_ctx_t ctx, void* dev, int64_t m, int64_t n, int64_t rs, int6
     Scratch qs(&ctx->c);
     (void)qs;
     REQUIRE(qr_workspace_bytes(m, n, ctx->c.num_cus) <= p.scr.used, "qr workspace");
-    launch_qr(dev, m, n, rs, cs, ctx->c.qr_tol, p.tau, ctx->c.num_cus, R_out, ctx->c.stream);
+    launch_qr(dev, m, n, rs, cs, ctx->c.qr_tol, p.tau, ctx->c.num_cus, R_out, ctx->c.stream, Bat(), ctx->c.qr_chain);
     ctx->c.sync();
   });
 }

@@ -117,8 +117,10 @@ void launch_gemm(const Gemm& g, Compute cmp, void* ws, size_t ws_bytes, int num_
 // Element (r, c) at base + r*rs + c*cs; m >= n.  ws: qr_workspace_bytes(m, n,
 // num_cus).  R_out (optional): n x n column-major upper triangle.
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus);
+// chain: 1 single-launch factorization where it applies (m <= 2560, n <= 256,
+// one copy), 0 launch per panel, -1 the process default (MPS_QR_CHAIN)
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
-               void* R_out, hipStream_t s, Bat bat = Bat());
+               void* R_out, hipStream_t s, Bat bat = Bat(), int chain = -1);
 // once per device, outside graph capture: checks the workgroup-to-XCD layout
 // the single-launch (chain) QR relies on; launch_qr uses that path only after it passed
 void qr_chain_probe(int device);

@@ -13,6 +13,7 @@ struct Ctx {
   bool own_stream = false;
   int num_cus = 256;
   double qr_tol = 0.0;
+  int qr_chain = -1;     // QR form: 1 single launch where it applies, 0 launch per panel, -1 process default
   int prec = 0;          // 0 FP64, 1 FP32 operator terms
   int refine = 0;        // FP64 half-sweeps refitting the last RK4 stage (even)
   bool graphs = true;

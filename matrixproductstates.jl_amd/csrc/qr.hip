@@ -1711,8 +1711,8 @@ void qr_chain_probe(int device) {
   for (int i = 0; i < PROBE_WG; ++i) ok = ok && h[i] == (unsigned)(i % CH_WG_PER_PANEL);
   g_chain_ok[device] = ok ? 1 : -1;
 }
-static bool chain_fits(int64_t n, int nt, const Bat& bat) {
-  if (!chain_enabled() || bat.nb != 1 || n > nt || n > FW_NMAX) return false;
+static bool chain_fits(int64_t n, int nt, const Bat& bat, int chain) {
+  if (!(chain < 0 ? chain_enabled() : chain != 0) || bat.nb != 1 || n > nt || n > FW_NMAX) return false;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
   return g_chain_ok[dev] == 1;
@@ -1790,7 +1790,7 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus) { return qr_layout(m, n, num_cus).total; }
 
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
-               void* R_out, hipStream_t s, Bat bat) {
+               void* R_out, hipStream_t s, Bat bat, int chain) {
   REQUIRE(m >= n && n >= 1, "qrq!: requires m >= n >= 1");
   REQUIRE(bat.nb >= 1 && bat.nb <= 65535, "qrq!: too many lockstep copies");
   REQUIRE(m < (1LL << 31) && m * n < (1LL << 40), "qrq!: panel too large");
@@ -1823,11 +1823,11 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
     qr_blocked<2, 8, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 768)
     qr_blocked<3, 8, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
-  else if (m <= 1280 && chain_fits(n, 256, bat))
+  else if (m <= 1280 && chain_fits(n, 256, bat, chain))
     qr_chain<5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 1280)
     qr_blocked<5, 8, 256, 20, 5, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
-  else if (m <= 2560 && chain_fits(n, 512, bat))
+  else if (m <= 2560 && chain_fits(n, 512, bat, chain))
     qr_chain<5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 2560)
     qr_blocked<5, 8, 512, 40, 5, 512>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);

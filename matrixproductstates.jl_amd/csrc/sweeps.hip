@@ -111,7 +111,7 @@ static void run_qr(Plan& p, void* base, i64 m, i64 n, i64 rs, i64 cs, void* Rout
   const bool prof = c->profiling && !c->capturing;
   hipEvent_t a = nullptr;
   if (prof) c->prof_begin(a);
-  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, c->num_cus, Rout, c->stream, c->bat);
+  launch_qr(base, m, n, rs, cs, c->qr_tol, p.tau, c->num_cus, Rout, c->stream, c->bat, c->qr_chain);
   // zgeqrf + zungqr: 2 x 8 (m n^2 - n^3 / 3) flops; panel read and written once
   if (prof) c->prof_end(a, 1, 16.0 * ((double)m * n * n - (double)n * n * n / 3.0), 32.0 * (double)m * n);
 }

@@ -41,6 +41,7 @@ SIGNATURES = {
     "mps_ctx_set_stream": [vp, vp],
     "mps_ctx_synchronize": [vp],
     "mps_ctx_set_qr_rank_tol": [vp, dbl],
+    "mps_ctx_set_qr_single_launch": [vp, i32],
     "mps_ctx_set_precision": [vp, i32],
     "mps_ctx_set_graphs": [vp, i32],
     "mps_ctx_set_refine_sweeps": [vp, i32],

@@ -86,6 +86,11 @@ class Context:
     def set_qr_rank_tol(self, tol: float):
         call("mps_ctx_set_qr_rank_tol", self.h, float(tol))
 
+    def set_qr_single_launch(self, on):
+        """QR form: True one launch per factorization where it applies, False
+        a launch per panel, None the process default."""
+        call("mps_ctx_set_qr_single_launch", self.h, -1 if on is None else int(bool(on)))
+
     def set_precision(self, prec: str):
         call("mps_ctx_set_precision", self.h, {"fp64": MPS_PREC_FP64, "fp32": MPS_PREC_FP32}[prec])
 

@@ -95,6 +95,35 @@ def test_qrq_full_rank(dev, mn, view, rng):
     assert np.abs(Qd @ R - M).max() < 1e-12 * np.abs(M).max() * n
 
 
+@pytest.mark.parametrize("mnr", [(1152, 128, 128), (1000, 50, 50), (1152, 128, 20), (2304, 256, 256),
+                                 (2304, 256, 30), (1280, 256, 256)])
+def test_qrq_single_launch_matches_per_panel(dev, mnr, rng):
+    # the single-launch (chain) factorization and the launch-per-panel one
+    # build the same zgeqr2 reflectors: Q and R agree to rounding, with and
+    # without the rank cutoff (rank-r panels plus rounding-level noise)
+    m, n, r = mnr
+    print("case", mnr, flush=True)
+    M = crandn(rng, (m, r)) @ crandn(rng, (r, n))
+    if r < n:
+        M = M + 1e-17 * crandn(rng, (m, n))
+    from mpsamd import default_context
+    ctx = default_context()
+    tol = 1e-13 if r < n else 0.0
+    ctx.set_qr_rank_tol(tol)
+    try:
+        ctx.set_qr_single_launch(True)
+        Qc, Rc = dev.qrq(M, want_r=True)
+        ctx.set_qr_single_launch(False)
+        Qp, Rp = dev.qrq(M, want_r=True)
+    finally:
+        ctx.set_qr_single_launch(None)
+        ctx.set_qr_rank_tol(0.0)
+    scale = np.abs(M).max() * n
+    assert np.abs(Qc - Qp).max() < 1e-12
+    assert np.abs(Rc - Rp).max() < 1e-12 * scale
+    assert np.abs(Qc.conj().T @ Qc - np.eye(n)).max() < 1e-12
+
+
 def test_qrq_zero_padded_closed_form(dev):
     # zero-padded product-state panel: LAPACK gives tau = 0 for the zero
     # columns; the completion is fixed by the one non-trivial reflector.
