@@ -124,6 +124,8 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
 // once per device, outside graph capture: checks the workgroup-to-XCD layout
 // the single-launch (chain) QR relies on; launch_qr uses that path only after it passed
 void qr_chain_probe(int device);
+// whether launch_qr factors an m x n panel in one launch (chain as for launch_qr)
+bool qr_single_launch_applies(int64_t m, int64_t n, const Bat& bat, int chain);
 
 // ------------------------------------------------------- device helpers
 // DPP move of a double (two 32-bit halves), bound_ctrl: out-of-row sources read 0

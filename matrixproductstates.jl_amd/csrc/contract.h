@@ -58,6 +58,8 @@ struct Ctx {
   // workgroup cap of the aux stream's GEMMs: they run while the main stream
   // factors a site (a QR on a few CUs at a time) and must leave CUs to it
   int aux_max_wg = 0;
+  bool aux_wg_env = false;   // MPS_AUX_MAX_WG set: its cap holds for every plan
+  int aux_cap = 0;           // the cap of the current plan's aux-stream GEMMs (0: none)
   void ensure_aux();
   double* dscal = nullptr;      // 256 doubles of device scalars
   double* hscal = nullptr;      // pinned host mirror

@@ -45,11 +45,18 @@ void Ctx::ensure_aux() {
   if (aux) return;
   REQUIRE(!capturing, "aux stream creation during graph capture");
   HIPCHK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
-  // half the CUs' worth of workgroups (measured at N = 100, D = 128, d = 9:
-  // 229.6 ms per RK4 step at 128 on 256 CUs, 241 uncapped, 238.7 at 96, 232.6
-  // at 160); MPS_AUX_MAX_WG overrides it (0: no cap)
+  // half the CUs' worth of workgroups where the sites' QR is a launch per
+  // panel, which needs free CUs on every XCD (measured at N = 100, D = 128,
+  // d = 9 with that QR: 229.6 ms per RK4 step at 128 on 256 CUs, 241
+  // uncapped, 238.7 at 96, 232.6 at 160); Plan::init lifts the cap where the
+  // sites' QR is the single launch (183.5 ms uncapped vs 200.0 at 128, driver's
+  // step count); MPS_AUX_MAX_WG overrides both (0: no cap)
   aux_max_wg = std::max(1, num_cus / 2);
-  if (const char* e = getenv("MPS_AUX_MAX_WG")) aux_max_wg = std::max(0, atoi(e));
+  if (const char* e = getenv("MPS_AUX_MAX_WG")) {
+    aux_max_wg = std::max(0, atoi(e));
+    aux_wg_env = true;
+  }
+  aux_cap = aux_max_wg;
   HIPCHK(hipEventCreateWithFlags(&ev_site, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&ev_aux, hipEventDisableTiming));
 }
@@ -152,7 +159,7 @@ static double esize(DType t) { return t == F64 ? 16.0 : 8.0; }
 void run_gemm(Ctx* c, Gemm& g, Compute cmp, void* splitk, size_t splitk_bytes, hipStream_t s) {
   g.bat = c->bat;
   g.tunable = true;
-  if (s != nullptr && s == c->aux) g.max_wg = c->aux_max_wg;
+  if (s != nullptr && s == c->aux) g.max_wg = c->aux_cap;
   if (splitk == nullptr) g.max_splits = 1;   // no partials buffer: K is not split
   if (c->gemm_rec && !c->capturing) c->rec.push_back(Ctx::RecGemm{g, cmp, splitk, splitk_bytes, s ? s : c->stream});
   if (!c->profiling || c->capturing) {

@@ -1789,6 +1789,11 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
 
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus) { return qr_layout(m, n, num_cus).total; }
 
+bool qr_single_launch_applies(int64_t m, int64_t n, const Bat& bat, int chain) {
+  return (m > 768 && m <= 1280 && chain_fits(n, 256, bat, chain)) ||
+         (m > 1280 && m <= 2560 && chain_fits(n, 512, bat, chain));
+}
+
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
                void* R_out, hipStream_t s, Bat bat, int chain) {
   REQUIRE(m >= n && n >= 1, "qrq!: requires m >= n >= 1");

@@ -81,6 +81,9 @@ void Plan::init(i64 Dm, i64 dm, i64 wm, int n_ovl, size_t extra_bytes) {
   if (conc) {
     c->ensure_side();
     c->ensure_aux();
+    // the aux stream's GEMMs run beside the sites' QRs: capped where those
+    // are a launch per panel, uncapped beside the single-launch QR
+    c->aux_cap = (!c->aux_wg_env && qr_single_launch_applies(Dm * dm, Dm, c->bat, c->qr_chain)) ? 0 : c->aux_max_wg;
   }
   plan_layout(this, Dm, dm, wm, n_ovl, extra_bytes, c->num_cus, c->bat.nb, conc,
               [&](void** slot, size_t b) { scr.add(slot, b); });
