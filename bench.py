@@ -574,6 +574,9 @@ def main():
     ap.add_argument("--timing-pass-only", action="store_true",
                     help="run only the event-timed single-stream pass (for a kernel trace whose per-launch "
                          "durations compare with the roofline figures); value is that pass's rate")
+    ap.add_argument("--timing-prec", choices=["fp64", "fp32"], default="fp64",
+                    help="precision of the event-timed profiling pass (fp32: the precision form's operator "
+                         "contractions; for a kernel trace of that form with --timing-pass-only)")
     ap.add_argument("--qr-panels", action="store_true",
                     help="only time leftorth! on 1152 x 128 panels of full and low rank (MPS_QR_CHAIN=0 in the "
                          "environment: the launch-per-panel factorization)")
@@ -717,24 +720,30 @@ def main():
     roof = None
     prof = None
     if not args.no_profile:
+        ctx.set_precision(args.timing_prec)
         ctx.set_profiling(True)
         tp = time.perf_counter()
         for i in range(args.steps):
             step()
         st = ctx.kernel_stats()
         ctx.set_profiling(False)
+        ctx.set_precision("fp64")
         if args.timing_pass_only:
             sweeps = args.steps / (time.perf_counter() - tp)
         achieved = st["gemm_flops"] / (st["gemm_ms"] * 1e-3) / 1e12
         dl = DOMINANT_LAUNCH
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK["fp64"], 4),
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK[args.timing_prec],
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK[args.timing_prec], 4),
                 "traffic": dl["hbm_read_bytes"] + dl["hbm_write_bytes"],
                 "traffic_scope": f"HBM bytes of one {dl['kernel']} launch of shape {dl['shape']} "
                                  f"(rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated); algorithmic "
                                  f"{dl['algorithmic_bytes']:.4g} B; MFMA busy {dl['mfma_busy']}",
-                "kernel": "zgemm_f64_kernel (+ split-K reduce)",
+                "kernel": ("zgemm_f64_kernel" if args.timing_prec == "fp64" else "zgemm_f32_3m_kernel")
+                          + " (+ split-K reduce)",
                 "launches": st["gemm_launches"], "avg_launch_us": round(st["gemm_ms"] * 1e3 / max(st["gemm_launches"], 1), 2)}
+        if args.timing_prec != "fp64":   # the counter figures are the FP64 launch's
+            roof["traffic"] = None
+            roof["traffic_scope"] = "not measured for this precision"
         prof = {"gemm_ms_per_step": st["gemm_ms"] / args.steps, "qr_ms_per_step": st["qr_ms"] / args.steps,
                 "qr_launches_per_step": st["qr_launches"] / args.steps,
                 "launches_checked_in_process": st["launches_checked"],
@@ -784,6 +793,7 @@ def main():
     }
     if args.timing_pass_only:
         out["mode"] = "timing pass only (single stream, every GEMM bracketed by events)"
+        out["timing_precision"] = args.timing_prec
     print(json.dumps(out))
 
 
