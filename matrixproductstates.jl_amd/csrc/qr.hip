@@ -355,22 +355,17 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // factorization from the Gram GEMM.
 // Exchange between the chain kernel's workgroups, which all run on one XCD:
 // plain stores (the L1 writes through; a store is complete once that XCD's
-// L2 holds it) and raw-buffer loads with sc1 set (device scope), which miss
-// the reading CU's L1 and read the L2 the stores went to.
+// L2 holds it) and raw-buffer loads with sc1 set (device scope, straight into
+// LDS), which miss the reading CU's L1 and read the L2 the stores went to.
 constexpr int AUX_SC1 = 16;                 // cache-policy operand: sc1
 constexpr uint32_t OFF_NONE = 0xFFFFFFF0u;  // past every buffer: the hardware drops the access
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ double2 ld_dev(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, AUX_SC1);
-  return make_double2(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
-}
-template <int AUX>
 __device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, double2 x) {
   typedef int v4i __attribute__((ext_vector_type(4)));
   const v4i v = {__double2loint(x.x), __double2hiint(x.x), __double2loint(x.y), __double2hiint(x.y)};
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)byte_off, 0, AUX);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)byte_off, 0, 0);
 }
 // where the chain kernel's panel_factor writes: R entries to A, the
 // reflectors to Vs and tau; byte offsets of this thread's slot-0 row of
@@ -438,15 +433,15 @@ __device__ __forceinline__ void panel_factor(double2 (&a)[R][NB], const int tk, 
   // is zero)
   auto chain_store = [&](int k, double2 rval, const double2* vv, double2 tval) {
     if constexpr (CHAIN) {
-      st_buf<0>(io->rA, tk <= k ? io->abase + (uint32_t)k * io->acol : OFF_NONE, rval);
+      st_buf(io->rA, tk <= k ? io->abase + (uint32_t)k * io->acol : OFF_NONE, rval);
       if (vv != nullptr) {
 #pragma unroll
         for (int i = 0; i < R; ++i)
-          st_buf<0>(io->rV,
+          st_buf(io->rV,
                     tid + NT * i < M ? io->vbase + (uint32_t)k * io->vcol + (uint32_t)i * io->vslot : OFF_NONE,
                     vv[i]);
       }
-      st_buf<0>(io->rT, tid == 0 ? (uint32_t)(c0 + k) * 16u : OFF_NONE, tval);
+      st_buf(io->rT, tid == 0 ? (uint32_t)(c0 + k) * 16u : OFF_NONE, tval);
       stored = k + 1;
     }
   };
