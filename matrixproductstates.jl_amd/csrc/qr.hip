@@ -219,15 +219,21 @@ constexpr int CN_NT = 256;
 __global__ __launch_bounds__(CN_NT) void qr_colnorm_kernel(const double2* __restrict__ A, int m, int64_t rs,
                                                            int64_t cs, double* __restrict__ colnrm,
                                                            unsigned* __restrict__ prog, int nprog,
-                                                           double2* __restrict__ Vs, bool norms, int64_t bs) {
+                                                           double2* __restrict__ Vs, double2* __restrict__ tau,
+                                                           bool norms, int64_t bs) {
   A = boff(A, blockIdx.y * bs);
   colnrm = boff(colnrm, blockIdx.y * bs);
   const int c = blockIdx.x, tid = threadIdx.x;
   if (prog != nullptr) {
+    // the chain factorization's inputs: progress words zero, V zero (its
+    // identity columns are never stored), tau NaN (every factored column
+    // stores its own, so a panel that no workgroup took shows as NaN in Q)
     prog = boff(prog, blockIdx.y * bs);
     Vs = boff(Vs, blockIdx.y * bs);
+    tau = boff(tau, blockIdx.y * bs);
     if (c == 0)
       for (int i = tid; i < nprog; i += CN_NT) prog[i] = 0u;
+    if (tid == 0) tau[c] = make_double2(__builtin_nan(""), __builtin_nan(""));
     for (int r = tid; r < m; r += CN_NT) Vs[(int64_t)c * m + r] = make_double2(0.0, 0.0);
   }
   if (!norms) return;
@@ -937,11 +943,15 @@ __global__ __launch_bounds__(64) void qr_tfactor_kernel(const double2* __restric
 // produced, so no panel waits for a separate trailing-update launch, and the
 // factorization is one dispatch instead of a panel and an update launch per
 // 8 columns.
-// One XCD: the grid has 8 workgroups per panel, dealt to the 8 XCDs round
-// robin (qr_chain_probe checks that once per device), and only those on XCD 0
-// (read from the XCC_ID hardware register) work, so every exchange goes
-// through that XCD's L2.  They take panel tickets from prog[P] in arrival
-// order (an atomic in that same L2); the others exit at once.
+// One XCD per factorization: the grid has 8 workgroups per panel, dealt to
+// the 8 XCDs round robin (qr_chain_probe checks that once per device), and
+// only those on one XCD (read from the XCC_ID hardware register) work, so
+// every exchange goes through that XCD's L2.  They take panel tickets from
+// prog[P] in arrival order (an atomic in that same L2); the others exit at
+// once.  Lockstep copies: block g = [8 P g, 8 P (g + 1)) of a one-dimensional
+// grid of ceil(nb / 8) blocks serves copies 8 g .. 8 g + 7, copy 8 g + x on
+// XCD x (round robin puts exactly P of the block's workgroups on each XCD,
+// whichever XCD the launch starts on).
 // prog[0 .. P+1) are zero at launch (qr_colnorm_kernel clears them); bits 0-7
 // of prog[k] count the complete reflectors of panel k, bits 8-15 mark which
 // of them are identities (those are skipped without loading them), and
@@ -963,14 +973,17 @@ __device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg(
 template <int R, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void qr_chain_kernel(
     double2* __restrict__ A, int m, int n, int64_t rs, int64_t cs, double tol, const double* __restrict__ colnrm,
-    double2* Vs, double2* tau, unsigned* prog, int64_t bs) {
+    double2* Vs, double2* tau, unsigned* prog, unsigned nb, int64_t bs) {
   // Vs, tau and prog are written by other workgroups while this one reads
   // them: no __restrict__ (with it the compiler takes a polled word as
   // invariant and hoists its load out of the wait loop)
   constexpr int NB = CH_NB;
   static_assert(NT >= 256, "the first row slot must cover every diagonal row (n <= NT)");
+  const int P = (n + NB - 1) / NB;
+  const unsigned copy = (blockIdx.x / (unsigned)(CH_WG_PER_PANEL * P)) * CH_WG_PER_PANEL + xcc_id();
+  if (copy >= nb) return;
   {
-    const int64_t bo = blockIdx.y * bs;   // lockstep copy
+    const int64_t bo = (int64_t)copy * bs;   // lockstep copy
     A = boff(A, bo);
     colnrm = boff(colnrm, bo);
     Vs = boff(Vs, bo);
@@ -989,15 +1002,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   __shared__ double2 ring[CH_SLOTS][R * NT];
   __shared__ double2 ring_tau[CH_SLOTS][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int P = (n + NB - 1) / NB;
-  if (tid == 0) {
-    int t = P;
-    if (xcc_id() == 0u) t = (int)__hip_atomic_fetch_add(prog + P, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_ticket = t;
-  }
+  if (tid == 0) s_ticket = (int)__hip_atomic_fetch_add(prog + P, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int p = s_ticket;
-  if (p >= P) return;   // another XCD, or every panel taken
+  if (p >= P) return;   // every panel taken (never under round robin)
   const int c0 = NB * p, nbp = min(NB, n - c0);
   ChainIO io;
   io.rA = buf_rsrc(A, (uint32_t)(((int64_t)(m - 1) * rs + (int64_t)(n - 1) * cs + 1) * sizeof(double2)));
@@ -1665,9 +1673,10 @@ static void qr_chain(double2* A, int m, int n, int64_t rs, int64_t cs, double to
   const int64_t bs = bat.bs;
   const int P = (n + CH_NB - 1) / CH_NB;
   MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, q.colnrm, q.prog, P + 1,
-             q.Vs, tol > 0.0, bs);
-  MPS_LAUNCH((qr_chain_kernel<R, NT>), dim3((unsigned)(CH_WG_PER_PANEL * P), nb), dim3(NT), 0, s, A, m, n, rs, cs,
-             tol, q.colnrm, q.Vs, q.tau, q.prog, bs);
+             q.Vs, q.tau, tol > 0.0, bs);
+  const unsigned blocks = (nb + CH_WG_PER_PANEL - 1) / CH_WG_PER_PANEL;
+  MPS_LAUNCH((qr_chain_kernel<R, NT>), dim3((unsigned)(CH_WG_PER_PANEL * P) * blocks), dim3(NT), 0, s, A, m, n, rs,
+             cs, tol, q.colnrm, q.Vs, q.tau, q.prog, nb, bs);
   qr_form_q(A, m, n, rs, cs, q, R_out, s, bat);
 }
 
@@ -1707,7 +1716,8 @@ void qr_chain_probe(int device) {
   g_chain_ok[device] = ok ? 1 : -1;
 }
 static bool chain_fits(int64_t n, int nt, const Bat& bat, int chain) {
-  if (!(chain < 0 ? chain_enabled() : chain != 0) || bat.nb != 1 || n > nt || n > FW_NMAX) return false;
+  if (!(chain < 0 ? chain_enabled() : chain != 0) || n > nt || n > FW_NMAX) return false;
+  if ((int64_t)(bat.nb + CH_WG_PER_PANEL - 1) * ((n + CH_NB - 1) / CH_NB) > 0x7FFFFFFF) return false;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
   return g_chain_ok[dev] == 1;
@@ -1727,7 +1737,7 @@ static void qr_blocked(double2* A, int m, int n, int64_t rs, int64_t cs, double 
   const int64_t bs = bat.bs;
   if (tol > 0.0)
     MPS_LAUNCH(qr_colnorm_kernel, dim3((unsigned)n, nb), dim3(CN_NT), 0, s, A, m, rs, cs, colnrm,
-               (unsigned*)nullptr, 0, (double2*)nullptr, true, bs);
+               (unsigned*)nullptr, 0, (double2*)nullptr, (double2*)nullptr, true, bs);
   for (int c0 = 0; c0 < n; c0 += NB) {
     const int nbp = std::min(NB, n - c0);
     double2* Tp = Tall + (int64_t)(c0 / NB) * NB * NB;
@@ -1784,9 +1794,11 @@ static QrLayout qr_layout(int64_t m, int64_t n, int num_cus) {
 
 size_t qr_workspace_bytes(int64_t m, int64_t n, int num_cus) { return qr_layout(m, n, num_cus).total; }
 
+// (plain contexts: the RK4 schedule lifts its third stream's GEMM cap where
+// the bulk sites' QR is this single launch on one XCD)
 bool qr_single_launch_applies(int64_t m, int64_t n, const Bat& bat, int chain) {
-  return (m > 768 && m <= 1280 && chain_fits(n, 256, bat, chain)) ||
-         (m > 1280 && m <= 2560 && chain_fits(n, 512, bat, chain));
+  return bat.nb == 1 && ((m > 768 && m <= 1280 && chain_fits(n, 256, bat, chain)) ||
+                         (m > 1280 && m <= 2560 && chain_fits(n, 512, bat, chain)));
 }
 
 void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double tol, void* ws, int num_cus,
@@ -1817,10 +1829,21 @@ void launch_qr(void* base, int64_t m, int64_t n, int64_t rs, int64_t cs, double 
   // panels keep the rows of a thread in registers up to m = 1280 (measured:
   // m = 1152 panel 3.6 us/column vs 5.7 with 16 columns on 6 waves; at m = 256
   // the 16-column panel wins, at m = 2304 two waves per SIMD beat AGPR storage)
-  if (m <= 256)
+  // (m <= 256: the single launch for pools of up to 64 copies; above that the
+  // per-panel launches' WY updates over every copy's columns take less time,
+  // measured: 256 copies of 64 x 32 in 205 us against 278.  The pool size,
+  // not the copies of this launch, decides, so a ragged pool's copy factors
+  // its panels the same way at every site)
+  if (m <= 256 && n > 16 && (bat.plan_nb > 0 ? bat.plan_nb : bat.nb) <= 64 && chain_fits(n, 256, bat, chain))
+    qr_chain<1, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
+  else if (m <= 256)
     qr_blocked<1, 16, 256, 4, 1, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
+  else if (m <= 512 && chain_fits(n, 256, bat, chain))
+    qr_chain<2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 512)
     qr_blocked<2, 8, 256, 8, 2, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
+  else if (m <= 768 && chain_fits(n, 256, bat, chain))
+    qr_chain<3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 768)
     qr_blocked<3, 8, 256, 12, 3, 256>(A, (int)m, (int)n, rs, cs, tol, q, R, s, bat);
   else if (m <= 1280 && chain_fits(n, 256, bat, chain))

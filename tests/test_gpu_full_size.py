@@ -104,13 +104,18 @@ def test_rydberg_density_matrix_N100_D128_trajectory_against_oracle_fixture():
     nrm = M.MPS.from_host(models.sum_local_vec_mps(N, models.HSS3), ctx)
     X = [M.MPS.from_host(fingerprint_mps(k), ctx) for k in range(3)]
     worst = {"trace": 0.0, "n_e": 0.0, "n_r": 0.0, "hs": 0.0, "fp": 0.0}
+    # <n_r> is real in exact arithmetic and tiny here (4e-11 .. 6e-6): the
+    # imaginary part of the oracle's value is its own rounding error, so the
+    # relative 1e-9 bound on it holds above that floor
+    nr_floor = 2.0 * float(np.abs(np.imag(f["n_r"])).max())
     for s, _ in enumerate(f["steps"]):
         M.RK4stp_apply_H_half_(dt, gL, rho, *r, *e, eop)
         tr = M.scal_prod_no_conj(idm, rho)
         got = {"trace": tr, "n_e": M.scal_prod_no_conj(nem, rho) / tr, "n_r": M.scal_prod_no_conj(nrm, rho) / tr,
                "hs": M.scal_prod(rho, rho)}
-        for k in ("n_e", "n_r", "hs"):
+        for k in ("n_e", "hs"):
             worst[k] = max(worst[k], abs(got[k] - f[k][s]) / abs(f[k][s]))
+        worst["n_r"] = max(worst["n_r"], max(abs(got["n_r"] - f["n_r"][s]) - nr_floor, 0.0) / abs(f["n_r"][s]))
         worst["trace"] = max(worst["trace"], abs(got["trace"] - f["trace"][s]))
         fp = np.array([M.scal_prod(x, rho) for x in X])
         worst["fp"] = max(worst["fp"], float(np.max(np.abs(fp - f["fp"][s]) / np.abs(f["fp"][s]))))

@@ -96,7 +96,9 @@ def test_qrq_full_rank(dev, mn, view, rng):
 
 
 @pytest.mark.parametrize("mnr", [(1152, 128, 128), (1000, 50, 50), (1152, 128, 20), (2304, 256, 256),
-                                 (2304, 256, 30), (1280, 256, 256)])
+                                 (2304, 256, 30), (1280, 256, 256), (512, 256, 256), (512, 256, 40),
+                                 (300, 77, 77), (768, 256, 256), (700, 100, 30), (288, 32, 32),
+                                 (256, 128, 128), (256, 128, 20), (200, 40, 40), (64, 32, 32), (64, 17, 5)])
 def test_qrq_single_launch_matches_per_panel(dev, mnr, rng):
     # the single-launch (chain) factorization and the launch-per-panel one
     # build the same zgeqr2 reflectors: Q and R agree to rounding, with and
