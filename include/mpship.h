@@ -102,9 +102,10 @@ int mps_ctx_synchronize(mps_ctx_t ctx);
 int mps_ctx_set_qr_rank_tol(mps_ctx_t ctx, double tol);
 /* Form of the QR behind qrq!/leftorth!/rightorth! (no counterpart in the
  * reference, which calls LAPACK): 1 = one persistent launch per factorization
- * where it applies (m <= 2560, n <= 256, plain contexts), 0 = a panel and a
- * trailing-update launch per 8 columns, -1 = the process default (1 unless
- * MPS_QR_CHAIN=0 is set).  Both give zgeqr2's reflectors (the same Q to
+ * where it applies (m <= 2560, n <= 256; on a batched context one launch for
+ * every copy; panels of m <= 256 only for n > 16 on up to 64 copies), 0 = a
+ * panel and a trailing-update launch per 8 or 16 columns, -1 = the process
+ * default (1 unless MPS_QR_CHAIN=0 is set).  Both give zgeqr2's reflectors (the same Q to
  * rounding); the setting is for callers that compare or time them. */
 int mps_ctx_set_qr_single_launch(mps_ctx_t ctx, int on);
 /* MPS_PREC_FP64 (default) or MPS_PREC_FP32 for the operator (MPO) contractions. */
