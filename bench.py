@@ -740,7 +740,10 @@ def main():
                                  f"{dl['algorithmic_bytes']:.4g} B; MFMA busy {dl['mfma_busy']}",
                 "kernel": ("zgemm_f64_kernel" if args.timing_prec == "fp64" else "zgemm_f32_3m_kernel")
                           + " (+ split-K reduce)",
-                "launches": st["gemm_launches"], "avg_launch_us": round(st["gemm_ms"] * 1e3 / max(st["gemm_launches"], 1), 2)}
+                "launches": st["gemm_launches"], "avg_launch_us": round(st["gemm_ms"] * 1e3 / max(st["gemm_launches"], 1), 2),
+                # the kernels execute the 3M product (6 real flops per complex MAC against the
+                # 8 counted in `achieved`): the MFMA units' share of their peak
+                "frac_3m": round(achieved * 0.75 / PEAK[args.timing_prec], 4)}
         if args.timing_prec != "fp64":   # the counter figures are the FP64 launch's
             roof["traffic"] = None
             roof["traffic_scope"] = "not measured for this precision"
