@@ -1689,11 +1689,13 @@ static bool chain_enabled() {
   return on;
 }
 
-// The chain kernel needs workgroup x of a launch on XCD x % 8 (so its grid of
-// 8 workgroups per panel puts one per panel on XCD 0).  qr_chain_probe runs
-// once per device, from context creation (never under graph capture), and
-// records whether 64 workgroups land that way; the chain path is used only
-// where they did.
+// The chain kernel needs the workgroups of a launch dealt to the 8 XCDs round
+// robin (so each block of 8 P workgroups puts exactly P on every XCD, one
+// factorization's panels per XCD).  qr_chain_probe runs once per device, from
+// context creation (never under graph capture), and records whether 64
+// workgroups land on XCD x % 8; the chain path is used only where they did.
+// (Were a launch still to miss an XCD's share, the panels no workgroup took
+// keep the NaN tau the column-norm kernel wrote, and Q shows NaN.)
 constexpr int PROBE_WG = 64;
 __global__ void xcc_probe_kernel(unsigned* out) {
   if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
