@@ -549,6 +549,54 @@ def qr_panels_leg(seed, m=1152, n=128, reps=60, ranks=(128, 60, 20), tol=1e-13):
     return out
 
 
+def qr_batched_leg(seed, cases=((2, 256, 16), (9, 128, 16), (2, 256, 64), (9, 32, 64)), reps=20, tol=1e-13):
+    """leftorth! (orthonormalisation.jl:164-171) of one site on every copy of
+    a lockstep batch at once (a pool's QR): nb copies of a (D, d, D) site,
+    panel D*d x D, full rank and of rank D/5 plus rounding-level noise under
+    the 1e-13 cutoff.  Each call is timed from the host around the call and a
+    synchronize; the context's QR form follows MPS_QR_CHAIN (single launch
+    unless it is 0)."""
+    import mpsamd as M
+    from mpsamd.trajectory import _state_bytes
+    rng = np.random.default_rng(seed)
+    out = {"reps": reps, "chain_path": os.environ.get("MPS_QR_CHAIN", "1") != "0", "cases": []}
+    for d, D, nb in cases:
+        dims = [D, D, D]
+        ctx = M.Context.batched(nb, _state_bytes(dims, d) + 4096, M.batch_workspace_bytes(nb, D, d, 1), 0)
+        comp = ctx.companion()
+        ctx.set_qr_rank_tol(tol)
+        g = M.MPS(2, d, dims, ctx)
+        views = [g.view(b, comp) for b in range(nb)]
+        row = {"d": d, "D": D, "copies": nb, "panel": [D * d, D]}
+        for label, r in (("full", D), ("rank_D/5", D // 5)):
+            sites = []
+            for _ in range(nb):
+                a = (rng.standard_normal((D * d, r)) + 1j * rng.standard_normal((D * d, r))) @ \
+                    (rng.standard_normal((r, D)) + 1j * rng.standard_normal((r, D)))
+                if r < D:
+                    a = a + 1e-17 * (rng.standard_normal((D * d, D)) + 1j * rng.standard_normal((D * d, D)))
+                s1 = rng.standard_normal((D, d, D)) + 1j * rng.standard_normal((D, d, D))
+                sites.append([a.reshape((D, d, D), order="F"), s1])
+            ts = []
+            for i in range(reps):
+                for v, st in zip(views, sites):
+                    v.upload(st)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                M.leftorth_(g, 0)
+                ctx.synchronize()
+                ts.append(time.perf_counter() - t0)
+            row[f"{label}_us"] = round(float(np.median(ts[3:])) * 1e6, 1)
+        out["cases"].append(row)
+        print(f"qr batched: {row}", file=sys.stderr, flush=True)
+        for v in views:
+            v.__del__()
+        g.__del__()
+        comp.close()
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -580,6 +628,9 @@ def main():
     ap.add_argument("--qr-panels", action="store_true",
                     help="only time leftorth! on 1152 x 128 panels of full and low rank (MPS_QR_CHAIN=0 in the "
                          "environment: the launch-per-panel factorization)")
+    ap.add_argument("--qr-batched", action="store_true",
+                    help="only time leftorth! on every copy of lockstep batches (MPS_QR_CHAIN=0 in the "
+                         "environment: the launch-per-panel form)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -597,6 +648,9 @@ def main():
         return
     if args.qr_panels:
         print(json.dumps({"qr_panels": qr_panels_leg(args.seed)}))
+        return
+    if args.qr_batched:
+        print(json.dumps({"qr_batched": qr_batched_leg(args.seed)}))
         return
     import mpsamd as M
     if args.three_forms_only:
