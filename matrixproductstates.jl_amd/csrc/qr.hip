@@ -1553,11 +1553,14 @@ __global__ __launch_bounds__(256) void qr_wsolve_kernel(const double2* __restric
       sG[e / NR][e % NR] = reg[t];
     }
   };
-  const int nch = (n + CH - 1) / CH;
-  fetch(nch - 1);
+  // W is upper triangular (w_r = 0 in column c for r > c: b_r = conj(v_r[c])
+  // is zero there, and a zero w_r changes no other row), so the chunks above
+  // this workgroup's last column are skipped; the result is the same
+  const int qtop = min(4 * (int)blockIdx.x + 3, n - 1) / CH;
+  fetch(qtop);
   put();
   __syncthreads();
-  for (int q = nch - 1; q >= 0; --q) {
+  for (int q = qtop; q >= 0; --q) {
     if (q > 0) fetch(q - 1);
     double2 gv[CH][NS], tv[CH];
 #pragma unroll
