@@ -193,3 +193,17 @@ def test_request_streams_mix_kinds_and_replay():
     shapes = {(r.kind, r.N, r.D, r.pos_seed) for r in a}
     assert shapes == {("spin", 8, 8, 1), ("rydberg", 5, 9, 2), ("spin", 10, 8, 3)}
     assert all(3 <= r.steps <= 7 for r in a)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU per run: bench.py --gpus N (N > 1) exits with a message before any device work
+    import json
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(repo, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
