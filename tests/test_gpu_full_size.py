@@ -162,3 +162,60 @@ def test_rydberg_density_matrix_N100_D128_200_steps_properties():
     pops = [M.scal_prod_no_conj(M.MPS.from_host(models.local_population_vec_mps(N, j, models.HEE3), ctx), rho).real
             for j in range(N)]
     assert min(pops) >= -1e-8 and max(pops) <= 1.0
+
+
+def test_rydberg_density_matrix_N100_D128_precision_forms():
+    # The precision form at the full size: 10 RK4 steps with the operator
+    # contractions in complex FP32, without and with 2 FP64 refinement
+    # half-sweeps per call, against the FP64 run from the same start.
+    # Populations within 1e-6, 1 - fidelity below 1e-12, and the FP32
+    # kernels visibly ran (the populations differ by more than FP64 rounding).
+    N, D, dt, n_exp, seed, steps = 100, 128, 0.005, 4, 1, 10
+    p = models.RydbergParams(n_exp=n_exp)
+    z = models.positions(N, float(N), seed)
+    wop = [1] + [p.w] * (N - 1) + [1]
+    ctx = M.Context(0)
+    ctx.set_qr_rank_tol(1e-13)
+    rho, dims = M.mpsgroundstate(N, D, 9, ctx)
+    rho0 = rho.to_host()
+    r = [rho.similar() for _ in range(3)]
+    e = [M.build_env(dims, dims, ctx=ctx) for _ in range(3)]
+    eop = M.build_env(dims, dims, wop, ctx=ctx)
+    gL = [M.MPO(h, ctx) for h in models.rydberg_rk4_mpos(p, z, dt)]
+    idm = M.MPS.from_host(models.identity_vec_mps(N), ctx)
+    nem = M.MPS.from_host(models.sum_local_vec_mps(N, models.HEE3), ctx)
+    nrm = M.MPS.from_host(models.sum_local_vec_mps(N, models.HSS3), ctx)
+
+    def run(prec, nref):
+        rho.upload(rho0)
+        for s in r:
+            s.copy_from(rho)
+        ctx.set_precision(prec)
+        ctx.set_refine_sweeps(nref)
+        try:
+            for _ in range(steps):
+                M.RK4stp_apply_H_half_(dt, gL, rho, *r, *e, eop)
+        finally:
+            ctx.set_precision("fp64")
+            ctx.set_refine_sweeps(0)
+        return rho.similar().copy_from(rho)
+
+    def pops(x):
+        tr = M.scal_prod_no_conj(idm, x)
+        return (M.scal_prod_no_conj(nem, x) / tr).real, (M.scal_prod_no_conj(nrm, x) / tr).real
+
+    def infidelity(a, b):
+        ab = abs(M.scal_prod(a, b))
+        return 1.0 - ab * ab / (abs(M.scal_prod(a, a)) * abs(M.scal_prod(b, b)))
+
+    ref = run("fp64", 0)
+    ne64, nr64 = pops(ref)
+    assert ne64 > 1e-4   # the steps did excite the chain
+    for nref in (0, 2):
+        got = run("fp32", nref)
+        ne, nr = pops(got)
+        inf = infidelity(got, ref)
+        print("refine", nref, "n_e", ne, ne64, "n_r", nr, nr64, "1 - fidelity", inf, flush=True)
+        assert abs(ne - ne64) < 1e-6 and abs(nr - nr64) < 1e-6
+        assert inf < 1e-12
+        assert abs(ne - ne64) > 1e-14
