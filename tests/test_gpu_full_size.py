@@ -164,13 +164,15 @@ def test_rydberg_density_matrix_N100_D128_200_steps_properties():
     assert min(pops) >= -1e-8 and max(pops) <= 1.0
 
 
-def test_rydberg_density_matrix_N100_D128_precision_forms():
-    # The precision form at the full size: 10 RK4 steps with the operator
-    # contractions in complex FP32, without and with 2 FP64 refinement
-    # half-sweeps per call, against the FP64 run from the same start.
-    # Populations within 1e-6, 1 - fidelity below 1e-12, and the FP32
+@pytest.mark.parametrize("D,steps", [(128, 10), (256, 4)])
+def test_rydberg_density_matrix_N100_precision_forms(D, steps):
+    # The precision form at the full size (D_max 128 and 256): RK4 steps with
+    # the operator contractions in complex FP32, without and with 2 FP64
+    # refinement half-sweeps per call, against the FP64 run from the same
+    # start.  Populations within 1e-6, 1 - fidelity below 1e-12, and the FP32
     # kernels visibly ran (the populations differ by more than FP64 rounding).
-    N, D, dt, n_exp, seed, steps = 100, 128, 0.005, 4, 1, 10
+    print("case D", D, "steps", steps, flush=True)
+    N, dt, n_exp, seed = 100, 0.005, 4, 1
     p = models.RydbergParams(n_exp=n_exp)
     z = models.positions(N, float(N), seed)
     wop = [1] + [p.w] * (N - 1) + [1]
