@@ -59,6 +59,33 @@ def test_ragged_pool_equals_uniform_pools_bit_for_bit(precision):
     rag.close()
 
 
+@pytest.mark.parametrize("D", [64, 128])
+def test_ragged_pool_equals_uniform_pools_at_benchmark_lengths(D):
+    # chains of 40-120 sites (the uneven batch's range) at the batch's bond
+    # dimensions, FP32 contractions and the batch leg's settings, in one
+    # ragged pool against the same trajectories in uniform pools of as many
+    # copies: bit for bit, as at the small sizes above
+    Ns, pos, seeds, steps = [120, 57, 83, 40], [11, 12, 13, 14], [51, 52, 53, 54], 2
+    kw = dict(dt=0.01, qr_rank_tol=1e-13)
+    print("case ragged vs uniform at benchmark lengths", D, Ns, flush=True)
+    rag = RaggedSpinEnsemble(Ns, D, pos, seeds, precision="fp32", **kw)
+    unis = []
+    for i, n in enumerate(Ns):
+        u = SpinEnsemble(n, D, pos[i], [seeds[i]] + [200 + k for k in range(len(Ns) - 1)], **kw)
+        u.ctx.set_precision("fp32")
+        unis.append(u)
+    for s in range(steps):
+        p = rag.step()
+        for i, u in enumerate(unis):
+            pu = u.step()
+            assert p[i] == pu[0], (s, i, p[i], pu[0])
+            assert rag.jumps[i] == u.jumps[0], (s, i)
+    for i, u in enumerate(unis):
+        assert np.array_equal(rag.sigma_z(i), u.sigma_z(0)), i
+        u.close()
+    rag.close()
+
+
 def test_ragged_pool_matches_single_trajectories():
     D, steps = 4, 12
     print("case ragged vs single", NS, flush=True)
