@@ -49,8 +49,8 @@ void Ctx::ensure_aux() {
   // panel, which needs free CUs on every XCD (measured at N = 100, D = 128,
   // d = 9 with that QR: 229.6 ms per RK4 step at 128 on 256 CUs, 241
   // uncapped, 238.7 at 96, 232.6 at 160); Plan::init lifts the cap where the
-  // sites' QR is the single launch (183.5 ms uncapped vs 200.0 at 128, driver's
-  // step count); MPS_AUX_MAX_WG overrides both (0: no cap)
+  // sites' QR is the single launch (183.5 ms uncapped vs 200.0 at 128, over 20
+  // steps after 5 warm-up steps); MPS_AUX_MAX_WG overrides both (0: no cap)
   aux_max_wg = std::max(1, num_cus / 2);
   if (const char* e = getenv("MPS_AUX_MAX_WG")) {
     aux_max_wg = std::max(0, atoi(e));
